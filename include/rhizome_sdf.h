@@ -1,0 +1,236 @@
+/*
+ * rhizome_sdf.h -- C-ABI of the MI355X (gfx950) mesh-to-SDF library.
+ *
+ * Plain C types only (pointers, sizes, POD structs); no HIP or torch types.
+ * Every pipeline-level entry point replaces one reference interface, cited in
+ * its comment (paths relative to the reference repository root).  Status
+ * codes: 0 = OK, >0 = declined (the caller falls back, as the reference's
+ * fast path does), <0 = error (rhz_last_error() has the message).
+ *
+ * Threading: calls on one device are serialised by the caller; each call
+ * uses the HIP stream it is given (NULL = the legacy default stream).
+ */
+#ifndef RHIZOME_SDF_H
+#define RHIZOME_SDF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RHZ_OK 0
+#define RHZ_DECLINED 1
+#define RHZ_ERR_ARG (-1)
+#define RHZ_ERR_HIP (-2)
+#define RHZ_ERR_NOMEM (-3)
+#define RHZ_ERR_NODEV (-4)
+
+/* 48-byte triangle; identical layout to the reference CudaTriangle
+ * (native/src/engine/sdf/cuda_types.hpp:18-21). */
+typedef struct rhz_triangle {
+    float v0[3];
+    float v1[3];
+    float v2[3];
+    float normal[3];
+} rhz_triangle;
+
+/* Which pipeline rhz_sdf_compute runs. */
+enum rhz_method {
+    RHZ_METHOD_AUTO = 0,         /* reference gate: FAST if V >= fast_min_voxels, else LEGACY;
+                                    a declined FAST falls through to LEGACY
+                                    (cuda_rasterizer.cu:1189-1195) */
+    RHZ_METHOD_FAST = 1,         /* reference fast path only (may decline) */
+    RHZ_METHOD_LEGACY = 2,       /* exact per-voxel distance + winding (cuda_rasterizer.cu:97-185) */
+    RHZ_METHOD_PLAIN_EXACT = 3,  /* exact winding sign for every voxel + band + FSM */
+    RHZ_METHOD_FAST_LEARNED = 4  /* fast path with risk-ranked exact re-probes */
+};
+
+/* Far-field sweep policy. */
+enum rhz_fsm_policy {
+    RHZ_FSM_REFERENCE = 0, /* <= max_rounds rounds (exit on no change), sentinel audit,
+                              <= max_extra_rounds audit-gated rounds, safe clamp
+                              (cuda_rasterizer.cu:1010-1063) */
+    RHZ_FSM_CONVERGE = 1   /* rounds until no voxel changes (cap fsm_round_cap) */
+};
+
+/* Winding-number arithmetic used by every exact-sign decision. */
+enum rhz_winding_precision {
+    RHZ_WINDING_FP32 = 0,        /* plain FP32 sum (reference arithmetic) */
+    RHZ_WINDING_FP32_KAHAN = 1   /* compensated FP32; FP64 recompute where |w-0.5| < fp64_band */
+};
+
+/* Constants of the path; rhz_sdf_params_default() returns the reference
+ * values (cuda_rasterizer.cu:74-82, :825, :867, :1010, :1027, :298-299). */
+typedef struct rhz_sdf_params {
+    int32_t struct_size;          /* sizeof(rhz_sdf_params) */
+    int32_t method;               /* enum rhz_method */
+    int64_t fast_min_voxels;      /* 32Mi */
+    int32_t band_voxels;          /* 8 */
+    int32_t parity_max_hits;      /* 128 */
+    int32_t verify_samples;       /* 4096 */
+    int32_t verify_gens;          /* 3 */
+    int32_t max_sites_per_probe;  /* 64 */
+    int32_t patch_radius;         /* 24 */
+    int32_t max_rounds;           /* 8 */
+    int32_t max_extra_rounds;     /* 16 */
+    int32_t sup;                  /* 4 (supercell edge in columns) */
+    int32_t fsm_policy;           /* enum rhz_fsm_policy */
+    int64_t csr_max_entries;      /* 1e8 */
+    float jitter_x;               /* 0.037 */
+    float jitter_y;               /* 0.051 */
+    int32_t clamp_intended_sign;  /* 1: clamped voxels ship as -band inside (the behaviour
+                                     the reference comment :536-543 states); 0: the
+                                     reference's double negation (:553 + :659) */
+    int32_t fsm_round_cap;        /* cap for RHZ_FSM_CONVERGE */
+    int32_t winding_precision;    /* enum rhz_winding_precision */
+    float fp64_band;              /* 1e-3 */
+    float reprobe_fraction;       /* FAST_LEARNED: fraction of voxels re-probed exactly */
+    int32_t learned_max_patch_gens; /* FAST_LEARNED: mismatch-region growth generations */
+    const void* mlp_weights;      /* FAST_LEARNED: packed proposer weights (NULL = built-in) */
+} rhz_sdf_params;
+
+/* Counters of one call; the fields of the reference's log line
+ * (cuda_rasterizer.cu:1145-1157) plus device stage times. */
+typedef struct rhz_sdf_stats {
+    int32_t status;
+    int32_t decline_reason;       /* 0 none, 1 int range, 2 CSR, 3 verify, 4 memory */
+    int32_t method_used;
+    int32_t unknown_flag;
+    int32_t dims[3];
+    int32_t eikonal_rounds;
+    int32_t extra_rounds;
+    int32_t sweeps;
+    int64_t n_triangles;
+    int64_t fixup_voxels;
+    int64_t far_residual;
+    int64_t safe_clamped;
+    int32_t probes;
+    int32_t sites;
+    int64_t patched_voxels;
+    int32_t patch_rounds;
+    int32_t last_probe_m;
+    int64_t csr_entries;
+    int64_t reprobed_voxels;      /* FAST_LEARNED / PLAIN_EXACT exact evaluations */
+    int64_t fp64_voxels;          /* voxels re-evaluated in FP64 */
+    double ms_total;              /* device time, triangles resident -> field resident */
+    double ms_splat;
+    double ms_parity;
+    double ms_fixup;
+    double ms_eikonal;
+    double ms_verify;
+    double ms_apply;
+    double ms_host_total;         /* host wall time of the call (the reference's out_time_ms) */
+    double ms_sweeps;             /* device time of the directional sweep kernels alone */
+    double ms_winding;            /* device time of the exact winding kernels alone */
+    int64_t winding_pairs;        /* (voxel, triangle) pairs those kernels evaluated */
+} rhz_sdf_stats;
+
+void rhz_sdf_params_default(rhz_sdf_params* p);
+const char* rhz_last_error(void);
+/* Number of visible devices (0 when no GPU). */
+int rhz_device_count(void);
+/* Frees the cached device workspace of the current device. */
+void rhz_release_workspace(void);
+
+/* Signed distance field of triangles on the lattice [min_c, max_c] (inclusive,
+ * x fastest, lattice point (min+i)*h), host buffers.
+ * Replaces CudaRasterizer::compute_mesh_sdf
+ * (native/src/engine/sdf/cuda_rasterizer.hpp:21-28, cuda_rasterizer.cu:1163-1265).
+ * out_host holds (max-min+1) product floats; it is written only on RHZ_OK. */
+int rhz_sdf_compute(const rhz_triangle* tris, int64_t n_tris, float h,
+                    const int32_t min_c[3], const int32_t max_c[3],
+                    const rhz_sdf_params* params, float* out_host,
+                    rhz_sdf_stats* stats);
+
+/* Same, with triangles and output already in device memory (d_out holds the
+ * field on return), launched on `stream` (hipStream_t).  Replaces the device
+ * part of rhi_compute_mesh_sdf_fast (cuda_rasterizer.cu:796-1159). */
+int rhz_sdf_compute_device(const rhz_triangle* d_tris, int64_t n_tris, float h,
+                           const int32_t min_c[3], const int32_t max_c[3],
+                           const rhz_sdf_params* params, float* d_out,
+                           void* stream, rhz_sdf_stats* stats);
+
+/* Indexed mesh -> triangles, dropping |(v1-v0)x(v2-v0)|^2 <= 1e-12 and
+ * out-of-range indices, in index order.  Replaces gpu_build_cuda_triangles_kernel
+ * + dev_try_build_cuda_triangle (cuda_rasterizer.cu:1321-1337,
+ * core/math_utils.cuh:484-514).  Host buffers; *n_out receives the count. */
+int rhz_build_triangles(const float* verts, int64_t n_verts, const uint32_t* idx,
+                        int64_t n_tris, rhz_triangle* out, int64_t* n_out);
+
+/* Vertex bounding box.  Replaces compute_bounding_box_gpu
+ * (engine/mesh/cuda_mesh_ops.cu:197-255). */
+int rhz_mesh_bbox(const float* verts, int64_t n_verts, float bmin[3], float bmax[3]);
+
+/* build_from_mesh grid framing: bbox padded by 10 h, min = floor, max = ceil
+ * (cuda_rasterizer.cu:1358-1372). */
+int rhz_frame_grid(const float* verts, int64_t n_verts, float h,
+                   int32_t min_c[3], int32_t max_c[3]);
+
+/* Generalised winding number of an indexed mesh at query points.  Replaces
+ * CudaRasterizer::compute_winding_numbers_gpu (cuda_rasterizer.hpp:280-285,
+ * engine/mesh/cuda_mesh_ops.cu:774-891). Host buffers. */
+int rhz_winding_numbers(const float* verts, int64_t n_verts, const uint32_t* idx,
+                        int64_t n_tris, const float* pts, int64_t n_pts, float* out_w);
+
+/* Red-black Gauss-Seidel eikonal on a signed grid, `iters` iterations (the
+ * reference uses 8).  Replaces CudaRasterizer::fast_sweeping_eikonal_gpu
+ * (cuda_rasterizer.hpp:288-291, cuda_rasterizer.cu:1536-1629).  Host grid. */
+int rhz_eikonal_redblack(float* grid, const int32_t dims[3], float h, int32_t iters);
+
+/* ---- stage-level entry points (device pointers, one stream) -------------
+ * Each runs one stage of the fast path on device buffers; the pipeline above
+ * is composed of exactly these.  Grids are x fastest with dims = max-min+1. */
+
+/* [init] d[i] = value (cuda_rasterizer.cu:194-203). */
+int rhz_stage_fill(float* d, int64_t n, float value, void* stream);
+/* [A] exact +-band splat by atomic min (cuda_rasterizer.cu:211-268). */
+int rhz_stage_splat(const rhz_triangle* d_tris, int64_t n_tris, float h, float band_width,
+                    const int32_t min_c[3], const int32_t dims[3], float* d, void* stream);
+/* [B] scanline parity signs (0 out, 1 in, 2 unknown) with the supercell
+ * candidate lists built on the device (cuda_rasterizer.cu:279-368, :824-882).
+ * *unknown_flag = 1 if any column was odd or overflowed; *csr_entries gets the
+ * candidate-list size.  Returns RHZ_DECLINED if the list exceeds csr_max_entries. */
+int rhz_stage_parity(const rhz_triangle* d_tris, int64_t n_tris, float h,
+                     float jitter_x, float jitter_y, int32_t max_hits, int32_t sup,
+                     int64_t csr_max_entries, const int32_t min_c[3], const int32_t dims[3],
+                     uint8_t* d_sign, int32_t* unknown_flag, int64_t* csr_entries,
+                     void* stream);
+/* [C] exact winding classification of every sign==2 voxel; *fixed = count
+ * (cuda_rasterizer.cu:374-400). */
+int rhz_stage_fixup(const rhz_triangle* d_tris, int64_t n_tris, float h,
+                    const int32_t min_c[3], const int32_t dims[3], int32_t precision,
+                    uint8_t* d_sign, int64_t* fixed, void* stream);
+/* [D] one directional sweep, pinned schedule: Gauss-Seidel along the swept
+ * line, cross-line neighbours read from src; writes all of dst
+ * (cuda_rasterizer.cu:405-513).  *changed |= 1 if any voxel decreased. */
+int rhz_stage_sweep(const float* src, float* dst, const int32_t dims[3], float h,
+                    int32_t axis, int32_t dir, int32_t* changed, void* stream);
+/* [D2] voxels >= 1e29 (cuda_rasterizer.cu:523-534). */
+int rhz_stage_count_far(const float* d, int64_t n, int64_t* count, void* stream);
+/* [D3] safe clamp (cuda_rasterizer.cu:544-556). */
+int rhz_stage_clamp_far(float* d, const uint8_t* sign, int64_t n, float band_width,
+                        int32_t intended_sign, void* stream);
+/* [E] one stratified probe generation: exact winding at the 4096-style
+ * sample voxels; mismatching voxel ids go to d_sites (capacity samples),
+ * *m = number of mismatches (cuda_rasterizer.cu:566-607). */
+int rhz_stage_verify(const rhz_triangle* d_tris, int64_t n_tris, float h,
+                     const int32_t min_c[3], const int32_t dims[3], int32_t samples,
+                     int32_t gen, int32_t precision, const uint8_t* d_sign,
+                     int64_t* d_sites, int32_t* m, void* stream);
+/* [E2] mark the (2r+1)^3 box around each site unknown (cuda_rasterizer.cu:615-646). */
+int rhz_stage_mark_patch(uint8_t* d_sign, const int64_t* d_sites, int32_t n_sites,
+                         int32_t radius, const int32_t dims[3], void* stream);
+/* [F] signed = inside ? -d : d, d > 1e29 -> 1e6 (cuda_rasterizer.cu:650-661). */
+int rhz_stage_apply(float* d, const uint8_t* sign, int64_t n, void* stream);
+/* Exact winding numbers (sum / 4 pi) at lattice voxels (device list). */
+int rhz_stage_winding_voxels(const rhz_triangle* d_tris, int64_t n_tris, float h,
+                             const int32_t min_c[3], const int32_t dims[3],
+                             const int64_t* d_voxels, int64_t n_voxels, int32_t precision,
+                             double* d_w, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,365 @@
+"""MI355X (gfx950) mesh-to-SDF library: ctypes binding of ``librhizome_sdf.so``.
+
+The product is the C-ABI library (``include/rhizome_sdf.h``) and its C++
+drop-in ``rhizome::CudaRasterizer`` (``include/rhizome/cuda_rasterizer.hpp``).
+This module exposes the same entry points to Python callers.  There is no CPU
+fallback: every call goes to the HIP library and raises if it is missing or
+no GPU is visible.
+
+Reference interfaces (golemid/rhizome_v24.15):
+  compute_sdf          CudaRasterizer::compute_mesh_sdf   native/src/engine/sdf/cuda_rasterizer.hpp:21-28
+  build_from_mesh      CudaRasterizer::build_from_mesh    cuda_rasterizer.hpp:31-35
+  winding_numbers      compute_winding_numbers_gpu        cuda_rasterizer.hpp:280-285
+  eikonal_redblack     fast_sweeping_eikonal_gpu          cuda_rasterizer.hpp:288-291
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librhizome_sdf.so")
+
+RHZ_OK = 0
+RHZ_DECLINED = 1
+
+METHOD_AUTO = 0
+METHOD_FAST = 1
+METHOD_LEGACY = 2
+METHOD_PLAIN_EXACT = 3
+METHOD_FAST_LEARNED = 4
+
+FSM_REFERENCE = 0
+FSM_CONVERGE = 1
+
+WINDING_FP32 = 0
+WINDING_FP32_KAHAN = 1
+
+
+class RhzError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_int32),
+        ("method", C.c_int32),
+        ("fast_min_voxels", C.c_int64),
+        ("band_voxels", C.c_int32),
+        ("parity_max_hits", C.c_int32),
+        ("verify_samples", C.c_int32),
+        ("verify_gens", C.c_int32),
+        ("max_sites_per_probe", C.c_int32),
+        ("patch_radius", C.c_int32),
+        ("max_rounds", C.c_int32),
+        ("max_extra_rounds", C.c_int32),
+        ("sup", C.c_int32),
+        ("fsm_policy", C.c_int32),
+        ("csr_max_entries", C.c_int64),
+        ("jitter_x", C.c_float),
+        ("jitter_y", C.c_float),
+        ("clamp_intended_sign", C.c_int32),
+        ("fsm_round_cap", C.c_int32),
+        ("winding_precision", C.c_int32),
+        ("fp64_band", C.c_float),
+        ("reprobe_fraction", C.c_float),
+        ("learned_max_patch_gens", C.c_int32),
+        ("mlp_weights", C.c_void_p),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32),
+        ("decline_reason", C.c_int32),
+        ("method_used", C.c_int32),
+        ("unknown_flag", C.c_int32),
+        ("dims", C.c_int32 * 3),
+        ("eikonal_rounds", C.c_int32),
+        ("extra_rounds", C.c_int32),
+        ("sweeps", C.c_int32),
+        ("n_triangles", C.c_int64),
+        ("fixup_voxels", C.c_int64),
+        ("far_residual", C.c_int64),
+        ("safe_clamped", C.c_int64),
+        ("probes", C.c_int32),
+        ("sites", C.c_int32),
+        ("patched_voxels", C.c_int64),
+        ("patch_rounds", C.c_int32),
+        ("last_probe_m", C.c_int32),
+        ("csr_entries", C.c_int64),
+        ("reprobed_voxels", C.c_int64),
+        ("fp64_voxels", C.c_int64),
+        ("ms_total", C.c_double),
+        ("ms_splat", C.c_double),
+        ("ms_parity", C.c_double),
+        ("ms_fixup", C.c_double),
+        ("ms_eikonal", C.c_double),
+        ("ms_verify", C.c_double),
+        ("ms_apply", C.c_double),
+        ("ms_host_total", C.c_double),
+        ("ms_sweeps", C.c_double),
+        ("ms_winding", C.c_double),
+        ("winding_pairs", C.c_int64),
+    ]
+
+    def as_dict(self):
+        d = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            d[k] = list(v) if k == "dims" else v
+        return d
+
+    def log_line(self) -> str:
+        """The reference's one-line summary (cuda_rasterizer.cu:1150-1156)."""
+        d = self.dims
+        return (f"Fast SDF path OK: {self.ms_total:.0f} ms total (splat+parity {self.ms_splat + self.ms_parity:.0f} ms"
+                f" | eikonal {self.eikonal_rounds} rounds {self.ms_eikonal:.0f} ms | verify+sign"
+                f" {self.ms_verify + self.ms_apply:.0f} ms) | {d[0]}x{d[1]}x{d[2]} voxels, {self.n_triangles} tris,"
+                f" fixup {self.fixup_voxels} | verify {self.probes} probes, {self.sites} sites,"
+                f" {self.patched_voxels} voxels patched in {self.patch_rounds} rounds | far residual"
+                f" {self.far_residual}, extra rounds {self.extra_rounds}, safe-clamped {self.safe_clamped}")
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def lib():
+    """Loads the HIP library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RhzError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                       "(hipcc --offload-arch=gfx950)")
+    L = C.CDLL(LIB_PATH)
+    vp, i64, i32, f32 = C.c_void_p, C.c_int64, C.c_int32, C.c_float
+    L.rhz_sdf_params_default.argtypes = [C.POINTER(Params)]
+    L.rhz_last_error.restype = C.c_char_p
+    L.rhz_device_count.restype = C.c_int
+    L.rhz_sdf_compute.argtypes = [vp, i64, f32, vp, vp, C.POINTER(Params), vp, C.POINTER(Stats)]
+    L.rhz_sdf_compute_device.argtypes = [vp, i64, f32, vp, vp, C.POINTER(Params), vp, vp, C.POINTER(Stats)]
+    L.rhz_build_triangles.argtypes = [vp, i64, vp, i64, vp, C.POINTER(C.c_int64)]
+    L.rhz_mesh_bbox.argtypes = [vp, i64, vp, vp]
+    L.rhz_frame_grid.argtypes = [vp, i64, f32, vp, vp]
+    L.rhz_winding_numbers.argtypes = [vp, i64, vp, i64, vp, i64, vp]
+    L.rhz_eikonal_redblack.argtypes = [vp, vp, f32, i32]
+    L.rhz_stage_fill.argtypes = [vp, i64, f32, vp]
+    L.rhz_stage_splat.argtypes = [vp, i64, f32, f32, vp, vp, vp, vp]
+    L.rhz_stage_parity.argtypes = [vp, i64, f32, f32, f32, i32, i32, i64, vp, vp, vp, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int64), vp]
+    L.rhz_stage_fixup.argtypes = [vp, i64, f32, vp, vp, i32, vp, C.POINTER(C.c_int64), vp]
+    L.rhz_stage_sweep.argtypes = [vp, vp, vp, f32, i32, i32, vp, vp]
+    L.rhz_stage_count_far.argtypes = [vp, i64, C.POINTER(C.c_int64), vp]
+    L.rhz_stage_clamp_far.argtypes = [vp, vp, i64, f32, i32, vp]
+    L.rhz_stage_verify.argtypes = [vp, i64, f32, vp, vp, i32, i32, i32, vp, vp, C.POINTER(C.c_int32), vp]
+    L.rhz_stage_mark_patch.argtypes = [vp, vp, i32, i32, vp, vp]
+    L.rhz_stage_apply.argtypes = [vp, vp, i64, vp]
+    L.rhz_stage_winding_voxels.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, vp, vp]
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "call") -> int:
+    if rc < 0:
+        msg = lib().rhz_last_error().decode(errors="replace")
+        raise RhzError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().rhz_sdf_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def _i32x3(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.int32).reshape(3))
+
+
+def _ptr(a) -> int:
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+def device_count() -> int:
+    return lib().rhz_device_count()
+
+
+# ----------------------------------------------------------------------------
+# pipeline-level API (host buffers)
+# ----------------------------------------------------------------------------
+def compute_sdf(tris, h, min_c, max_c, params: Params | None = None, **kw):
+    """Signed field of ``tris`` (float32[T, 12]) on the lattice [min_c, max_c].
+
+    Returns ``(rc, field, stats)``; ``field`` is float32[nz, ny, nx] on
+    RHZ_OK, None when the method declined."""
+    p = params if params is not None else default_params(**kw)
+    t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 12)
+    mn, mx = _i32x3(min_c), _i32x3(max_c)
+    dims = mx - mn + 1
+    out = np.empty(int(np.prod(dims.astype(np.int64))), dtype=np.float32)
+    st = Stats()
+    rc = check(lib().rhz_sdf_compute(t.ctypes.data, len(t), float(h), mn.ctypes.data, mx.ctypes.data, C.byref(p),
+                                     out.ctypes.data, C.byref(st)), "rhz_sdf_compute")
+    field = out.reshape(int(dims[2]), int(dims[1]), int(dims[0])) if rc == RHZ_OK else None
+    return rc, field, st
+
+
+def compute_sdf_device(d_tris, n_tris, h, min_c, max_c, d_out, params: Params | None = None, stream=None, **kw):
+    """Device-resident variant: ``d_tris``/``d_out`` are device pointers or
+    torch tensors on the GPU."""
+    p = params if params is not None else default_params(**kw)
+    mn, mx = _i32x3(min_c), _i32x3(max_c)
+    st = Stats()
+    s = stream.cuda_stream if stream is not None and hasattr(stream, "cuda_stream") else stream
+    rc = check(lib().rhz_sdf_compute_device(_ptr(d_tris), int(n_tris), float(h), mn.ctypes.data, mx.ctypes.data,
+                                            C.byref(p), _ptr(d_out), s, C.byref(st)), "rhz_sdf_compute_device")
+    return rc, st
+
+
+def build_triangles(verts, idx):
+    """Indexed mesh -> float32[T', 12] triangles (degenerates dropped, index order)."""
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
+    i = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    nt = len(i) // 3
+    out = np.empty((max(nt, 1), 12), dtype=np.float32)
+    k = C.c_int64(0)
+    check(lib().rhz_build_triangles(v.ctypes.data, len(v) // 3, i.ctypes.data, nt, out.ctypes.data, C.byref(k)),
+          "rhz_build_triangles")
+    return out[: k.value].copy()
+
+
+def mesh_bbox(verts):
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
+    lo = np.zeros(3, np.float32)
+    hi = np.zeros(3, np.float32)
+    check(lib().rhz_mesh_bbox(v.ctypes.data, len(v) // 3, lo.ctypes.data, hi.ctypes.data), "rhz_mesh_bbox")
+    return lo, hi
+
+
+def frame_grid(verts, h):
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
+    mn = np.zeros(3, np.int32)
+    mx = np.zeros(3, np.int32)
+    check(lib().rhz_frame_grid(v.ctypes.data, len(v) // 3, float(h), mn.ctypes.data, mx.ctypes.data),
+          "rhz_frame_grid")
+    return tuple(int(x) for x in mn), tuple(int(x) for x in mx)
+
+
+def build_from_mesh(verts, idx, h, params: Params | None = None, **kw):
+    """Reference build_from_mesh: bbox framing (pad 10 h), triangle build,
+    compute_sdf.  Returns ``(min_c, max_c, field, stats)`` or raises."""
+    mn, mx = frame_grid(verts, h)
+    tris = build_triangles(verts, idx)
+    if len(tris) == 0:
+        raise RhzError("mesh has no non-degenerate triangles")
+    rc, field, st = compute_sdf(tris, h, mn, mx, params, **kw)
+    if rc != RHZ_OK:
+        raise RhzError(f"compute_sdf declined (reason {st.decline_reason})")
+    return mn, mx, field, st
+
+
+def winding_numbers(verts, idx, pts):
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
+    i = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    p = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1)
+    out = np.empty(len(p) // 3, dtype=np.float32)
+    check(lib().rhz_winding_numbers(v.ctypes.data, len(v) // 3, i.ctypes.data, len(i) // 3, p.ctypes.data,
+                                    len(out), out.ctypes.data), "rhz_winding_numbers")
+    return out
+
+
+def eikonal_redblack(grid, h, iters=8):
+    g = np.ascontiguousarray(grid, dtype=np.float32).copy()
+    dims = np.array([g.shape[2], g.shape[1], g.shape[0]], dtype=np.int32)
+    check(lib().rhz_eikonal_redblack(g.ctypes.data, dims.ctypes.data, float(h), int(iters)), "rhz_eikonal_redblack")
+    return g
+
+
+# ----------------------------------------------------------------------------
+# stage-level API (device pointers or torch GPU tensors)
+# ----------------------------------------------------------------------------
+class Stages:
+    """Thin wrappers of the rhz_stage_* entry points."""
+
+    @staticmethod
+    def fill(d, n, value, stream=None):
+        check(lib().rhz_stage_fill(_ptr(d), int(n), float(value), stream), "rhz_stage_fill")
+
+    @staticmethod
+    def splat(d_tris, n_tris, h, band_width, min_c, dims, d, stream=None):
+        check(lib().rhz_stage_splat(_ptr(d_tris), int(n_tris), float(h), float(band_width),
+                                    _i32x3(min_c).ctypes.data, _i32x3(dims).ctypes.data, _ptr(d), stream),
+              "rhz_stage_splat")
+
+    @staticmethod
+    def parity(d_tris, n_tris, h, min_c, dims, d_sign, jitter=(0.037, 0.051), max_hits=128, sup=4,
+               csr_max_entries=100000000, stream=None):
+        flag = C.c_int32(0)
+        ent = C.c_int64(0)
+        rc = check(lib().rhz_stage_parity(_ptr(d_tris), int(n_tris), float(h), float(jitter[0]), float(jitter[1]),
+                                          int(max_hits), int(sup), int(csr_max_entries), _i32x3(min_c).ctypes.data,
+                                          _i32x3(dims).ctypes.data, _ptr(d_sign), C.byref(flag), C.byref(ent),
+                                          stream), "rhz_stage_parity")
+        return rc, flag.value, ent.value
+
+    @staticmethod
+    def fixup(d_tris, n_tris, h, min_c, dims, d_sign, precision=WINDING_FP32_KAHAN, stream=None):
+        k = C.c_int64(0)
+        check(lib().rhz_stage_fixup(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c).ctypes.data,
+                                    _i32x3(dims).ctypes.data, int(precision), _ptr(d_sign), C.byref(k), stream),
+              "rhz_stage_fixup")
+        return k.value
+
+    @staticmethod
+    def sweep(src, dst, dims, h, axis, direction, d_changed, stream=None):
+        check(lib().rhz_stage_sweep(_ptr(src), _ptr(dst), _i32x3(dims).ctypes.data, float(h), int(axis),
+                                    int(direction), _ptr(d_changed), stream), "rhz_stage_sweep")
+
+    @staticmethod
+    def count_far(d, n, stream=None):
+        k = C.c_int64(0)
+        check(lib().rhz_stage_count_far(_ptr(d), int(n), C.byref(k), stream), "rhz_stage_count_far")
+        return k.value
+
+    @staticmethod
+    def clamp_far(d, sign, n, band_width, intended=1, stream=None):
+        check(lib().rhz_stage_clamp_far(_ptr(d), _ptr(sign), int(n), float(band_width), int(intended), stream),
+              "rhz_stage_clamp_far")
+
+    @staticmethod
+    def verify(d_tris, n_tris, h, min_c, dims, samples, gen, d_sign, d_sites, precision=WINDING_FP32_KAHAN,
+               stream=None):
+        m = C.c_int32(0)
+        check(lib().rhz_stage_verify(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c).ctypes.data,
+                                     _i32x3(dims).ctypes.data, int(samples), int(gen), int(precision), _ptr(d_sign),
+                                     _ptr(d_sites), C.byref(m), stream), "rhz_stage_verify")
+        return m.value
+
+    @staticmethod
+    def mark_patch(d_sign, d_sites, n_sites, radius, dims, stream=None):
+        check(lib().rhz_stage_mark_patch(_ptr(d_sign), _ptr(d_sites), int(n_sites), int(radius),
+                                         _i32x3(dims).ctypes.data, stream), "rhz_stage_mark_patch")
+
+    @staticmethod
+    def apply(d, sign, n, stream=None):
+        check(lib().rhz_stage_apply(_ptr(d), _ptr(sign), int(n), stream), "rhz_stage_apply")
+
+    @staticmethod
+    def winding_voxels(d_tris, n_tris, h, min_c, dims, d_voxels, n_voxels, d_w, precision=WINDING_FP32_KAHAN,
+                       stream=None):
+        check(lib().rhz_stage_winding_voxels(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c).ctypes.data,
+                                             _i32x3(dims).ctypes.data, _ptr(d_voxels), int(n_voxels),
+                                             int(precision), _ptr(d_w), stream), "rhz_stage_winding_voxels")

@@ -1,0 +1,97 @@
+"""Compiles the gfx950 library ``librhizome_sdf.so`` in-tree with hipcc.
+
+Usage: ``python rhizome_v24.15_amd/build.py`` (or ``__graft_entry__.build()``).
+Each source compiles to an object under ``_build/`` in parallel, then one
+link step writes the shared library next to this file.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "librhizome_sdf.so")
+
+SOURCES = [
+    "k_grid.hip",
+    "k_splat.hip",
+    "k_parity.hip",
+    "k_fsm.hip",
+    "k_winding.hip",
+    "k_mesh.hip",
+    "k_learned.hip",
+    "sdf_pipeline.cpp",
+    "learned_path.cpp",
+    "cuda_rasterizer.cpp",
+]
+
+ARCH = os.environ.get("RHZ_OFFLOAD_ARCH", "gfx950")
+
+# -ffp-contract=off: the kernels compared bit-for-bit with the CPU oracle
+# (band distances, scanline hits, Godunov) must round every product;
+# kernels that want FMAs write __builtin_fmaf.  Packed FP32 ops are disabled.
+COMMON = [
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    f"--offload-arch={ARCH}",
+    "-ffp-contract=off",
+    "-fno-fast-math",
+    "-fno-gpu-flush-denormals-to-zero",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+    "-Wno-unused-variable",
+    "-Wno-unused-but-set-variable",
+    f"-I{os.path.join(REPO, 'include')}",
+    f"-I{CSRC}",
+]
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found: the gfx950 library cannot be built")
+
+
+def _compile(src: str) -> str:
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    cmd = [hipcc(), *COMMON, "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    # hipcc's host pass warns that it does not know -packed-fp32-ops; drop that line
+    noise = ("packed-fp32-ops", "not a recognized feature", "warning generated", "warnings generated")
+    out = "\n".join(l for l in (r.stderr or "").splitlines() if not any(n in l for n in noise)).strip()
+    if out:
+        print(out, file=sys.stderr)
+    return obj
+
+
+def build(verbose: bool = True) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")) or 1)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,106 @@
+// rhz_internal.hpp -- launchers of the gfx950 kernels, called by the C++ host
+// orchestration (sdf_pipeline.cpp).  Every launcher enqueues on `s` and
+// returns the launch status; none synchronises.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rhz {
+
+struct Grid {
+    int32_t mn[3];
+    int32_t dims[3];
+    float h;
+    int64_t n() const { return (int64_t)dims[0] * dims[1] * dims[2]; }
+};
+
+// ---- k_grid.hip -----------------------------------------------------------
+hipError_t launch_fill(float* d, int64_t n, float v, hipStream_t s);
+hipError_t launch_apply(float* d, const uint8_t* sign, int64_t n, hipStream_t s);
+hipError_t launch_clamp_far(float* d, const uint8_t* sign, int64_t n, float bw, int intended,
+                            hipStream_t s);
+// counter[0] += number of d[i] >= 1e29
+hipError_t launch_count_far(const float* d, int64_t n, unsigned long long* counter, hipStream_t s);
+// Appends i for every sign[i] == 2 to list (capacity cap); counter += count.
+hipError_t launch_compact_unknown(const uint8_t* sign, int64_t n, int64_t* list, int64_t cap,
+                                  unsigned long long* counter, hipStream_t s);
+hipError_t launch_mark_patch(uint8_t* sign, const int64_t* sites, int n_sites, int radius,
+                             const int32_t dims[3], hipStream_t s);
+hipError_t launch_verify_indices(int64_t total, int samples, int gen, int64_t* out, hipStream_t s);
+hipError_t launch_gather_u8(const uint8_t* src, const int64_t* idx, int64_t n, uint8_t* dst,
+                            hipStream_t s);
+hipError_t launch_scatter_decision(const double* w, const int64_t* idx, int64_t n, uint8_t* sign,
+                                   hipStream_t s);
+// sign[base + i] = |w[i]| >= 0.5
+hipError_t launch_range_decision(const double* w, int64_t base, int64_t n, uint8_t* sign, hipStream_t s);
+// Mismatch of sample decisions: for i < n with sign[idx[i]] != 2 and
+// (sign == 1) != (|w| >= 0.5), append idx[i] to sites; counter[0] += 1.
+hipError_t launch_compare_samples(const double* w, const int64_t* idx, int64_t n,
+                                  const uint8_t* sign, int64_t* sites, int64_t cap,
+                                  unsigned long long* counter, hipStream_t s);
+// Exclusive prefix sum of n uint32 counts into n+1 int64 starts; needs
+// scan_scratch_bytes(n) bytes of scratch.
+size_t scan_scratch_bytes(int64_t n);
+hipError_t launch_scan_u32(const uint32_t* in, int64_t n, int64_t* out, void* scratch,
+                           hipStream_t s);
+
+// ---- k_splat.hip ----------------------------------------------------------
+hipError_t launch_splat(const float* tris, int64_t nt, const Grid& g, float bw, float* d,
+                        hipStream_t s);
+
+// ---- k_parity.hip ---------------------------------------------------------
+struct CsrBuffers {
+    float4* xyab;       // [nt]
+    int4* cover;        // [nt] sx0, sx1, sy0, sy1
+    uint32_t* counts;   // [nsup]
+    int64_t* start;     // [nsup + 1]
+    uint32_t* cursor;   // [nsup]
+    int32_t* ids;       // [entries]
+};
+int64_t num_supercells(const Grid& g, int sup);
+hipError_t launch_csr_count(const float* tris, int64_t nt, const Grid& g, int sup,
+                            const CsrBuffers& b, unsigned long long* total, hipStream_t s);
+hipError_t launch_csr_fill(int64_t nt, const Grid& g, int sup, const CsrBuffers& b, hipStream_t s);
+hipError_t launch_parity(const float* tris, const Grid& g, int sup, const CsrBuffers& b,
+                         float jx, float jy, int max_hits, uint8_t* sign, int* flag,
+                         hipStream_t s);
+
+// ---- k_fsm.hip ------------------------------------------------------------
+hipError_t launch_sweep(const float* src, float* dst, const Grid& g, int axis, int dir,
+                        int* changed, hipStream_t s);
+hipError_t launch_redblack(float* g, const int32_t dims[3], float h, int parity, hipStream_t s);
+
+// ---- k_winding.hip --------------------------------------------------------
+// Winding numbers w = sum/(4 pi) at lattice voxels idx[0..n) (or base + i
+// when idx is null).  precision 0 = plain FP32 in triangle order,
+// 1 = compensated FP32 with a binary64 recompute where ||w| - 0.5| < fp64_band.
+// part: split partials scratch (winding_scratch_bytes), fp64_list/fp64_cnt:
+// recompute list (capacity n) and its counter.
+size_t winding_scratch_bytes(int64_t n_points, int64_t n_tris);
+hipError_t launch_winding(const float* tris, int64_t nt, const Grid& g, const int64_t* idx,
+                          int64_t base, int64_t n, int precision, float fp64_band, double* w,
+                          void* scratch, int64_t* fp64_list, unsigned long long* fp64_cnt,
+                          hipStream_t s);
+// Second half of launch_winding's binary64 step (needs the count on the host).
+hipError_t launch_winding_fp64(const float* tris, int64_t nt, const Grid& g, const int64_t* idx,
+                               int64_t base, const int64_t* fp64_list, int64_t n_fp64, double* w,
+                               hipStream_t s);
+// Legacy exact per-voxel kernel (distance + winding sign).
+hipError_t launch_legacy(const float* tris, int64_t nt, const Grid& g, float* out, void* scratch,
+                         hipStream_t s);
+size_t legacy_scratch_bytes(int64_t nt);
+// Standalone winding numbers over an indexed mesh.
+hipError_t launch_winding_points(const float* verts, const uint32_t* idx, int64_t nt,
+                                 const float* pts, int64_t np, float* out, hipStream_t s);
+
+// ---- k_mesh.hip -----------------------------------------------------------
+hipError_t launch_tri_valid(const float* verts, int64_t nv, const uint32_t* idx, int64_t nt,
+                            uint32_t* valid, hipStream_t s);
+hipError_t launch_tri_emit(const float* verts, const uint32_t* idx, int64_t nt,
+                           const uint32_t* valid, const int64_t* pos, float* out, hipStream_t s);
+hipError_t launch_bbox(const float* verts, int64_t nv, float* partial, int nblocks, float* out6,
+                       hipStream_t s);
+int bbox_blocks(int64_t nv);
+
+}  // namespace rhz

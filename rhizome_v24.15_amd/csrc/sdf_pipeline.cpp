@@ -1,0 +1,1051 @@
+// sdf_pipeline.cpp -- C++ host orchestration of the mesh-to-SDF path and the
+// C-ABI (include/rhizome_sdf.h).
+//
+// The policy is the reference rhi_compute_mesh_sdf_fast
+// (native/src/engine/sdf/cuda_rasterizer.cu:796-1159) and compute_mesh_sdf
+// (:1163-1265): guards, candidate lists, fill, splat, parity, exact fixup, FSM
+// rounds with sentinel audit / audit-gated extension / safe clamp, stratified
+// exact-winding verification with patch boxes, sign apply, and the 32Mi gate
+// with the legacy fallback.  Differences by design:
+//   * device memory comes from a per-device grow-only workspace instead of
+//     eight cudaMalloc/cudaFree per call;
+//   * candidate lists are built on the device;
+//   * every stage is enqueued on one stream; the host reads back only the
+//     small counters the policy branches on (as the reference does);
+//   * the FSM uses a fixed sweep schedule (see k_fsm.hip) and two buffers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rhizome_sdf.h"
+#include "rhz_internal.hpp"
+
+namespace rhz {
+
+namespace {
+
+constexpr float kFar = 1e30f;  // unreached marker (cuda_rasterizer.cu:77)
+
+thread_local std::string g_err;
+
+int set_err(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            return set_err(RHZ_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e));       \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// per-device workspace
+// ---------------------------------------------------------------------------
+enum Slot {
+    S_DIST2 = 0,   // second FSM buffer
+    S_SIGN,        // sign volume
+    S_XYAB,
+    S_COVER,
+    S_COUNTS,
+    S_START,
+    S_CURSOR,
+    S_IDS,
+    S_SCAN,
+    S_LIST,        // voxel index list
+    S_W,           // winding numbers (double)
+    S_WSCR,        // winding split partials
+    S_FP64,        // binary64 recompute list
+    S_SITES,
+    S_SAMPLES,
+    S_TRIS,        // triangles staged by the host API
+    S_OUT,         // field staged by the host API
+    S_MISC,
+    S_MISC2,
+    S_COUNT
+};
+
+constexpr int kNumCounters = 32;
+
+struct Workspace {
+    void* p[S_COUNT] = {};
+    size_t bytes[S_COUNT] = {};
+    unsigned long long* d_cnt = nullptr;  // device counters (ull[16] + int[32])
+    unsigned long long* h_cnt = nullptr;  // pinned mirror
+    bool init = false;
+};
+
+std::mutex g_ws_mu;
+Workspace g_ws[64];
+
+int current_device() {
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess) return -1;
+    return d;
+}
+
+Workspace* workspace() {
+    int d = current_device();
+    if (d < 0 || d >= 64) return nullptr;
+    Workspace& w = g_ws[d];
+    if (!w.init) {
+        if (hipMalloc(&w.d_cnt, 512) != hipSuccess) return nullptr;
+        if (hipHostMalloc((void**)&w.h_cnt, 512, hipHostMallocDefault) != hipSuccess) return nullptr;
+        w.init = true;
+    }
+    return &w;
+}
+
+// Returns a device buffer of at least `bytes` for `slot` (grow-only).
+void* ws_get(Workspace* w, int slot, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (w->bytes[slot] >= bytes) return w->p[slot];
+    if (w->p[slot]) {
+        (void)hipFree(w->p[slot]);
+        w->p[slot] = nullptr;
+        w->bytes[slot] = 0;
+    }
+    size_t rounded = (bytes + 4095) & ~size_t(4095);
+    if (hipMalloc(&w->p[slot], rounded) != hipSuccess) {
+        (void)hipGetLastError();
+        w->p[slot] = nullptr;
+        return nullptr;
+    }
+    w->bytes[slot] = rounded;
+    return w->p[slot];
+}
+
+// counters: ull slots 0..15, int slots at byte offset 128 (32 ints)
+enum CounterId { C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4 };
+enum FlagId { F_UNKNOWN = 0, F_CHANGED = 1 };
+
+unsigned long long* cnt_ptr(Workspace* w, int id) { return w->d_cnt + id; }
+int* flag_ptr(Workspace* w, int id) { return reinterpret_cast<int*>(reinterpret_cast<char*>(w->d_cnt) + 128) + id; }
+
+struct Timer {
+    hipEvent_t ev[8] = {};
+    int n = 0;
+    bool ok = false;
+    explicit Timer(bool enable) {
+        if (!enable) return;
+        ok = true;
+        for (int i = 0; i < 8; ++i)
+            if (hipEventCreate(&ev[i]) != hipSuccess) ok = false;
+    }
+    ~Timer() {
+        for (int i = 0; i < 8; ++i)
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
+    }
+    void mark(int i, hipStream_t s) {
+        if (ok) (void)hipEventRecord(ev[i], s);
+    }
+    double ms(int a, int b) {
+        float t = 0.f;
+        if (!ok || hipEventElapsedTime(&t, ev[a], ev[b]) != hipSuccess) return 0.0;
+        return t;
+    }
+};
+
+// Accumulates device time of bracketed launch sequences; events are read
+// once, after the caller's final synchronisation.
+struct EventAcc {
+    std::vector<hipEvent_t> evs;
+    size_t used = 0;
+    ~EventAcc() {
+        for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    }
+    hipEvent_t next() {
+        if (used == evs.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            evs.push_back(e);
+        }
+        return evs[used++];
+    }
+    void begin(hipStream_t s) {
+        hipEvent_t e = next();
+        if (e) (void)hipEventRecord(e, s);
+    }
+    void end(hipStream_t s) { begin(s); }
+    double total_ms() {
+        double t = 0.0;
+        for (size_t i = 0; i + 1 < used; i += 2) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, evs[i], evs[i + 1]) == hipSuccess) t += ms;
+        }
+        return t;
+    }
+};
+
+struct Ctx {
+    const float* tris;
+    int64_t nt;
+    Grid g;
+    const rhz_sdf_params* p;
+    float* d;  // field buffer (device), V floats
+    hipStream_t s;
+    rhz_sdf_stats* st;
+    Workspace* w;
+    EventAcc* sweeps_t = nullptr;
+    EventAcc* wind_t = nullptr;
+};
+
+// Reads `count` ull counters starting at `first` into h_cnt (synchronises).
+int read_counters(Ctx& c) {
+    HIP_TRY(hipMemcpyAsync(c.w->h_cnt, c.w->d_cnt, 512, hipMemcpyDeviceToHost, c.s));
+    HIP_TRY(hipStreamSynchronize(c.s));
+    return RHZ_OK;
+}
+unsigned long long hc(Ctx& c, int id) { return c.w->h_cnt[id]; }
+int hf(Ctx& c, int id) { return reinterpret_cast<int*>(reinterpret_cast<char*>(c.w->h_cnt) + 128)[id]; }
+
+int zero_counters(Ctx& c) {
+    HIP_TRY(hipMemsetAsync(c.w->d_cnt, 0, 512, c.s));
+    return RHZ_OK;
+}
+
+// Exact winding decisions for `n` voxels (explicit list or base + i), written
+// as signs 0/1 into `sign`.  Processes in chunks so scratch stays bounded.
+int exact_classify(Ctx& c, const int64_t* list, int64_t base, int64_t n, uint8_t* sign, int64_t* fp64_total) {
+    const int prec = c.p->winding_precision;
+    const int64_t chunk = std::min<int64_t>(n, 8ll << 20);
+    for (int64_t off = 0; off < n; off += chunk) {
+        int64_t m = std::min(chunk, n - off);
+        double* w = (double*)ws_get(c.w, S_W, sizeof(double) * (size_t)m);
+        void* scr = ws_get(c.w, S_WSCR, winding_scratch_bytes(m, c.nt));
+        int64_t* f64 = (int64_t*)ws_get(c.w, S_FP64, sizeof(int64_t) * (size_t)m);
+        if (!w || !scr || !f64) return set_err(RHZ_ERR_NOMEM, "winding scratch allocation failed");
+        const int64_t* idx = list ? list + off : nullptr;
+        int64_t b = list ? 0 : base + off;
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FP64), 0, sizeof(unsigned long long), c.s));
+        if (c.wind_t) c.wind_t->begin(c.s);
+        HIP_TRY(launch_winding(c.tris, c.nt, c.g, idx, b, m, prec, c.p->fp64_band, w, scr, f64,
+                               cnt_ptr(c.w, C_FP64), c.s));
+        if (c.wind_t) c.wind_t->end(c.s);
+        c.st->winding_pairs += m * c.nt;
+        if (prec != 0) {
+            int rc = read_counters(c);
+            if (rc) return rc;
+            int64_t nf = (int64_t)hc(c, C_FP64);
+            if (fp64_total) *fp64_total += nf;
+            HIP_TRY(launch_winding_fp64(c.tris, c.nt, c.g, idx, b, f64, nf, w, c.s));
+        }
+        if (idx) {
+            HIP_TRY(launch_scatter_decision(w, idx, m, sign, c.s));
+        } else {
+            HIP_TRY(launch_range_decision(w, b, m, sign, c.s));
+        }
+    }
+    return RHZ_OK;
+}
+
+// [C] exact fixup of every sign == 2 voxel; returns the count in *fixed.
+int fixup_unknown(Ctx& c, uint8_t* sign, int64_t* fixed) {
+    const int64_t V = c.g.n();
+    int64_t cap = (int64_t)(c.w->bytes[S_LIST] / sizeof(int64_t));
+    if (cap < (1 << 20)) cap = 1 << 20;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        int64_t* list = (int64_t*)ws_get(c.w, S_LIST, sizeof(int64_t) * (size_t)cap);
+        if (!list) return set_err(RHZ_ERR_NOMEM, "unknown-voxel list allocation failed");
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_LIST), 0, sizeof(unsigned long long), c.s));
+        HIP_TRY(launch_compact_unknown(sign, V, list, cap, cnt_ptr(c.w, C_LIST), c.s));
+        int rc = read_counters(c);
+        if (rc) return rc;
+        int64_t n = (int64_t)hc(c, C_LIST);
+        if (n <= cap) {
+            *fixed = n;
+            if (n == 0) return RHZ_OK;
+            return exact_classify(c, list, 0, n, sign, &c.st->fp64_voxels);
+        }
+        cap = n;
+    }
+    return set_err(RHZ_ERR_HIP, "unknown-voxel compaction did not converge");
+}
+
+int sweep_round(Ctx& c, float* a, float* b, int* changed) {
+    // x+, x-, y+, y-, z+, z- (cuda_rasterizer.cu:1012-1017); six sweeps
+    // alternate the two buffers, so the round ends back in `a`.
+    const int axes[6] = {0, 0, 1, 1, 2, 2};
+    const int dirs[6] = {+1, -1, +1, -1, +1, -1};
+    float* src = a;
+    float* dst = b;
+    if (c.sweeps_t) c.sweeps_t->begin(c.s);
+    for (int k = 0; k < 6; ++k) {
+        HIP_TRY(launch_sweep(src, dst, c.g, axes[k], dirs[k], changed, c.s));
+        std::swap(src, dst);
+    }
+    if (c.sweeps_t) c.sweeps_t->end(c.s);
+    c.st->sweeps += 6;
+    return RHZ_OK;
+}
+
+int audit(Ctx& c, int64_t* far) {
+    HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FAR), 0, sizeof(unsigned long long), c.s));
+    HIP_TRY(launch_count_far(c.d, c.g.n(), cnt_ptr(c.w, C_FAR), c.s));
+    int rc = read_counters(c);
+    if (rc) return rc;
+    *far = (int64_t)hc(c, C_FAR);
+    return RHZ_OK;
+}
+
+// [D] far-field re-distancing with the reference round policy.
+int run_fsm(Ctx& c, const uint8_t* sign) {
+    const int64_t V = c.g.n();
+    float* b = (float*)ws_get(c.w, S_DIST2, sizeof(float) * (size_t)V);
+    if (!b) return set_err(RHZ_ERR_NOMEM, "second sweep buffer allocation failed");
+    int* changed = flag_ptr(c.w, F_CHANGED);
+    const bool converge = c.p->fsm_policy == RHZ_FSM_CONVERGE;
+    const int cap = converge ? c.p->fsm_round_cap : c.p->max_rounds;
+    int rounds = 0;
+    for (int r = 0; r < cap; ++r) {
+        HIP_TRY(hipMemsetAsync(changed, 0, sizeof(int), c.s));
+        int rc = sweep_round(c, c.d, b, changed);
+        if (rc) return rc;
+        rc = read_counters(c);
+        if (rc) return rc;
+        rounds = r + 1;
+        if (hf(c, F_CHANGED) == 0) break;
+    }
+    int64_t far = 0;
+    int rc = audit(c, &far);
+    if (rc) return rc;
+    int extra = 0;
+    while (far > 0 && extra < c.p->max_extra_rounds) {
+        rc = sweep_round(c, c.d, b, changed);
+        if (rc) return rc;
+        ++extra;
+        rc = audit(c, &far);
+        if (rc) return rc;
+    }
+    c.st->eikonal_rounds = rounds + extra;
+    c.st->extra_rounds = extra;
+    c.st->safe_clamped = 0;
+    if (far > 0) {
+        c.st->safe_clamped = far;
+        HIP_TRY(launch_clamp_far(c.d, sign, V, (float)c.p->band_voxels * c.g.h, c.p->clamp_intended_sign, c.s));
+        rc = audit(c, &far);
+        if (rc) return rc;
+    }
+    c.st->far_residual = far;
+    return RHZ_OK;
+}
+
+bool int_range_ok(const Grid& g, int64_t nt) {
+    const int64_t dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
+    // cuda_rasterizer.cu:815-820
+    if (g.n() > 2000000000LL) return false;
+    if (nt * dz > 2000000000LL) return false;
+    if (dx * dy > 2000000000LL || dx * dz > 2000000000LL || dy * dz > 2000000000LL) return false;
+    return true;
+}
+
+// [B] candidate lists + parity; returns RHZ_DECLINED on a pathological list.
+int run_parity(Ctx& c, uint8_t* sign, int* unknown_flag) {
+    const rhz_sdf_params* p = c.p;
+    const int sup = p->sup;
+    const int64_t nsup = num_supercells(c.g, sup);
+    CsrBuffers b;
+    b.xyab = (float4*)ws_get(c.w, S_XYAB, sizeof(float4) * (size_t)c.nt);
+    b.cover = (int4*)ws_get(c.w, S_COVER, sizeof(int4) * (size_t)c.nt);
+    b.counts = (uint32_t*)ws_get(c.w, S_COUNTS, sizeof(uint32_t) * (size_t)nsup);
+    b.start = (int64_t*)ws_get(c.w, S_START, sizeof(int64_t) * (size_t)(nsup + 1));
+    b.cursor = (uint32_t*)ws_get(c.w, S_CURSOR, sizeof(uint32_t) * (size_t)nsup);
+    void* scan = ws_get(c.w, S_SCAN, scan_scratch_bytes(nsup));
+    if (!b.xyab || !b.cover || !b.counts || !b.start || !b.cursor || !scan)
+        return set_err(RHZ_ERR_NOMEM, "candidate-list allocation failed");
+    HIP_TRY(launch_csr_count(c.tris, c.nt, c.g, sup, b, cnt_ptr(c.w, C_CSR), c.s));
+    int rc = read_counters(c);
+    if (rc) return rc;
+    int64_t entries = (int64_t)hc(c, C_CSR);
+    c.st->csr_entries = entries;
+    if (entries <= 0 || entries > p->csr_max_entries) {
+        c.st->decline_reason = 2;
+        return RHZ_DECLINED;  // pathological triangle spread (:867-869)
+    }
+    b.ids = (int32_t*)ws_get(c.w, S_IDS, sizeof(int32_t) * (size_t)entries);
+    if (!b.ids) return set_err(RHZ_ERR_NOMEM, "candidate-list allocation failed");
+    HIP_TRY(launch_scan_u32(b.counts, nsup, b.start, scan, c.s));
+    HIP_TRY(launch_csr_fill(c.nt, c.g, sup, b, c.s));
+    HIP_TRY(launch_parity(c.tris, c.g, sup, b, p->jitter_x, p->jitter_y, p->parity_max_hits, sign,
+                          flag_ptr(c.w, F_UNKNOWN), c.s));
+    rc = read_counters(c);
+    if (rc) return rc;
+    *unknown_flag = hf(c, F_UNKNOWN);
+    return RHZ_OK;
+}
+
+// [E] stratified exact verification with patch boxes.  Returns RHZ_OK when a
+// probe comes back clean, RHZ_DECLINED otherwise.
+int run_verify(Ctx& c, uint8_t* sign) {
+    const rhz_sdf_params* p = c.p;
+    const int N = p->verify_samples;
+    const int64_t V = c.g.n();
+    int64_t* idx = (int64_t*)ws_get(c.w, S_SAMPLES, sizeof(int64_t) * (size_t)N);
+    int64_t* sites = (int64_t*)ws_get(c.w, S_SITES, sizeof(int64_t) * (size_t)N);
+    double* w = (double*)ws_get(c.w, S_MISC, sizeof(double) * (size_t)N);
+    void* scr = ws_get(c.w, S_WSCR, winding_scratch_bytes(N, c.nt));
+    int64_t* f64 = (int64_t*)ws_get(c.w, S_FP64, sizeof(int64_t) * (size_t)N);
+    if (!idx || !sites || !w || !scr || !f64) return set_err(RHZ_ERR_NOMEM, "verify scratch allocation failed");
+    bool ok = false;
+    for (int gen = 0; gen < p->verify_gens; ++gen) {
+        HIP_TRY(launch_verify_indices(V, N, gen, idx, c.s));
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FP64), 0, 2 * sizeof(unsigned long long), c.s));
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_MISM), 0, sizeof(unsigned long long), c.s));
+        if (c.wind_t) c.wind_t->begin(c.s);
+        HIP_TRY(launch_winding(c.tris, c.nt, c.g, idx, 0, N, p->winding_precision, p->fp64_band, w, scr, f64,
+                               cnt_ptr(c.w, C_FP64), c.s));
+        if (c.wind_t) c.wind_t->end(c.s);
+        c.st->winding_pairs += (int64_t)N * c.nt;
+        if (p->winding_precision != 0) {
+            int rc = read_counters(c);
+            if (rc) return rc;
+            int64_t nf = (int64_t)hc(c, C_FP64);
+            c.st->fp64_voxels += nf;
+            HIP_TRY(launch_winding_fp64(c.tris, c.nt, c.g, idx, 0, f64, nf, w, c.s));
+        }
+        HIP_TRY(launch_compare_samples(w, idx, N, sign, sites, N, cnt_ptr(c.w, C_MISM), c.s));
+        int rc = read_counters(c);
+        if (rc) return rc;
+        int m = (int)std::min<unsigned long long>(hc(c, C_MISM), (unsigned long long)N);
+        c.st->probes++;
+        c.st->last_probe_m = m;
+        if (m == 0) {
+            ok = true;
+            break;
+        }
+        if (m > p->max_sites_per_probe) break;  // systematic divergence
+        if (gen == p->verify_gens - 1) break;   // persistent after the patches
+        HIP_TRY(launch_mark_patch(sign, sites, m, p->patch_radius, c.g.dims, c.s));
+        int64_t fixed = 0;
+        rc = fixup_unknown(c, sign, &fixed);
+        if (rc) return rc;
+        c.st->patched_voxels += fixed;
+        c.st->sites += m;
+        c.st->patch_rounds++;
+    }
+    if (!ok) {
+        c.st->decline_reason = 3;
+        return RHZ_DECLINED;
+    }
+    return RHZ_OK;
+}
+
+// Reference fast path on device buffers.
+int run_fast(Ctx& c) {
+    const rhz_sdf_params* p = c.p;
+    rhz_sdf_stats* st = c.st;
+    const int64_t V = c.g.n();
+    if (!int_range_ok(c.g, c.nt) || c.nt <= 0) {
+        st->decline_reason = 1;
+        return RHZ_DECLINED;
+    }
+    uint8_t* sign = (uint8_t*)ws_get(c.w, S_SIGN, (size_t)V);
+    if (!sign) {
+        st->decline_reason = 4;
+        return RHZ_DECLINED;  // VRAM allocation failed (:914-918)
+    }
+    Timer tm(true);
+    tm.mark(0, c.s);
+    int rc = zero_counters(c);
+    if (rc) return rc;
+    HIP_TRY(launch_fill(c.d, V, kFar, c.s));
+    HIP_TRY(launch_splat(c.tris, c.nt, c.g, (float)p->band_voxels * c.g.h, c.d, c.s));
+    tm.mark(1, c.s);
+    int unknown = 0;
+    rc = run_parity(c, sign, &unknown);
+    if (rc) return rc;
+    tm.mark(2, c.s);
+    st->unknown_flag = unknown;
+    if (unknown) {
+        int64_t fixed = 0;
+        rc = fixup_unknown(c, sign, &fixed);
+        if (rc) return rc;
+        st->fixup_voxels = fixed;
+    }
+    tm.mark(3, c.s);
+    rc = run_fsm(c, sign);
+    if (rc) return rc;
+    tm.mark(4, c.s);
+    rc = run_verify(c, sign);
+    if (rc) return rc;
+    tm.mark(5, c.s);
+    HIP_TRY(launch_apply(c.d, sign, V, c.s));
+    tm.mark(6, c.s);
+    HIP_TRY(hipStreamSynchronize(c.s));
+    st->ms_splat = tm.ms(0, 1);
+    st->ms_parity = tm.ms(1, 2);
+    st->ms_fixup = tm.ms(2, 3);
+    st->ms_eikonal = tm.ms(3, 4);
+    st->ms_verify = tm.ms(4, 5);
+    st->ms_apply = tm.ms(5, 6);
+    st->ms_total = tm.ms(0, 6);
+    return RHZ_OK;
+}
+
+int run_legacy(Ctx& c) {
+    void* bb = ws_get(c.w, S_MISC, legacy_scratch_bytes(c.nt));
+    if (!bb) return set_err(RHZ_ERR_NOMEM, "legacy scratch allocation failed");
+    Timer tm(true);
+    tm.mark(0, c.s);
+    HIP_TRY(launch_legacy(c.tris, c.nt, c.g, c.d, bb, c.s));
+    tm.mark(1, c.s);
+    HIP_TRY(hipStreamSynchronize(c.s));
+    c.st->ms_total = tm.ms(0, 1);
+    return RHZ_OK;
+}
+
+// Exact winding sign for every voxel, exact band, FSM, apply.
+int run_plain_exact(Ctx& c) {
+    const rhz_sdf_params* p = c.p;
+    const int64_t V = c.g.n();
+    uint8_t* sign = (uint8_t*)ws_get(c.w, S_SIGN, (size_t)V);
+    if (!sign) return set_err(RHZ_ERR_NOMEM, "sign volume allocation failed");
+    Timer tm(true);
+    tm.mark(0, c.s);
+    int rc = zero_counters(c);
+    if (rc) return rc;
+    HIP_TRY(launch_fill(c.d, V, kFar, c.s));
+    HIP_TRY(launch_splat(c.tris, c.nt, c.g, (float)p->band_voxels * c.g.h, c.d, c.s));
+    tm.mark(1, c.s);
+    rc = exact_classify(c, nullptr, 0, V, sign, &c.st->fp64_voxels);
+    if (rc) return rc;
+    c.st->reprobed_voxels = V;
+    tm.mark(2, c.s);
+    tm.mark(3, c.s);
+    rc = run_fsm(c, sign);
+    if (rc) return rc;
+    tm.mark(4, c.s);
+    tm.mark(5, c.s);
+    HIP_TRY(launch_apply(c.d, sign, V, c.s));
+    tm.mark(6, c.s);
+    HIP_TRY(hipStreamSynchronize(c.s));
+    c.st->ms_splat = tm.ms(0, 1);
+    c.st->ms_parity = tm.ms(1, 2);
+    c.st->ms_eikonal = tm.ms(3, 4);
+    c.st->ms_apply = tm.ms(5, 6);
+    c.st->ms_total = tm.ms(0, 6);
+    return RHZ_OK;
+}
+
+int check_params(const rhz_sdf_params* p) {
+    if (!p || p->struct_size != (int32_t)sizeof(rhz_sdf_params))
+        return set_err(RHZ_ERR_ARG, "rhz_sdf_params.struct_size mismatch (rebuild against rhizome_sdf.h)");
+    if (p->parity_max_hits < 1 || p->parity_max_hits > 128)
+        return set_err(RHZ_ERR_ARG, "parity_max_hits must be in [1, 128]");
+    if (p->sup < 1 || p->band_voxels < 0 || p->verify_samples < 1 || p->verify_gens < 1 || p->verify_gens > 3)
+        return set_err(RHZ_ERR_ARG, "invalid policy constants");
+    return RHZ_OK;
+}
+
+int dispatch_inner(Ctx& c);
+
+int dispatch(Ctx& c) {
+    EventAcc sw, wd;
+    c.sweeps_t = &sw;
+    c.wind_t = &wd;
+    int rc = dispatch_inner(c);
+    if (rc >= 0 && hipStreamSynchronize(c.s) == hipSuccess) {
+        c.st->ms_sweeps = sw.total_ms();
+        c.st->ms_winding = wd.total_ms();
+    }
+    c.sweeps_t = nullptr;
+    c.wind_t = nullptr;
+    return rc;
+}
+
+int dispatch_inner(Ctx& c) {
+    const rhz_sdf_params* p = c.p;
+    const int64_t V = c.g.n();
+    int method = p->method;
+    if (method == RHZ_METHOD_AUTO) {
+        if (V >= p->fast_min_voxels) {
+            int rc = run_fast(c);
+            if (rc == RHZ_OK) {
+                c.st->method_used = RHZ_METHOD_FAST;
+                return RHZ_OK;
+            }
+            if (rc < 0) return rc;
+            // declined: fall through to the exact kernel (:1189-1195)
+        }
+        c.st->method_used = RHZ_METHOD_LEGACY;
+        return run_legacy(c);
+    }
+    c.st->method_used = method;
+    switch (method) {
+        case RHZ_METHOD_FAST:
+            return run_fast(c);
+        case RHZ_METHOD_LEGACY:
+            return run_legacy(c);
+        case RHZ_METHOD_PLAIN_EXACT:
+            return run_plain_exact(c);
+        default:
+            return set_err(RHZ_ERR_ARG, "method %d is not available", method);
+    }
+}
+
+int make_grid(const int32_t min_c[3], const int32_t max_c[3], float h, Grid* g) {
+    if (!min_c || !max_c) return set_err(RHZ_ERR_ARG, "null lattice bounds");
+    if (!(h > 0.0f) || !std::isfinite(h)) return set_err(RHZ_ERR_ARG, "voxel size must be positive");
+    for (int j = 0; j < 3; ++j) {
+        int64_t d = (int64_t)max_c[j] - (int64_t)min_c[j] + 1;
+        if (d <= 0 || d > 2000000000LL) return set_err(RHZ_ERR_ARG, "empty or oversized lattice on axis %d", j);
+        g->mn[j] = min_c[j];
+        g->dims[j] = (int32_t)d;
+    }
+    g->h = h;
+    if (g->n() > 2000000000LL) return set_err(RHZ_ERR_ARG, "lattice exceeds 2e9 voxels");
+    return RHZ_OK;
+}
+
+int need_device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return set_err(RHZ_ERR_NODEV, "no HIP device available");
+    }
+    return RHZ_OK;
+}
+
+}  // namespace
+}  // namespace rhz
+
+using namespace rhz;
+
+extern "C" {
+
+void rhz_sdf_params_default(rhz_sdf_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->struct_size = (int32_t)sizeof(rhz_sdf_params);
+    p->method = RHZ_METHOD_AUTO;
+    p->fast_min_voxels = 32LL * 1024LL * 1024LL;
+    p->band_voxels = 8;
+    p->parity_max_hits = 128;
+    p->verify_samples = 4096;
+    p->verify_gens = 3;
+    p->max_sites_per_probe = 64;
+    p->patch_radius = 24;
+    p->max_rounds = 8;
+    p->max_extra_rounds = 16;
+    p->sup = 4;
+    p->fsm_policy = RHZ_FSM_REFERENCE;
+    p->csr_max_entries = 100000000LL;
+    p->jitter_x = 0.037f;
+    p->jitter_y = 0.051f;
+    p->clamp_intended_sign = 1;
+    p->fsm_round_cap = 512;
+    p->winding_precision = RHZ_WINDING_FP32_KAHAN;
+    p->fp64_band = 1e-3f;
+    p->reprobe_fraction = 1e-3f;
+    p->learned_max_patch_gens = 64;
+    p->mlp_weights = nullptr;
+}
+
+const char* rhz_last_error(void) { return g_err.c_str(); }
+
+int rhz_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+void rhz_release_workspace(void) {
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return;
+    for (int i = 0; i < S_COUNT; ++i) {
+        if (w->p[i]) (void)hipFree(w->p[i]);
+        w->p[i] = nullptr;
+        w->bytes[i] = 0;
+    }
+}
+
+int rhz_sdf_compute_device(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],
+                           const int32_t max_c[3], const rhz_sdf_params* params, float* d_out, void* stream,
+                           rhz_sdf_stats* stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = need_device();
+    if (rc) return rc;
+    rc = check_params(params);
+    if (rc) return rc;
+    Grid g;
+    rc = make_grid(min_c, max_c, h, &g);
+    if (rc) return rc;
+    if (n_tris < 0 || (n_tris > 0 && !d_tris) || !d_out) return set_err(RHZ_ERR_ARG, "null buffer");
+    rhz_sdf_stats local;
+    rhz_sdf_stats* st = stats ? stats : &local;
+    std::memset(st, 0, sizeof(*st));
+    for (int j = 0; j < 3; ++j) st->dims[j] = g.dims[j];
+    st->n_triangles = n_tris;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    Ctx c{reinterpret_cast<const float*>(d_tris), n_tris, g, params, d_out, (hipStream_t)stream, st, w};
+    if (n_tris == 0) return set_err(RHZ_ERR_ARG, "no triangles");
+    rc = dispatch(c);
+    st->status = rc;
+    st->ms_host_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+int rhz_sdf_compute(const rhz_triangle* tris, int64_t n_tris, float h, const int32_t min_c[3],
+                    const int32_t max_c[3], const rhz_sdf_params* params, float* out_host, rhz_sdf_stats* stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = need_device();
+    if (rc) return rc;
+    rc = check_params(params);
+    if (rc) return rc;
+    Grid g;
+    rc = make_grid(min_c, max_c, h, &g);
+    if (rc) return rc;
+    if (n_tris <= 0 || !tris || !out_host) return set_err(RHZ_ERR_ARG, "null or empty buffer");
+    rhz_sdf_stats local;
+    rhz_sdf_stats* st = stats ? stats : &local;
+    float* d_tris = nullptr;
+    float* d_out = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        Workspace* w = workspace();
+        if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+        d_tris = (float*)ws_get(w, S_TRIS, sizeof(rhz_triangle) * (size_t)n_tris);
+        d_out = (float*)ws_get(w, S_OUT, sizeof(float) * (size_t)g.n());
+        if (!d_tris || !d_out) return set_err(RHZ_ERR_NOMEM, "device buffer allocation failed");
+    }
+    HIP_TRY(hipMemcpy(d_tris, tris, sizeof(rhz_triangle) * (size_t)n_tris, hipMemcpyHostToDevice));
+    rc = rhz_sdf_compute_device(reinterpret_cast<const rhz_triangle*>(d_tris), n_tris, h, min_c, max_c, params,
+                                d_out, nullptr, st);
+    if (rc == RHZ_OK) HIP_TRY(hipMemcpy(out_host, d_out, sizeof(float) * (size_t)g.n(), hipMemcpyDeviceToHost));
+    st->ms_host_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+int rhz_build_triangles(const float* verts, int64_t n_verts, const uint32_t* idx, int64_t n_tris, rhz_triangle* out,
+                        int64_t* n_out) {
+    int rc = need_device();
+    if (rc) return rc;
+    if (!verts || !idx || !out || !n_out || n_verts < 0 || n_tris < 0) return set_err(RHZ_ERR_ARG, "null buffer");
+    *n_out = 0;
+    if (n_tris == 0) return RHZ_OK;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    float* dv = (float*)ws_get(w, S_MISC, sizeof(float) * 3 * (size_t)std::max<int64_t>(n_verts, 1));
+    uint32_t* di = (uint32_t*)ws_get(w, S_MISC2, sizeof(uint32_t) * 3 * (size_t)n_tris);
+    uint32_t* valid = (uint32_t*)ws_get(w, S_COUNTS, sizeof(uint32_t) * (size_t)n_tris);
+    int64_t* pos = (int64_t*)ws_get(w, S_START, sizeof(int64_t) * (size_t)(n_tris + 1));
+    void* scan = ws_get(w, S_SCAN, scan_scratch_bytes(n_tris));
+    float* dt = (float*)ws_get(w, S_TRIS, sizeof(rhz_triangle) * (size_t)n_tris);
+    if (!dv || !di || !valid || !pos || !scan || !dt) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+    HIP_TRY(hipMemcpy(dv, verts, sizeof(float) * 3 * (size_t)n_verts, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(di, idx, sizeof(uint32_t) * 3 * (size_t)n_tris, hipMemcpyHostToDevice));
+    HIP_TRY(launch_tri_valid(dv, n_verts, di, n_tris, valid, nullptr));
+    HIP_TRY(launch_scan_u32(valid, n_tris, pos, scan, nullptr));
+    HIP_TRY(launch_tri_emit(dv, di, n_tris, valid, pos, dt, nullptr));
+    int64_t k = 0;
+    HIP_TRY(hipMemcpy(&k, pos + n_tris, sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (k > 0) HIP_TRY(hipMemcpy(out, dt, sizeof(rhz_triangle) * (size_t)k, hipMemcpyDeviceToHost));
+    *n_out = k;
+    return RHZ_OK;
+}
+
+int rhz_mesh_bbox(const float* verts, int64_t n_verts, float bmin[3], float bmax[3]) {
+    int rc = need_device();
+    if (rc) return rc;
+    if (!verts || !bmin || !bmax || n_verts < 0) return set_err(RHZ_ERR_ARG, "null buffer");
+    if (n_verts == 0) {
+        for (int j = 0; j < 3; ++j) bmin[j] = bmax[j] = 0.0f;  // cuda_mesh_ops.cu:206-210
+        return RHZ_OK;
+    }
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    int nb = bbox_blocks(n_verts);
+    float* dv = (float*)ws_get(w, S_MISC, sizeof(float) * 3 * (size_t)n_verts);
+    float* part = (float*)ws_get(w, S_MISC2, sizeof(float) * 6 * (size_t)nb + 64);
+    if (!dv || !part) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+    HIP_TRY(hipMemcpy(dv, verts, sizeof(float) * 3 * (size_t)n_verts, hipMemcpyHostToDevice));
+    HIP_TRY(launch_bbox(dv, n_verts, part, nb, part + 6 * nb, nullptr));
+    float r[6];
+    HIP_TRY(hipMemcpy(r, part + 6 * nb, sizeof(r), hipMemcpyDeviceToHost));
+    for (int j = 0; j < 3; ++j) {
+        bmin[j] = r[j];
+        bmax[j] = r[3 + j];
+    }
+    return RHZ_OK;
+}
+
+int rhz_frame_grid(const float* verts, int64_t n_verts, float h, int32_t min_c[3], int32_t max_c[3]) {
+    if (!(h > 0.0f)) return set_err(RHZ_ERR_ARG, "voxel size must be positive");
+    float bmin[3], bmax[3];
+    int rc = rhz_mesh_bbox(verts, n_verts, bmin, bmax);
+    if (rc) return rc;
+    float pad = h * 10.0f;
+    for (int j = 0; j < 3; ++j) {
+        float lo = bmin[j] - pad, hi = bmax[j] + pad;
+        min_c[j] = (int32_t)std::floor(lo / h);
+        max_c[j] = (int32_t)std::ceil(hi / h);
+    }
+    return RHZ_OK;
+}
+
+int rhz_winding_numbers(const float* verts, int64_t n_verts, const uint32_t* idx, int64_t n_tris, const float* pts,
+                        int64_t n_pts, float* out_w) {
+    int rc = need_device();
+    if (rc) return rc;
+    if (n_pts < 0 || n_tris < 0 || (n_pts > 0 && (!pts || !out_w))) return set_err(RHZ_ERR_ARG, "null buffer");
+    if (n_pts == 0) return RHZ_OK;
+    if (n_tris == 0) {
+        for (int64_t i = 0; i < n_pts; ++i) out_w[i] = 0.0f;  // cuda_mesh_ops.cu:834-838
+        return RHZ_OK;
+    }
+    for (int64_t t = 0; t < 3 * n_tris; ++t)
+        if ((int64_t)idx[t] >= n_verts) return set_err(RHZ_ERR_ARG, "index out of range");
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    float* dv = (float*)ws_get(w, S_MISC, sizeof(float) * 3 * (size_t)n_verts);
+    uint32_t* di = (uint32_t*)ws_get(w, S_MISC2, sizeof(uint32_t) * 3 * (size_t)n_tris);
+    float* dp = (float*)ws_get(w, S_TRIS, sizeof(float) * 3 * (size_t)n_pts);
+    float* dw = (float*)ws_get(w, S_OUT, sizeof(float) * (size_t)n_pts);
+    if (!dv || !di || !dp || !dw) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+    HIP_TRY(hipMemcpy(dv, verts, sizeof(float) * 3 * (size_t)n_verts, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(di, idx, sizeof(uint32_t) * 3 * (size_t)n_tris, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dp, pts, sizeof(float) * 3 * (size_t)n_pts, hipMemcpyHostToDevice));
+    HIP_TRY(launch_winding_points(dv, di, n_tris, dp, n_pts, dw, nullptr));
+    HIP_TRY(hipMemcpy(out_w, dw, sizeof(float) * (size_t)n_pts, hipMemcpyDeviceToHost));
+    return RHZ_OK;
+}
+
+int rhz_eikonal_redblack(float* grid, const int32_t dims[3], float h, int32_t iters) {
+    int rc = need_device();
+    if (rc) return rc;
+    if (!grid || !dims || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0 || iters < 0)
+        return set_err(RHZ_ERR_ARG, "invalid grid");
+    int64_t n = (int64_t)dims[0] * dims[1] * dims[2];
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    float* dg = (float*)ws_get(w, S_OUT, sizeof(float) * (size_t)n);
+    if (!dg) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+    HIP_TRY(hipMemcpy(dg, grid, sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
+    for (int it = 0; it < iters; ++it) {
+        HIP_TRY(launch_redblack(dg, dims, h, 0, nullptr));
+        HIP_TRY(launch_redblack(dg, dims, h, 1, nullptr));
+    }
+    HIP_TRY(hipMemcpy(grid, dg, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    return RHZ_OK;
+}
+
+// ---- stage-level entry points -------------------------------------------------
+int rhz_stage_fill(float* d, int64_t n, float value, void* stream) {
+    HIP_TRY(launch_fill(d, n, value, (hipStream_t)stream));
+    return RHZ_OK;
+}
+
+int rhz_stage_splat(const rhz_triangle* d_tris, int64_t n_tris, float h, float band_width, const int32_t min_c[3],
+                    const int32_t dims[3], float* d, void* stream) {
+    Grid g;
+    for (int j = 0; j < 3; ++j) {
+        g.mn[j] = min_c[j];
+        g.dims[j] = dims[j];
+    }
+    g.h = h;
+    HIP_TRY(launch_splat(reinterpret_cast<const float*>(d_tris), n_tris, g, band_width, d, (hipStream_t)stream));
+    return RHZ_OK;
+}
+
+int rhz_stage_parity(const rhz_triangle* d_tris, int64_t n_tris, float h, float jitter_x, float jitter_y,
+                     int32_t max_hits, int32_t sup, int64_t csr_max_entries, const int32_t min_c[3],
+                     const int32_t dims[3], uint8_t* d_sign, int32_t* unknown_flag, int64_t* csr_entries,
+                     void* stream) {
+    rhz_sdf_params p;
+    rhz_sdf_params_default(&p);
+    p.jitter_x = jitter_x;
+    p.jitter_y = jitter_y;
+    p.parity_max_hits = max_hits;
+    p.sup = sup;
+    p.csr_max_entries = csr_max_entries;
+    if (max_hits < 1 || max_hits > 128 || sup < 1) return set_err(RHZ_ERR_ARG, "invalid parity constants");
+    rhz_sdf_stats st;
+    std::memset(&st, 0, sizeof(st));
+    Grid g;
+    for (int j = 0; j < 3; ++j) {
+        g.mn[j] = min_c[j];
+        g.dims[j] = dims[j];
+    }
+    g.h = h;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    Ctx c{reinterpret_cast<const float*>(d_tris), n_tris, g, &p, nullptr, (hipStream_t)stream, &st, w};
+    int rc = zero_counters(c);
+    if (rc) return rc;
+    int flag = 0;
+    rc = run_parity(c, d_sign, &flag);
+    if (unknown_flag) *unknown_flag = flag;
+    if (csr_entries) *csr_entries = st.csr_entries;
+    return rc;
+}
+
+int rhz_stage_fixup(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],
+                    const int32_t dims[3], int32_t precision, uint8_t* d_sign, int64_t* fixed, void* stream) {
+    rhz_sdf_params p;
+    rhz_sdf_params_default(&p);
+    p.winding_precision = precision;
+    rhz_sdf_stats st;
+    std::memset(&st, 0, sizeof(st));
+    Grid g;
+    for (int j = 0; j < 3; ++j) {
+        g.mn[j] = min_c[j];
+        g.dims[j] = dims[j];
+    }
+    g.h = h;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    Ctx c{reinterpret_cast<const float*>(d_tris), n_tris, g, &p, nullptr, (hipStream_t)stream, &st, w};
+    int64_t k = 0;
+    int rc = fixup_unknown(c, d_sign, &k);
+    if (fixed) *fixed = k;
+    if (rc == RHZ_OK) HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return rc;
+}
+
+int rhz_stage_sweep(const float* src, float* dst, const int32_t dims[3], float h, int32_t axis, int32_t dir,
+                    int32_t* changed, void* stream) {
+    if (axis < 0 || axis > 2 || (dir != 1 && dir != -1)) return set_err(RHZ_ERR_ARG, "invalid sweep");
+    Grid g;
+    for (int j = 0; j < 3; ++j) {
+        g.mn[j] = 0;
+        g.dims[j] = dims[j];
+    }
+    g.h = h;
+    HIP_TRY(launch_sweep(src, dst, g, axis, dir, changed, (hipStream_t)stream));
+    return RHZ_OK;
+}
+
+int rhz_stage_count_far(const float* d, int64_t n, int64_t* count, void* stream) {
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipMemsetAsync(w->d_cnt + C_FAR, 0, sizeof(unsigned long long), s));
+    HIP_TRY(launch_count_far(d, n, w->d_cnt + C_FAR, s));
+    unsigned long long v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, w->d_cnt + C_FAR, sizeof(v), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (count) *count = (int64_t)v;
+    return RHZ_OK;
+}
+
+int rhz_stage_clamp_far(float* d, const uint8_t* sign, int64_t n, float band_width, int32_t intended_sign,
+                        void* stream) {
+    HIP_TRY(launch_clamp_far(d, sign, n, band_width, intended_sign, (hipStream_t)stream));
+    return RHZ_OK;
+}
+
+int rhz_stage_verify(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],
+                     const int32_t dims[3], int32_t samples, int32_t gen, int32_t precision, const uint8_t* d_sign,
+                     int64_t* d_sites, int32_t* m, void* stream) {
+    if (samples < 1 || gen < 0 || gen > 2) return set_err(RHZ_ERR_ARG, "invalid probe");
+    rhz_sdf_params p;
+    rhz_sdf_params_default(&p);
+    p.winding_precision = precision;
+    rhz_sdf_stats st;
+    std::memset(&st, 0, sizeof(st));
+    Grid g;
+    for (int j = 0; j < 3; ++j) {
+        g.mn[j] = min_c[j];
+        g.dims[j] = dims[j];
+    }
+    g.h = h;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    Ctx c{reinterpret_cast<const float*>(d_tris), n_tris, g, &p, nullptr, (hipStream_t)stream, &st, w};
+    int64_t* idx = (int64_t*)ws_get(w, S_SAMPLES, sizeof(int64_t) * (size_t)samples);
+    double* wv = (double*)ws_get(w, S_MISC, sizeof(double) * (size_t)samples);
+    void* scr = ws_get(w, S_WSCR, winding_scratch_bytes(samples, n_tris));
+    int64_t* f64 = (int64_t*)ws_get(w, S_FP64, sizeof(int64_t) * (size_t)samples);
+    if (!idx || !wv || !scr || !f64) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+    int rc = zero_counters(c);
+    if (rc) return rc;
+    HIP_TRY(launch_verify_indices(g.n(), samples, gen, idx, c.s));
+    HIP_TRY(launch_winding(c.tris, n_tris, g, idx, 0, samples, precision, p.fp64_band, wv, scr, f64,
+                           cnt_ptr(w, C_FP64), c.s));
+    if (precision != 0) {
+        rc = read_counters(c);
+        if (rc) return rc;
+        HIP_TRY(launch_winding_fp64(c.tris, n_tris, g, idx, 0, f64, (int64_t)hc(c, C_FP64), wv, c.s));
+    }
+    HIP_TRY(launch_compare_samples(wv, idx, samples, d_sign, d_sites, samples, cnt_ptr(w, C_MISM), c.s));
+    rc = read_counters(c);
+    if (rc) return rc;
+    if (m) *m = (int32_t)hc(c, C_MISM);
+    return RHZ_OK;
+}
+
+int rhz_stage_mark_patch(uint8_t* d_sign, const int64_t* d_sites, int32_t n_sites, int32_t radius,
+                         const int32_t dims[3], void* stream) {
+    HIP_TRY(launch_mark_patch(d_sign, d_sites, n_sites, radius, dims, (hipStream_t)stream));
+    return RHZ_OK;
+}
+
+int rhz_stage_apply(float* d, const uint8_t* sign, int64_t n, void* stream) {
+    HIP_TRY(launch_apply(d, sign, n, (hipStream_t)stream));
+    return RHZ_OK;
+}
+
+int rhz_stage_winding_voxels(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],
+                             const int32_t dims[3], const int64_t* d_voxels, int64_t n_voxels, int32_t precision,
+                             double* d_w, void* stream) {
+    rhz_sdf_params p;
+    rhz_sdf_params_default(&p);
+    p.winding_precision = precision;
+    rhz_sdf_stats st;
+    std::memset(&st, 0, sizeof(st));
+    Grid g;
+    for (int j = 0; j < 3; ++j) {
+        g.mn[j] = min_c[j];
+        g.dims[j] = dims[j];
+    }
+    g.h = h;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    Ctx c{reinterpret_cast<const float*>(d_tris), n_tris, g, &p, nullptr, (hipStream_t)stream, &st, w};
+    void* scr = ws_get(w, S_WSCR, winding_scratch_bytes(n_voxels, n_tris));
+    int64_t* f64 = (int64_t*)ws_get(w, S_FP64, sizeof(int64_t) * (size_t)std::max<int64_t>(n_voxels, 1));
+    if (!scr || !f64) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+    int rc = zero_counters(c);
+    if (rc) return rc;
+    HIP_TRY(launch_winding(c.tris, n_tris, g, d_voxels, 0, n_voxels, precision, p.fp64_band, d_w, scr, f64,
+                           cnt_ptr(w, C_FP64), c.s));
+    if (precision != 0) {
+        rc = read_counters(c);
+        if (rc) return rc;
+        HIP_TRY(launch_winding_fp64(c.tris, n_tris, g, d_voxels, 0, f64, (int64_t)hc(c, C_FP64), d_w, c.s));
+    }
+    HIP_TRY(hipStreamSynchronize(c.s));
+    return RHZ_OK;
+}
+
+}  // extern "C"
