@@ -57,20 +57,20 @@ def test_fast_path_config1(rhz, orc, config1):
 
 
 def test_safe_clamp_path(rhz, orc, small_sphere):
-    """One round and no extension rounds leave sentinels: the safe clamp runs
+    """No rounds and no extension rounds leave sentinels: the safe clamp runs
     and ships -band inside / +band outside."""
     tris, h, mn, mx = small_sphere
     for intended in (1, 0):
         st, ost, got, ref = _compare_fast(rhz, orc, tris, h, mn, mx,
-                                          {"max_rounds": 1, "max_extra_rounds": 0, "clamp_intended_sign": intended},
-                                          {"max_rounds": 1, "max_extra_rounds": 0, "clamp_intended_sign": intended})
+                                          {"max_rounds": 0, "max_extra_rounds": 0, "clamp_intended_sign": intended},
+                                          {"max_rounds": 0, "max_extra_rounds": 0, "clamp_intended_sign": intended})
         assert st.safe_clamped > 0 and st.far_residual == 0
 
 
 def test_extension_rounds(rhz, orc, small_sphere):
     tris, h, mn, mx = small_sphere
-    st, ost, got, ref = _compare_fast(rhz, orc, tris, h, mn, mx, {"max_rounds": 1}, {"max_rounds": 1})
-    assert st.extra_rounds >= 1 and st.far_residual == 0
+    st, ost, got, ref = _compare_fast(rhz, orc, tris, h, mn, mx, {"max_rounds": 0}, {"max_rounds": 0})
+    assert st.extra_rounds == 1 and st.far_residual == 0
 
 
 def test_decline_csr(rhz, orc, small_sphere):

@@ -1,0 +1,116 @@
+"""CPU-only checks of the host side: the synthetic workloads, the reference
+Test 5 fixtures through the oracle, the C-ABI library exports, and that the
+product fails loudly (no CPU fallback) when no GPU is present."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import REPO
+
+
+def edge_manifold(f):
+    """Every directed edge appears once and its reverse once (closed, oriented)."""
+    e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]).astype(np.int64)
+    key = e[:, 0] * (1 << 32) + e[:, 1]
+    rev = e[:, 1] * (1 << 32) + e[:, 0]
+    u = np.unique(key)
+    return len(u) == len(key) and np.array_equal(np.sort(key), np.sort(rev))
+
+
+def test_icosphere_workload(geom):
+    v, f = geom.perturbed_icosphere(seed=1)
+    assert f.shape == (20480, 3) and v.shape == (10242, 3)
+    assert edge_manifold(f)
+    assert geom.signed_volume(v, f) > 0
+    r = np.linalg.norm(v.astype(np.float64), axis=1)
+    assert 17.0 < r.min() and r.max() < 23.0
+
+
+def test_vessel_workload(geom):
+    v, f = geom.vessel()
+    assert 190_000 <= len(f) <= 210_000
+    c = geom.cup(step=0.84)
+    t = geom.torus(center=(19.0, 0.0, 45.0), step=0.84)
+    assert edge_manifold(c[1]) and edge_manifold(t[1])
+    assert geom.signed_volume(*c) > 0 and geom.signed_volume(*t) > 0
+    mn, mx = geom.frame_grid(v, 0.1)
+    dims = [mx[i] - mn[i] + 1 for i in range(3)]
+    assert 600e6 < np.prod(np.array(dims, np.int64)) < 700e6
+
+
+def test_slivers_stay_watertight(geom):
+    v, f = geom.perturbed_icosphere(seed=1)
+    v2, f2 = geom.with_slivers(v, f, frac=0.01, seed=3)
+    assert len(f2) == len(f) + 2 * 205
+    assert edge_manifold(f2)
+    a = v2[f2[:, 1]].astype(np.float64) - v2[f2[:, 0]]
+    b = v2[f2[:, 2]].astype(np.float64) - v2[f2[:, 0]]
+    area = np.linalg.norm(np.cross(a, b), axis=1) / 2
+    assert area.min() < 1e-3 * np.median(area)
+
+
+def test_frame_grid_matches_oracle(geom, orc):
+    v, f = geom.perturbed_icosphere(seed=1)
+    for h in (0.1, 0.25, 0.5):
+        mn, mx = geom.frame_grid(v, h)
+        omn = np.zeros(3, np.int32)
+        omx = np.zeros(3, np.int32)
+        orc.lib().orc_frame_grid(orc._f32(v).reshape(-1), len(v), h, omn, omx)
+        assert tuple(omn) == mn and tuple(omx) == mx
+
+
+def test_reference_gwn_fixtures_through_oracle(geom, orc):
+    """Reference Test 5 (harness_vessel_tests.cpp:54-121, thresholds
+    cuda_verification.cu:274-275): >= 99% accuracy on each dirty mesh."""
+    cases = [(geom.open_box(), 0, 42), (geom.uv_sphere(15.0, 48, 48), 1, 1337), (geom.two_cubes(), 2, 9001)]
+    for (v, f), mode, seed in cases:
+        pts = orc.gwn_test_points(mode, seed)
+        w = orc.winding_points(v, f, pts)
+        in_t = 0.50 if mode == 1 else 0.45
+        out_t = 0.35 if mode == 0 else 0.25
+        ok = np.concatenate([w[:5000] >= in_t, w[5000:] < out_t])
+        assert ok.mean() >= 0.99, (mode, ok.mean())
+    # the overlap of the two cubes has winding number 2
+    v, f = geom.two_cubes()
+    assert orc.winding_points(v, f, np.array([[4.0, 0.0, 0.0]], np.float32))[0] == pytest.approx(2.0, abs=1e-5)
+
+
+def declared_symbols():
+    txt = open(os.path.join(REPO, "include", "rhizome_sdf.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(rhz_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(rhz):
+    path = rhz.library_path()
+    if not os.path.exists(path):
+        pytest.skip("library not built")
+    L = ctypes.CDLL(path)
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_params_struct_layout(rhz):
+    """The ctypes mirror and the C struct agree (struct_size is checked by the
+    library on every call)."""
+    import ctypes as C
+    p = rhz.default_params()
+    assert p.struct_size == C.sizeof(rhz.Params)
+    assert p.band_voxels == 8 and p.verify_samples == 4096 and p.patch_radius == 24
+    assert p.max_rounds == 8 and p.max_extra_rounds == 16 and p.sup == 4
+    assert abs(p.jitter_x - 0.037) < 1e-7 and abs(p.jitter_y - 0.051) < 1e-7
+    assert p.fast_min_voxels == 32 * 1024 * 1024 and p.csr_max_entries == 100_000_000
+
+
+def test_no_cpu_fallback_without_gpu(rhz):
+    if rhz.device_count() > 0:
+        pytest.skip("GPU present")
+    tris = np.zeros((1, 12), np.float32)
+    tris[0, 3] = tris[0, 7] = 1.0
+    with pytest.raises(rhz.RhzError, match="no HIP device"):
+        rhz.compute_sdf(tris, 1.0, (0, 0, 0), (3, 3, 3))
