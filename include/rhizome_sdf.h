@@ -207,6 +207,11 @@ int rhz_stage_fixup(const rhz_triangle* d_tris, int64_t n_tris, float h,
  * (cuda_rasterizer.cu:405-513).  *changed |= 1 if any voxel decreased. */
 int rhz_stage_sweep(const float* src, float* dst, const int32_t dims[3], float h,
                     int32_t axis, int32_t dir, int32_t* changed, void* stream);
+/* [D] one round x+, x-, y+, y-, z+, z- exactly as the pipeline runs it
+ * (cuda_rasterizer.cu:1012-1017): the field is in `a` before and after; `b`
+ * is the second buffer (same size). */
+int rhz_stage_sweep_round(float* a, float* b, const int32_t dims[3], float h, int32_t* changed,
+                          void* stream);
 /* [D2] voxels >= 1e29 (cuda_rasterizer.cu:523-534). */
 int rhz_stage_count_far(const float* d, int64_t n, int64_t* count, void* stream);
 /* [D3] safe clamp (cuda_rasterizer.cu:544-556). */

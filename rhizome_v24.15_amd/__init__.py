@@ -138,6 +138,13 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise RhzError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                        "(hipcc --offload-arch=gfx950)")
+    # torch bundles its own libamdhip64.so.7 (same SONAME as /opt/rocm's).
+    # Whichever copy is loaded first serves the whole process; loading torch
+    # first keeps one HIP runtime that both torch tensors and this library use.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i64, i32, f32 = C.c_void_p, C.c_int64, C.c_int32, C.c_float
     L.rhz_sdf_params_default.argtypes = [C.POINTER(Params)]
@@ -156,6 +163,7 @@ def lib():
                                    C.POINTER(C.c_int64), vp]
     L.rhz_stage_fixup.argtypes = [vp, i64, f32, vp, vp, i32, vp, C.POINTER(C.c_int64), vp]
     L.rhz_stage_sweep.argtypes = [vp, vp, vp, f32, i32, i32, vp, vp]
+    L.rhz_stage_sweep_round.argtypes = [vp, vp, vp, f32, vp, vp]
     L.rhz_stage_count_far.argtypes = [vp, i64, C.POINTER(C.c_int64), vp]
     L.rhz_stage_clamp_far.argtypes = [vp, vp, i64, f32, i32, vp]
     L.rhz_stage_verify.argtypes = [vp, i64, f32, vp, vp, i32, i32, i32, vp, vp, C.POINTER(C.c_int32), vp]
@@ -327,6 +335,11 @@ class Stages:
     def sweep(src, dst, dims, h, axis, direction, d_changed, stream=None):
         check(lib().rhz_stage_sweep(_ptr(src), _ptr(dst), _i32x3(dims).ctypes.data, float(h), int(axis),
                                     int(direction), _ptr(d_changed), stream), "rhz_stage_sweep")
+
+    @staticmethod
+    def sweep_round(a, b, dims, h, d_changed, stream=None):
+        check(lib().rhz_stage_sweep_round(_ptr(a), _ptr(b), _i32x3(dims).ctypes.data, float(h), _ptr(d_changed),
+                                          stream), "rhz_stage_sweep_round")
 
     @staticmethod
     def count_far(d, n, stream=None):

@@ -60,39 +60,54 @@ __device__ __forceinline__ Tri9 tile_tri(const float* s, int k) {
                 s[5 * kWTile + k], s[6 * kWTile + k], s[7 * kWTile + k], s[8 * kWTile + k]};
 }
 
+// Every lane owns one point; all lanes walk the same triangles, so triangle
+// loads are wave-uniform (scalar loads, no LDS staging, no barriers).
 template <int PREC>
 __global__ __launch_bounds__(kWBlock) void winding_partial_kernel(const float* __restrict__ tris, int64_t nt,
                                                                   PointSrc src, int64_t n, int64_t per_split,
                                                                   float2* __restrict__ part) {
-    __shared__ float s[9 * kWTile];
     const int64_t i = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
     const int64_t t0 = (int64_t)blockIdx.y * per_split;
     const int64_t t1 = min(nt, t0 + per_split);
-    float px = 0.f, py = 0.f, pz = 0.f;
-    const bool live = i < n;
-    if (live) src.pos(src.voxel(i), px, py, pz);
+    if (i >= n) return;
+    float px, py, pz;
+    src.pos(src.voxel(i), px, py, pz);
     float sum = 0.0f, comp = 0.0f;
-    for (int64_t tb = t0; tb < t1; tb += kWTile) {
-        const int cnt = (int)min((int64_t)kWTile, t1 - tb);
-        __syncthreads();
-        stage_tile(tris, tb, cnt, s);
-        __syncthreads();
-        if (live) {
-            if (PREC == 0) {
-                for (int k = 0; k < cnt; ++k) sum += solid_angle_ref(px, py, pz, tile_tri(s, k));
-            } else {
-#pragma unroll 4
-                for (int k = 0; k < cnt; ++k) {
-                    float term = solid_angle_fast(px, py, pz, tile_tri(s, k));
-                    float y = term - comp;
-                    float t = sum + y;
-                    comp = (t - sum) - y;
-                    sum = t;
-                }
-            }
+    if (PREC == 0) {
+        for (int64_t t = t0; t < t1; ++t) sum += solid_angle_ref(px, py, pz, load_tri(tris, t));
+    } else {
+        // two interleaved compensated accumulators (even / odd triangles)
+        float sum2 = 0.0f, comp2 = 0.0f;
+        int64_t t = t0;
+        for (; t + 1 < t1; t += 2) {
+            float a = solid_angle_fast(px, py, pz, load_tri(tris, t));
+            float b = solid_angle_fast(px, py, pz, load_tri(tris, t + 1));
+            float y = a - comp;
+            float u = sum + y;
+            comp = (u - sum) - y;
+            sum = u;
+            float y2 = b - comp2;
+            float u2 = sum2 + y2;
+            comp2 = (u2 - sum2) - y2;
+            sum2 = u2;
+        }
+        if (t < t1) {
+            float a = solid_angle_fast(px, py, pz, load_tri(tris, t));
+            float y = a - comp;
+            float u = sum + y;
+            comp = (u - sum) - y;
+            sum = u;
+        }
+        // fold the second accumulator into the first
+        float terms[2] = {sum2, -comp2};
+        for (int j = 0; j < 2; ++j) {
+            float y = terms[j] - comp;
+            float u = sum + y;
+            comp = (u - sum) - y;
+            sum = u;
         }
     }
-    if (live) part[(int64_t)blockIdx.y * n + i] = make_float2(sum, comp);
+    part[(int64_t)blockIdx.y * n + i] = make_float2(sum, comp);
 }
 
 __global__ void winding_finalize_kernel(const float2* __restrict__ part, int nsplit, int64_t n, int prec,
@@ -132,29 +147,15 @@ __global__ void winding_finalize_kernel(const float2* __restrict__ part, int nsp
 __global__ __launch_bounds__(kWBlock) void winding_fp64_kernel(const float* __restrict__ tris, int64_t nt,
                                                                PointSrc src, const int64_t* __restrict__ list,
                                                                int64_t m, double* __restrict__ w) {
-    __shared__ float s[9 * kWTile];
     const int64_t j = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
-    const bool live = j < m;
-    int64_t i = 0;
-    double px = 0, py = 0, pz = 0;
-    if (live) {
-        i = list[j];
-        float fx, fy, fz;
-        src.pos(src.voxel(i), fx, fy, fz);
-        px = fx;
-        py = fy;
-        pz = fz;
-    }
+    if (j >= m) return;
+    const int64_t i = list[j];
+    float fx, fy, fz;
+    src.pos(src.voxel(i), fx, fy, fz);
+    const double px = fx, py = fy, pz = fz;
     double sum = 0.0;
-    for (int64_t tb = 0; tb < nt; tb += kWTile) {
-        const int cnt = (int)min((int64_t)kWTile, nt - tb);
-        __syncthreads();
-        stage_tile(tris, tb, cnt, s);
-        __syncthreads();
-        if (live)
-            for (int k = 0; k < cnt; ++k) sum += solid_angle_f64(px, py, pz, tile_tri(s, k));
-    }
-    if (live) w[i] = sum / (4.0 * 3.14159265358979323846);
+    for (int64_t t = 0; t < nt; ++t) sum += solid_angle_f64(px, py, pz, load_tri(tris, t));
+    w[i] = sum / (4.0 * 3.14159265358979323846);
 }
 
 // ---- legacy exact kernel -------------------------------------------------------

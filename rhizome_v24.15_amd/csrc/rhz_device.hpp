@@ -84,6 +84,30 @@ __device__ __forceinline__ float point_tri_dist2(float px, float py, float pz, c
     return dot3(qx, qy, qz, qx, qy, qz);
 }
 
+// atan2 in FP32 for the winding sums: octant reduction to t = min/max in
+// [0, 1], then atan(t) = t * P(t^2) with a degree-7 polynomial fitted to
+// atan(t)/t on [0, 1] by iteratively reweighted least squares (Lawson's
+// minimax method); max |error| 1.1e-7 rad evaluated in FP32.  Zero for
+// (0, 0), signed by y.
+__device__ __forceinline__ float atan2_poly(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float t = (mx > 0.0f) ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+    const float s = t * t;
+    float p = -0.004055280238389969f;
+    p = __builtin_fmaf(p, s, 0.021865447983145714f);
+    p = __builtin_fmaf(p, s, -0.05591578036546707f);
+    p = __builtin_fmaf(p, s, 0.0964243933558464f);
+    p = __builtin_fmaf(p, s, -0.13908720016479492f);
+    p = __builtin_fmaf(p, s, 0.1994658261537552f);
+    p = __builtin_fmaf(p, s, -0.33329862356185913f);
+    p = __builtin_fmaf(p, s, 0.9999993443489075f);
+    float r = p * t;
+    r = (ay > ax) ? (1.5707963267948966f - r) : r;
+    r = (x < 0.0f) ? (3.141592653589793f - r) : r;
+    return copysignf(r, y);
+}
+
 // Van Oosterom-Strackee solid angle in FP32 (reference triangle_solid_angle,
 // math_utils.cuh:109-132: vertex guard <= 1e-6 returns 0).  The winding
 // kernels use this with hardware sqrt and fused dot products; the sign
@@ -92,9 +116,9 @@ __device__ __forceinline__ float solid_angle_fast(float px, float py, float pz, 
     float ax = t.ax - px, ay = t.ay - py, az = t.az - pz;
     float bx = t.bx - px, by = t.by - py, bz = t.bz - pz;
     float cx = t.cx - px, cy = t.cy - py, cz = t.cz - pz;
-    float la = __builtin_sqrtf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax)));
-    float lb = __builtin_sqrtf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx)));
-    float lc = __builtin_sqrtf(__builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx)));
+    float la = __builtin_amdgcn_sqrtf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax)));
+    float lb = __builtin_amdgcn_sqrtf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx)));
+    float lc = __builtin_amdgcn_sqrtf(__builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx)));
     if (la <= 1e-6f || lb <= 1e-6f || lc <= 1e-6f) return 0.0f;
     float crx = __builtin_fmaf(by, cz, -bz * cy);
     float cry = __builtin_fmaf(bz, cx, -bx * cz);
@@ -104,7 +128,7 @@ __device__ __forceinline__ float solid_angle_fast(float px, float py, float pz, 
     float dbc = __builtin_fmaf(bz, cz, __builtin_fmaf(by, cy, bx * cx));
     float dca = __builtin_fmaf(cz, az, __builtin_fmaf(cy, ay, cx * ax));
     float denom = __builtin_fmaf(la * lb, lc, __builtin_fmaf(dab, lc, __builtin_fmaf(dbc, la, dca * lb)));
-    return 2.0f * atan2f(det, denom);
+    return 2.0f * atan2_poly(det, denom);
 }
 
 // Same quantity in binary64 (the tie-break recompute).
@@ -160,7 +184,59 @@ __device__ __forceinline__ float godunov(float ax, float ay, float az, float h) 
     return updated;
 }
 
+// x / 3 correctly rounded for every finite x except -0 (which maps to +0):
+// q0 = x * RN(1/3), exact residual r = x - 3 q0 by FMA, q = q0 + r * RN(1/3).
+// Checked against IEEE division over all 2^32 bit patterns; the sweep
+// numerators are sums of non-negative terms, never -0.
+__device__ __forceinline__ float div3(float x) {
+    const float c = 0.3333333432674408f;  // RN(1/3)
+    float q0 = x * c;
+    float r = __builtin_fmaf(-q0, 3.0f, x);
+    return __builtin_fmaf(r, c, q0);
+}
+
+// The same update as godunov() (identical results bit for bit) arranged for
+// the sweeps: constants 2h^2 and 3h^2 precomputed with the reference's
+// evaluation order ((2h)h, (3h)h), an exit when no branch can go below `cur`
+// (with all three minima finite every branch returns at least
+// s0 + h/sqrt(3) > s0 + 0.57 h, so cur <= s0 + 0.57 h cannot be improved),
+// and early exits before each square root.
+// Returns min(cur, G).
+__device__ __forceinline__ float godunov_min(float cur, float ax, float ay, float az, float h, float h2x2,
+                                             float h2x3, float h057, float hbig) {
+    float s0 = fminf(ax, fminf(ay, az));
+    float s2 = fmaxf(ax, fmaxf(ay, az));
+    // The bound needs s0 <= s1 <= s2, which the sum-minus-extremes middle value
+    // only delivers when no axis holds the 1e30 sentinel; below 4096 h the
+    // rounding of s0 + 0.57 h stays far inside the margin.
+    if (!(cur > s0 + h057) && s2 < hbig) return cur;
+    float s1 = ax + ay + az - s0 - s2;
+    float u = s0 + h;
+    if (u > s1) {
+        float d10 = s1 - s0;
+        float disc = h2x2 - d10 * d10;
+        if (disc >= 0.0f) {
+            u = (s0 + s1 + sqrtf(disc)) * 0.5f;
+            if (u > s2) {
+                float d21 = s2 - s1, d20 = s2 - s0;
+                float disc3 = h2x3 - (d10 * d10 + d21 * d21 + d20 * d20) * 0.5f;
+                if (disc3 >= 0.0f) u = div3(s0 + s1 + s2 + sqrtf(disc3));
+            }
+        }
+    }
+    return (u < cur) ? u : cur;
+}
+
 __device__ __forceinline__ float lattice(int32_t mn, int i, float h) { return (float)(mn + i) * h; }
+
+// Block order for 8-XCD parts: dispatch hands block b to XCD b % 8, so
+// consecutive logical tiles are mapped to the same XCD (and its L2).
+__device__ __forceinline__ unsigned xcd_logical_block(unsigned b, unsigned nblocks) {
+    const unsigned per = nblocks / 8u;
+    const unsigned full = per * 8u;
+    if (b >= full) return b;  // tail blocks keep their index
+    return (b % 8u) * per + b / 8u;
+}
 
 // Wave-aggregated append of `want` items; returns this lane's slot or -1.
 __device__ __forceinline__ long long wave_append(unsigned long long* counter, bool want) {

@@ -69,6 +69,13 @@ hipError_t launch_parity(const float* tris, const Grid& g, int sup, const CsrBuf
 // ---- k_fsm.hip ------------------------------------------------------------
 hipError_t launch_sweep(const float* src, float* dst, const Grid& g, int axis, int dir,
                         int* changed, hipStream_t s);
+// True when the lane-walker sweeps apply (every axis has >= 2 cells and a
+// plane has < 2^29 cells); otherwise launch_sweep uses generic kernels.
+bool lane_sweeps_ok(const Grid& g);
+// Per-plane transpose between the x-fastest and the y-fastest layouts.
+hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s);
+// x-sweep on the y-fastest layout (src/dst both y-fastest).
+hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir, int* changed, hipStream_t s);
 hipError_t launch_redblack(float* g, const int32_t dims[3], float h, int parity, hipStream_t s);
 
 // ---- k_winding.hip --------------------------------------------------------

@@ -284,9 +284,22 @@ int sweep_round(Ctx& c, float* a, float* b, int* changed) {
     float* src = a;
     float* dst = b;
     if (c.sweeps_t) c.sweeps_t->begin(c.s);
-    for (int k = 0; k < 6; ++k) {
-        HIP_TRY(launch_sweep(src, dst, c.g, axes[k], dirs[k], changed, c.s));
-        std::swap(src, dst);
+    if (lane_sweeps_ok(c.g)) {
+        // x-sweeps run on a y-fastest copy: a -> b (transpose), x+ b -> a,
+        // x- a -> b, b -> a (transpose back); then y and z on x-fastest.
+        HIP_TRY(launch_transpose_xy(a, b, c.g, true, c.s));
+        HIP_TRY(launch_sweep_yx(b, a, c.g, +1, changed, c.s));
+        HIP_TRY(launch_sweep_yx(a, b, c.g, -1, changed, c.s));
+        HIP_TRY(launch_transpose_xy(b, a, c.g, false, c.s));
+        for (int k = 2; k < 6; ++k) {
+            HIP_TRY(launch_sweep(src, dst, c.g, axes[k], dirs[k], changed, c.s));
+            std::swap(src, dst);
+        }
+    } else {
+        for (int k = 0; k < 6; ++k) {
+            HIP_TRY(launch_sweep(src, dst, c.g, axes[k], dirs[k], changed, c.s));
+            std::swap(src, dst);
+        }
     }
     if (c.sweeps_t) c.sweeps_t->end(c.s);
     c.st->sweeps += 6;
@@ -940,6 +953,21 @@ int rhz_stage_sweep(const float* src, float* dst, const int32_t dims[3], float h
     g.h = h;
     HIP_TRY(launch_sweep(src, dst, g, axis, dir, changed, (hipStream_t)stream));
     return RHZ_OK;
+}
+
+int rhz_stage_sweep_round(float* a, float* b, const int32_t dims[3], float h, int32_t* changed, void* stream) {
+    rhz_sdf_params p;
+    rhz_sdf_params_default(&p);
+    rhz_sdf_stats st;
+    std::memset(&st, 0, sizeof(st));
+    Grid g;
+    for (int j = 0; j < 3; ++j) {
+        g.mn[j] = 0;
+        g.dims[j] = dims[j];
+    }
+    g.h = h;
+    Ctx c{nullptr, 0, g, &p, a, (hipStream_t)stream, &st, nullptr};
+    return sweep_round(c, a, b, changed);
 }
 
 int rhz_stage_count_far(const float* d, int64_t n, int64_t* count, void* stream) {

@@ -161,6 +161,32 @@ def test_sweep_odd_shapes(rhz, orc, torch_gpu, shape):
             assert np.array_equal(_host(dd).view(np.uint32), ref.view(np.uint32)), (shape, axis, direction)
 
 
+@pytest.mark.parametrize("shape", [(2, 2, 2), (65, 66, 3), (130, 67, 9), (883, 10, 4)])
+def test_sweep_round_bit_exact(rhz, orc, torch_gpu, shape):
+    """A full round as the pipeline runs it (x-sweeps on the transposed copy)
+    equals the oracle's six sweeps bit for bit, over three rounds."""
+    torch = torch_gpu
+    dx, dy, dz = shape
+    rng = np.random.default_rng(9)
+    src = np.full(dx * dy * dz, 1e30, dtype=np.float32)
+    seeds = rng.choice(src.size, size=max(1, src.size // 40), replace=False)
+    src[seeds] = rng.uniform(0.0, 3.0, size=len(seeds)).astype(np.float32)
+    dims = np.array(shape, dtype=np.int32)
+    a = _dev(torch, src)
+    b = torch.empty_like(a)
+    ref = src.copy()
+    for r in range(3):
+        ch_ref = 0
+        for axis in range(3):
+            for direction in (1, -1):
+                ch_ref |= orc.lib().orc_fsm_sweep(ref, orc._i32(dims), 0.3, axis, direction)
+        ch = torch.zeros(1, dtype=torch.int32, device="cuda")
+        rhz.Stages.sweep_round(a, b, dims, 0.3, ch)
+        torch.cuda.synchronize()
+        assert int(ch.item()) == ch_ref
+        assert np.array_equal(_host(a).view(np.uint32), ref.view(np.uint32)), (shape, r)
+
+
 def test_clamp_and_apply(rhz, orc, torch_gpu):
     torch = torch_gpu
     rng = np.random.default_rng(3)
@@ -224,7 +250,19 @@ def test_fixup_matches_oracle(rhz, orc, torch_gpu, geom):
     rhz.Stages.parity(_dev(torch, tris), len(tris), h, mn, dims, s)
     k = rhz.Stages.fixup(_dev(torch, tris), len(tris), h, mn, dims, s)
     assert k == k_ref > 0
-    assert np.array_equal(_host(s), ref)
+    got = _host(s)
+    # The box is aligned with the lattice, so two kinds of voxel have no
+    # well-defined sign: voxels on a face (distance 0 up to the rounding of
+    # the face coordinates, where w jumps and the coplanar triangles
+    # contribute +-2 pi by the sign of a zero determinant) and voxels on the
+    # plane of the missing face (w = 1/2 exactly by symmetry).  Any other
+    # difference is a failure.
+    diff = np.nonzero(got != ref)[0]
+    if len(diff):
+        d = orc.splat_band(tris, h, mn, dims)[diff]
+        w = orc.winding_at_voxels(tris, h, mn, dims, diff, mode=2)
+        tie = np.abs(np.abs(w) - 0.5) < 1e-12
+        assert np.all((d <= 1e-5 * h) | tie), (d, w)
 
 
 @pytest.mark.parametrize("gen", [0, 1, 2])
