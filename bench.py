@@ -64,16 +64,51 @@ def cpu_baseline(tris, h, mn, mx, layers: int):
     }
 
 
+def plain_exact_sample(rhz, torch, d_tris, n_tris, h, mn, mx, layers, learned_gvs):
+    """Plain exact-winding method (every voxel decided by the exact kernel) on
+    a z-slab of the same lattice: its Gvoxel/s rate, and the speed-up of the
+    measured method over it."""
+    zc = (mn[2] + mx[2]) // 2
+    smn = (mn[0], mn[1], zc - layers // 2)
+    smx = (mx[0], mx[1], zc - layers // 2 + layers - 1)
+    n = (smx[0] - smn[0] + 1) * (smx[1] - smn[1] + 1) * layers
+    d_out = torch.empty(n, dtype=torch.float32, device="cuda")
+    p = rhz.default_params(method=rhz.METHOD_PLAIN_EXACT)
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rc, st = rhz.compute_sdf_device(d_tris, n_tris, h, smn, smx, d_out, p, stream=stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rate = n / dt / 1e9
+    return {"value": rate, "unit": "Gvoxel/s", "status": rc, "sample": f"{layers}-layer z-slab ({n} voxels), {dt:.2f} s",
+            "speedup_of_measured_method": learned_gvs / rate if rate > 0 else None,
+            "winding_tpairs_per_s": st.winding_pairs / (st.ms_winding * 1e-3) / 1e12 if st.ms_winding else None}
+
+
+def load_traffic(repo, kernel):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
+    path = os.path.join(repo, "profiles", f"traffic_{kernel}.json")
+    try:
+        with open(path) as fh:
+            return float(json.load(fh)["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--h", type=float, default=0.10)
-    ap.add_argument("--method", default="fast", choices=["fast", "learned", "plain", "legacy"])
+    ap.add_argument("--method", default="learned", choices=["fast", "learned", "plain", "legacy"])
     ap.add_argument("--fsm", default="reference", choices=["reference", "converge"])
     ap.add_argument("--cpu-layers", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain-layers", type=int, default=12,
+                    help="z-layers of the plain exact-winding comparison run (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding is a design note only)"}))
@@ -122,27 +157,43 @@ def main():
     log("timed:", st.log_line())
     log("stats:", json.dumps(st.as_dict()))
 
-    # dominant-kernel roofline from the library's own event timings
+    # dominant-kernel roofline from the library's own HIP events (recorded on
+    # the launch stream around every sweep round / exact-winding launch)
     sweeps = sum(s.sweeps for s in stats)
+    rounds = sum(s.sweep_calls for s in stats)
     ms_sweeps = sum(s.ms_sweeps for s in stats)
     pairs = sum(s.winding_pairs for s in stats)
+    wcalls = sum(s.winding_calls for s in stats)
     ms_wind = sum(s.ms_winding for s in stats)
     rl_fsm = None
-    if sweeps:
-        per_launch_ms = ms_sweeps / sweeps
-        achieved = FSM_BYTES_PER_VOXEL * V / (per_launch_ms * 1e-3) / 1e9
-        rl_fsm = {"bound": "hbm", "kernel": "fsm sweep", "achieved": achieved, "peak": HBM_PEAK_GBS,
+    if sweeps and rounds:
+        # a round = 6 directional sweeps (8 B/voxel each: one read, one write)
+        # plus the two x<->y plane transposes, which are timed but not counted
+        round_ms = ms_sweeps / rounds
+        achieved = 6 * FSM_BYTES_PER_VOXEL * V / (round_ms * 1e-3) / 1e9
+        rl_fsm = {"bound": "hbm", "kernel": "fsm sweep round", "achieved": achieved, "peak": HBM_PEAK_GBS,
                   "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                  "per_launch_ms": per_launch_ms, "launches": sweeps}
+                  "avg_launch_ms": round_ms, "launches": rounds,
+                  "algorithmic_bytes_per_launch": 6 * FSM_BYTES_PER_VOXEL * V}
     rl_wind = None
     if pairs and ms_wind > 0:
         tflops = WINDING_OPS_PER_PAIR * pairs / (ms_wind * 1e-3) / 1e12
-        rl_wind = {"bound": "fp32-valu", "kernel": "exact winding", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
-                   "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS, "ms_per_step": ms_wind / len(stats)}
+        rl_wind = {"bound": "mfma", "kernel": "exact winding", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
+                   "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS, "traffic": None,
+                   "avg_launch_ms": ms_wind / max(wcalls, 1), "launches": wcalls,
+                   "pairs_per_step": pairs / len(stats),
+                   "note": "FP32 vector ALU work priced against MI355X FP32 157.3 TFLOP/s "
+                           f"({WINDING_OPS_PER_PAIR:.0f} FP32 ops per voxel-triangle pair)"}
     dominant = rl_fsm
     if rl_wind and (not rl_fsm or ms_wind > ms_sweeps):
-        dominant = dict(rl_wind, bound="mfma")
-        dominant["note"] = "FP32 vector ALU (no matrix op); peak = MI355X FP32 157.3 TFLOP/s"
+        dominant = rl_wind
+    traffic = load_traffic(REPO, "fsm" if dominant is rl_fsm else "winding")
+    if dominant is not None and traffic is not None:
+        dominant["traffic"] = traffic
+
+    plain = None
+    if args.plain_layers > 0 and method != rhz.METHOD_PLAIN_EXACT:
+        plain = plain_exact_sample(rhz, torch, d_tris, len(tris), args.h, mn, mx, args.plain_layers, value)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -174,6 +225,7 @@ def main():
         "stage_ms": {k: getattr(st, k) for k in ("ms_splat", "ms_parity", "ms_fixup", "ms_eikonal", "ms_verify",
                                                   "ms_apply", "ms_total")},
         "cpu_baseline": cpu,
+        "plain_exact": plain,
     }
     print(json.dumps(out))
 

@@ -86,9 +86,13 @@ typedef struct rhz_sdf_params {
     int32_t fsm_round_cap;        /* cap for RHZ_FSM_CONVERGE */
     int32_t winding_precision;    /* enum rhz_winding_precision */
     float fp64_band;              /* 1e-3 */
-    float reprobe_fraction;       /* FAST_LEARNED: fraction of voxels re-probed exactly */
+    float reprobe_fraction;       /* FAST_LEARNED: top-ranked fraction of all voxels re-probed exactly */
     int32_t learned_max_patch_gens; /* FAST_LEARNED: mismatch-region growth generations */
-    const void* mlp_weights;      /* FAST_LEARNED: packed proposer weights (NULL = built-in) */
+    float candidate_radius;       /* FAST_LEARNED: candidates have band distance <= this many voxels */
+    int32_t scorer;               /* FAST_LEARNED: 0 proposer network, 1 planar-margin rule,
+                                     2 stratified (no ranking: every k-th candidate) */
+    const float* mlp_weights;     /* FAST_LEARNED: proposer parameters, rhz_mlp_blob_count()
+                                     floats (host memory); NULL = the built-in trained set */
 } rhz_sdf_params;
 
 /* Counters of one call; the fields of the reference's log line
@@ -125,6 +129,11 @@ typedef struct rhz_sdf_stats {
     double ms_sweeps;             /* device time of the directional sweep kernels alone */
     double ms_winding;            /* device time of the exact winding kernels alone */
     int64_t winding_pairs;        /* (voxel, triangle) pairs those kernels evaluated */
+    int64_t candidates;           /* FAST_LEARNED: candidate voxels ranked */
+    int64_t mismatches_found;     /* FAST_LEARNED: scanline signs the exact kernel corrected */
+    double ms_proposer;           /* FAST_LEARNED: features + network + selection */
+    int64_t winding_calls;        /* bracketed exact-winding launches behind ms_winding */
+    int64_t sweep_calls;          /* bracketed sweep rounds behind ms_sweeps (6 sweeps each) */
 } rhz_sdf_stats;
 
 void rhz_sdf_params_default(rhz_sdf_params* p);
@@ -178,6 +187,33 @@ int rhz_winding_numbers(const float* verts, int64_t n_verts, const uint32_t* idx
  * reference uses 8).  Replaces CudaRasterizer::fast_sweeping_eikonal_gpu
  * (cuda_rasterizer.hpp:288-291, cuda_rasterizer.cu:1536-1629).  Host grid. */
 int rhz_eikonal_redblack(float* grid, const int32_t dims[3], float h, int32_t iters);
+
+/* ---- sign-risk proposer (no counterpart in the reference) ----------------
+ * MLP 12 -> 64 -> 64 -> 64 -> 1 (ReLU), BF16 matrix-core arithmetic with FP32
+ * accumulation.  Parameter blob (FP32): W1[12][64] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] w4[64] b4 (9217 trainable values) followed by the feature
+ * mean[12] and inverse std[12]. */
+int rhz_mlp_param_count(void);   /* 9217 */
+int rhz_mlp_blob_count(void);    /* 9241 */
+/* Copies the built-in trained blob (rhz_mlp_blob_count floats) to out;
+ * returns 0, or RHZ_DECLINED if the library was built without one. */
+int rhz_mlp_builtin(float* out);
+/* logits[i] = MLP(features[i]) for n rows of 12 features (device buffers). */
+int rhz_mlp_forward(const float* d_blob, const float* d_feats, int64_t n, float* d_logits, void* stream);
+/* Adds the gradient of sum_i c_i * BCE(logit_i, y_i) (c_i = pos_weight for
+ * y_i = 1) w.r.t. the 9217 parameters to d_grads, and the loss to *d_loss. */
+int rhz_mlp_grad(const float* d_blob, const float* d_feats, const float* d_labels, int64_t n,
+                 float pos_weight, float* d_grads, float* d_loss, void* stream);
+/* Adam step on the first n_params values of d_blob (grad scaled by grad_scale). */
+int rhz_mlp_adam(float* d_blob, const float* d_grads, float* d_m, float* d_v, int32_t n_params, float lr,
+                 float beta1, float beta2, float eps, int32_t step, float grad_scale, float weight_decay,
+                 void* stream);
+/* Candidate voxels and their 12 features as the FAST_LEARNED method sees
+ * them (after splat, signed scanline pass and fixup): up to cap voxel ids and
+ * cap x 12 features into device buffers; *n_out = total count (may exceed cap). */
+int rhz_learned_candidates(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],
+                           const int32_t max_c[3], const rhz_sdf_params* params, int64_t* d_voxels,
+                           float* d_feats, int64_t cap, int64_t* n_out, void* stream);
 
 /* ---- stage-level entry points (device pointers, one stream) -------------
  * Each runs one stage of the fast path on device buffers; the pipeline above

@@ -66,6 +66,8 @@ class Params(C.Structure):
         ("fp64_band", C.c_float),
         ("reprobe_fraction", C.c_float),
         ("learned_max_patch_gens", C.c_int32),
+        ("candidate_radius", C.c_float),
+        ("scorer", C.c_int32),
         ("mlp_weights", C.c_void_p),
     ]
 
@@ -103,6 +105,11 @@ class Stats(C.Structure):
         ("ms_sweeps", C.c_double),
         ("ms_winding", C.c_double),
         ("winding_pairs", C.c_int64),
+        ("candidates", C.c_int64),
+        ("mismatches_found", C.c_int64),
+        ("ms_proposer", C.c_double),
+        ("winding_calls", C.c_int64),
+        ("sweep_calls", C.c_int64),
     ]
 
     def as_dict(self):
@@ -170,6 +177,14 @@ def lib():
     L.rhz_stage_mark_patch.argtypes = [vp, vp, i32, i32, vp, vp]
     L.rhz_stage_apply.argtypes = [vp, vp, i64, vp]
     L.rhz_stage_winding_voxels.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, vp, vp]
+    L.rhz_mlp_param_count.restype = C.c_int
+    L.rhz_mlp_blob_count.restype = C.c_int
+    L.rhz_mlp_builtin.argtypes = [vp]
+    L.rhz_mlp_forward.argtypes = [vp, vp, i64, vp, vp]
+    L.rhz_mlp_grad.argtypes = [vp, vp, vp, i64, f32, vp, vp, vp]
+    L.rhz_mlp_adam.argtypes = [vp, vp, vp, vp, i32, f32, f32, f32, f32, i32, f32, f32, vp]
+    L.rhz_learned_candidates.argtypes = [vp, i64, f32, vp, vp, C.POINTER(Params), vp, vp, i64,
+                                         C.POINTER(C.c_int64), vp]
     _lib = L
     return L
 
@@ -376,3 +391,56 @@ class Stages:
         check(lib().rhz_stage_winding_voxels(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c).ctypes.data,
                                              _i32x3(dims).ctypes.data, _ptr(d_voxels), int(n_voxels),
                                              int(precision), _ptr(d_w), stream), "rhz_stage_winding_voxels")
+
+
+# ----------------------------------------------------------------------------
+# sign-risk proposer (12 -> 64 -> 64 -> 64 -> 1 MLP) and its training kernels
+# ----------------------------------------------------------------------------
+class Proposer:
+    """Wrappers of the rhz_mlp_* / rhz_learned_candidates entry points."""
+
+    N_FEATURES = 12
+
+    @staticmethod
+    def param_count() -> int:
+        return lib().rhz_mlp_param_count()
+
+    @staticmethod
+    def blob_count() -> int:
+        return lib().rhz_mlp_blob_count()
+
+    @staticmethod
+    def builtin():
+        """The trained blob compiled into the library, or None."""
+        out = np.zeros(Proposer.blob_count(), dtype=np.float32)
+        rc = check(lib().rhz_mlp_builtin(out.ctypes.data), "rhz_mlp_builtin")
+        return out if rc == 0 else None
+
+    @staticmethod
+    def forward(d_blob, d_feats, n, d_logits, stream=None):
+        check(lib().rhz_mlp_forward(_ptr(d_blob), _ptr(d_feats), int(n), _ptr(d_logits), stream),
+              "rhz_mlp_forward")
+
+    @staticmethod
+    def grad(d_blob, d_feats, d_labels, n, pos_weight, d_grads, d_loss, stream=None):
+        check(lib().rhz_mlp_grad(_ptr(d_blob), _ptr(d_feats), _ptr(d_labels), int(n), float(pos_weight),
+                                 _ptr(d_grads), _ptr(d_loss), stream), "rhz_mlp_grad")
+
+    @staticmethod
+    def adam(d_blob, d_grads, d_m, d_v, n_params, lr, step, beta1=0.9, beta2=0.999, eps=1e-8, grad_scale=1.0,
+             weight_decay=0.0, stream=None):
+        check(lib().rhz_mlp_adam(_ptr(d_blob), _ptr(d_grads), _ptr(d_m), _ptr(d_v), int(n_params), float(lr),
+                                 float(beta1), float(beta2), float(eps), int(step), float(grad_scale),
+                                 float(weight_decay), stream), "rhz_mlp_adam")
+
+    @staticmethod
+    def candidates(d_tris, n_tris, h, min_c, max_c, d_voxels, d_feats, cap, params: Params | None = None,
+                   stream=None, **kw):
+        p = params if params is not None else default_params(**kw)
+        n = C.c_int64(0)
+        check(lib().rhz_learned_candidates(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c).ctypes.data,
+                                           _i32x3(max_c).ctypes.data, C.byref(p),
+                                           _ptr(d_voxels) if d_voxels is not None else None,
+                                           _ptr(d_feats) if d_feats is not None else None, int(cap), C.byref(n),
+                                           stream), "rhz_learned_candidates")
+        return n.value

@@ -26,9 +26,8 @@ SOURCES = [
     "k_winding.hip",
     "k_mesh.hip",
     "k_learned.hip",
+    "k_mlp.hip",
     "sdf_pipeline.cpp",
-    "learned_path.cpp",
-    "cuda_rasterizer.cpp",
 ]
 
 ARCH = os.environ.get("RHZ_OFFLOAD_ARCH", "gfx950")

@@ -1,0 +1,336 @@
+// k_learned.hip -- risk-ranked exact re-probing (the learned sign gate).
+//
+// After the band splat and the scanline sign pass, the only voxels whose
+// scanline sign can disagree with the exact winding sign on a closed mesh are
+// (a) voxels within the jitter offset of the surface (the ray runs
+// (0.037, 0.051) h away from the voxel centre) and (b) voxels of deferred
+// columns, which the exact fixup already decided.  These kernels:
+//   * compact the candidate voxels (band distance <= r h),
+//   * compute 12 geometric features per candidate from the column's
+//     candidate-triangle list,
+//   * rank them (the proposer network or the planar-margin rule) and select
+//     the top-ranked ones with a score histogram,
+//   * commit exact decisions, and grow the set around every mismatch found
+//     (26-neighbourhood, each voxel probed at most once: a bitmap).
+#include "rhz_device.hpp"
+#include "rhz_internal.hpp"
+
+namespace rhz {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n, int per = kBlock) {
+    int64_t b = (n + per - 1) / per;
+    if (b > 8192) b = 8192;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+__global__ void candidates_kernel(const float* __restrict__ d, int64_t n, float thr, int64_t* __restrict__ list,
+                                  int64_t cap, unsigned long long* counter) {
+    const int64_t n4 = n >> 2;
+    const float4* d4 = reinterpret_cast<const float4*>(d);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t iters = (n4 + stride - 1) / stride;
+    for (int64_t it = 0; it < iters; ++it) {
+        const int64_t i = tid + it * stride;
+        float4 v = make_float4(1e30f, 1e30f, 1e30f, 1e30f);
+        if (i < n4) v = d4[i];
+        const bool any = (v.x <= thr) | (v.y <= thr) | (v.z <= thr) | (v.w <= thr);
+        if (__ballot(any) == 0) continue;
+        const float vals[4] = {v.x, v.y, v.z, v.w};
+        for (int j = 0; j < 4; ++j) {
+            const bool want = vals[j] <= thr;
+            long long slot = wave_append(counter, want);
+            if (want && slot < cap) list[slot] = i * 4 + j;
+        }
+    }
+    const int64_t t = (n4 << 2) + tid;
+    const bool want = (t < n) && (d[t] <= thr);
+    long long slot = wave_append(counter, want);
+    if (want && slot < cap) list[slot] = t;
+}
+
+// Closest-point region of p on triangle t: 0 face interior, 1 edge, 2 vertex
+// (same region tests as point_tri_dist2).
+__device__ __forceinline__ int closest_region(float px, float py, float pz, const Tri9& t) {
+    float abx = t.bx - t.ax, aby = t.by - t.ay, abz = t.bz - t.az;
+    float acx = t.cx - t.ax, acy = t.cy - t.ay, acz = t.cz - t.az;
+    float apx = px - t.ax, apy = py - t.ay, apz = pz - t.az;
+    float d1 = dot3(abx, aby, abz, apx, apy, apz), d2 = dot3(acx, acy, acz, apx, apy, apz);
+    if (d1 <= 0.0f && d2 <= 0.0f) return 2;
+    float bpx = px - t.bx, bpy = py - t.by, bpz = pz - t.bz;
+    float d3 = dot3(abx, aby, abz, bpx, bpy, bpz), d4 = dot3(acx, acy, acz, bpx, bpy, bpz);
+    if (d3 >= 0.0f && d4 <= d3) return 2;
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) return 1;
+    float cpx = px - t.cx, cpy = py - t.cy, cpz = pz - t.cz;
+    float d5 = dot3(abx, aby, abz, cpx, cpy, cpz), d6 = dot3(acx, acy, acz, cpx, cpy, cpz);
+    if (d6 >= 0.0f && d5 <= d6) return 2;
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) return 1;
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) return 1;
+    return 0;
+}
+
+// Features of candidate voxel v (nearest triangles searched in the voxel's
+// scanline supercell list):
+//  0 d/h  1 scan sign  2-4 nearest-triangle normal  5 jitter offset along the
+//  normal (voxels)  6 planar margin  7 second-nearest distance/h
+//  8 log1p(#triangles within 1 h)  9 normal spread within 1.5 h
+//  10 closest-point region  11 distance from the voxel centre to the nearest
+//  crossing of its jittered column ray (voxels)
+__global__ void features_kernel(const float* __restrict__ tris, const float4* __restrict__ xyab,
+                                const int64_t* __restrict__ start, const int32_t* __restrict__ ids,
+                                const uint8_t* __restrict__ sign, const int64_t* __restrict__ list, int64_t n,
+                                int mnx, int mny, int mnz, int dx, int dy, float h, int sup, int sdx, float jx,
+                                float jy, float* __restrict__ feats) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t v = list[i];
+    const int x = (int)(v % dx);
+    const int64_t r = v / dx;
+    const int y = (int)(r % dy);
+    const int z = (int)(r / dy);
+    const float px = lattice(mnx, x, h), py = lattice(mny, y, h), pz = lattice(mnz, z, h);
+    const float rx = ((float)(mnx + x) + jx) * h;
+    const float ry = ((float)(mny + y) + jy) * h;
+    const float h2 = h * h;
+    float best = 3.0e38f, second = 3.0e38f;
+    int best_t = -1;
+    int near_cnt = 0;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    int spread_cnt = 0;
+    float zhit = 2.0f * h;
+    const int64_t s = (int64_t)(y / sup) * sdx + (x / sup);
+    for (int64_t e = start[s]; e < start[s + 1]; ++e) {
+        const int ti = ids[e];
+        const Tri9 t = load_tri(tris, ti);
+        const float d2 = point_tri_dist2(px, py, pz, t);
+        if (d2 < best) {
+            second = best;
+            best = d2;
+            best_t = ti;
+        } else if (d2 < second) {
+            second = d2;
+        }
+        if (d2 <= h2) ++near_cnt;
+        if (d2 <= 2.25f * h2) {
+            const float4 nn = reinterpret_cast<const float4*>(tris + 12 * (int64_t)ti)[2];
+            sx += nn.y;
+            sy += nn.z;
+            sz += nn.w;
+            ++spread_cnt;
+        }
+        // crossing of the jittered column ray (same tests as the scan)
+        const float4 ab = xyab[ti];
+        if (!(rx < ab.x || rx > ab.z || ry < ab.y || ry > ab.w)) {
+            float e0 = (t.bx - t.ax) * (ry - t.ay) - (t.by - t.ay) * (rx - t.ax);
+            float e1 = (t.cx - t.bx) * (ry - t.by) - (t.cy - t.by) * (rx - t.bx);
+            float e2 = (t.ax - t.cx) * (ry - t.cy) - (t.ay - t.cy) * (rx - t.cx);
+            bool pos = (e0 > 0.0f && e1 > 0.0f && e2 > 0.0f);
+            bool neg = (e0 < 0.0f && e1 < 0.0f && e2 < 0.0f);
+            float area2 = e0 + e1 + e2;
+            if ((pos || neg) && fabsf(area2) >= 1e-12f) {
+                float zh = (e1 * t.az + e2 * t.bz + e0 * t.cz) / area2;
+                zhit = fminf(zhit, fabsf(zh - pz));
+            }
+        }
+    }
+    float nx = 0.f, ny = 0.f, nz = 0.f;
+    int region = 0;
+    if (best_t >= 0) {
+        const float4 nn = reinterpret_cast<const float4*>(tris + 12 * (int64_t)best_t)[2];
+        nx = nn.y;
+        ny = nn.z;
+        nz = nn.w;
+        region = closest_region(px, py, pz, load_tri(tris, best_t));
+    }
+    const float dist = sqrtf(best) / h;
+    const uint8_t sg = sign[v];
+    const float scan_in = (sg == 1) ? 1.0f : 0.0f;
+    const float t_off = jx * nx + jy * ny;
+    const float s_out = 1.0f - 2.0f * scan_in;
+    const float spread = spread_cnt ? 1.0f - sqrtf(sx * sx + sy * sy + sz * sz) / (float)spread_cnt : 0.0f;
+    float* f = feats + 12 * i;
+    f[0] = dist;
+    f[1] = scan_in;
+    f[2] = nx;
+    f[3] = ny;
+    f[4] = nz;
+    f[5] = t_off;
+    f[6] = s_out * t_off - dist;
+    f[7] = fminf(sqrtf(second) / h, 4.0f);
+    f[8] = log1pf((float)near_cnt);
+    f[9] = spread;
+    f[10] = 0.5f * (float)region;
+    f[11] = fminf(zhit / h, 2.0f);
+}
+
+// score = planar margin (feature 6): the rule the network starts from
+__global__ void margin_score_kernel(const float* __restrict__ feats, int64_t n, float* __restrict__ score) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) score[i] = feats[12 * i + 6];
+}
+
+constexpr int kBins = 4096;
+
+__device__ __forceinline__ int score_bin(float s, float lo, float inv_w) {
+    float b = (s - lo) * inv_w;
+    b = fminf(fmaxf(b, 0.0f), (float)(kBins - 1));
+    return (int)b;
+}
+
+__global__ void hist_kernel(const float* __restrict__ score, int64_t n, float lo, float inv_w,
+                            unsigned* __restrict__ hist) {
+    __shared__ unsigned sh[kBins];
+    for (int b = threadIdx.x; b < kBins; b += blockDim.x) sh[b] = 0;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        atomicAdd(&sh[score_bin(score[i], lo, inv_w)], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < kBins; b += blockDim.x)
+        if (sh[b]) atomicAdd(&hist[b], sh[b]);
+}
+
+__global__ void select_kernel(const float* __restrict__ score, const int64_t* __restrict__ list, int64_t n,
+                              float lo, float inv_w, int min_bin, int64_t* __restrict__ out, int64_t cap,
+                              unsigned long long* counter) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t iters = (n + stride - 1) / stride;
+    for (int64_t it = 0; it < iters; ++it) {
+        const int64_t i = tid + it * stride;
+        const bool want = (i < n) && score_bin(score[i], lo, inv_w) >= min_bin;
+        long long slot = wave_append(counter, want);
+        if (want && slot < cap) out[slot] = list[i];
+    }
+}
+
+__global__ void mark_probed_kernel(const int64_t* __restrict__ list, int64_t n, unsigned* __restrict__ bits) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t v = list[i];
+        atomicOr(&bits[v >> 5], 1u << (v & 31));
+    }
+}
+
+// Commit exact decisions; append voxels whose scanline sign was wrong.
+__global__ void commit_kernel(const double* __restrict__ w, const int64_t* __restrict__ list, int64_t n,
+                              uint8_t* __restrict__ sign, int64_t* __restrict__ mism, int64_t cap,
+                              unsigned long long* counter) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t iters = (n + stride - 1) / stride;
+    for (int64_t it = 0; it < iters; ++it) {
+        const int64_t i = tid + it * stride;
+        bool bad = false;
+        int64_t v = 0;
+        if (i < n) {
+            v = list[i];
+            const uint8_t exact = (fabs(w[i]) >= 0.5) ? 1 : 0;
+            bad = sign[v] != exact;
+            sign[v] = exact;
+        }
+        long long slot = wave_append(counter, bad);
+        if (bad && slot < cap) mism[slot] = v;
+    }
+}
+
+// Unprobed 26-neighbours of every mismatch, each claimed once via the bitmap.
+__global__ void grow_kernel(const int64_t* __restrict__ mism, int64_t m, int dx, int dy, int dz,
+                            unsigned* __restrict__ bits, int64_t* __restrict__ out, int64_t cap,
+                            unsigned long long* counter) {
+    const int64_t total = m * 27;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t iters = (total + stride - 1) / stride;
+    for (int64_t it = 0; it < iters; ++it) {
+        const int64_t k = tid + it * stride;
+        bool want = false;
+        int64_t u = 0;
+        if (k < total) {
+            const int64_t v = mism[k / 27];
+            const int o = (int)(k % 27);
+            const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
+            const int x = (int)(v % dx) + ox;
+            const int64_t r = v / dx;
+            const int y = (int)(r % dy) + oy;
+            const int z = (int)(r / dy) + oz;
+            if (o != 13 && x >= 0 && x < dx && y >= 0 && y < dy && z >= 0 && z < dz) {
+                u = ((int64_t)z * dy + y) * dx + x;
+                const unsigned bit = 1u << (u & 31);
+                want = (atomicOr(&bits[u >> 5], bit) & bit) == 0;
+            }
+        }
+        long long slot = wave_append(counter, want);
+        if (want && slot < cap) out[slot] = u;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_candidates(const float* d, int64_t n, float thr, int64_t* list, int64_t cap,
+                             unsigned long long* counter, hipStream_t s) {
+    candidates_kernel<<<grid_for(n / 4 + 1), kBlock, 0, s>>>(d, n, thr, list, cap, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_features(const float* tris, const CsrBuffers& b, const uint8_t* sign, const int64_t* list,
+                           int64_t n, const Grid& g, int sup, float jx, float jy, float* feats, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int sdx = (g.dims[0] + sup - 1) / sup;
+    features_kernel<<<(unsigned)((n + 127) / 128), 128, 0, s>>>(tris, b.xyab, b.start, b.ids, sign, list, n, g.mn[0],
+                                                              g.mn[1], g.mn[2], g.dims[0], g.dims[1], g.h, sup, sdx,
+                                                              jx, jy, feats);
+    return hipGetLastError();
+}
+
+hipError_t launch_margin_score(const float* feats, int64_t n, float* score, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    margin_score_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(feats, n, score);
+    return hipGetLastError();
+}
+
+int score_bins() { return kBins; }
+
+hipError_t launch_score_hist(const float* score, int64_t n, float lo, float hi, unsigned* hist, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(hist, 0, sizeof(unsigned) * kBins, s);
+    if (e != hipSuccess || n <= 0) return e;
+    hist_kernel<<<grid_for(n, 1024), 1024, 0, s>>>(score, n, lo, (float)kBins / (hi - lo), hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_select(const float* score, const int64_t* list, int64_t n, float lo, float hi, int min_bin,
+                         int64_t* out, int64_t cap, unsigned long long* counter, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    select_kernel<<<grid_for(n), kBlock, 0, s>>>(score, list, n, lo, (float)kBins / (hi - lo), min_bin, out, cap,
+                                                counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_probed(const int64_t* list, int64_t n, unsigned* bits, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    mark_probed_kernel<<<grid_for(n), kBlock, 0, s>>>(list, n, bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_commit(const double* w, const int64_t* list, int64_t n, uint8_t* sign, int64_t* mism, int64_t cap,
+                         unsigned long long* counter, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    commit_kernel<<<grid_for(n), kBlock, 0, s>>>(w, list, n, sign, mism, cap, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_grow(const int64_t* mism, int64_t m, const int32_t dims[3], unsigned* bits, int64_t* out,
+                       int64_t cap, unsigned long long* counter, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    grow_kernel<<<grid_for(m * 27), kBlock, 0, s>>>(mism, m, dims[0], dims[1], dims[2], bits, out, cap, counter);
+    return hipGetLastError();
+}
+
+}  // namespace rhz

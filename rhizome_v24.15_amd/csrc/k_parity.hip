@@ -94,6 +94,14 @@ __device__ __forceinline__ int toggle_index(float hit, int mnz, int dz, float h)
     return k;
 }
 
+// SIGNED = false: reference parity (inside = odd number of hits below).
+// SIGNED = true: signed crossing count along the same jittered ray, i.e. the
+// winding number of the ray point w = sum over hits above of +1 (triangle
+// facing +z: the ray leaves the solid) / -1 (facing -z).  Inside = w != 0,
+// which also classifies overlapping shells (w = 2) that parity calls outside.
+// A column whose crossings do not cancel (w above the top != 0) is deferred
+// like an odd one.
+template <bool SIGNED>
 __global__ __launch_bounds__(kParityBlock) void parity_kernel(
     const float* __restrict__ tris, const float4* __restrict__ xyab, const int64_t* __restrict__ start,
     const int32_t* __restrict__ ids, int mnx, int mny, int mnz, int dx, int dy, int dz, float h, int sup,
@@ -107,7 +115,7 @@ __global__ __launch_bounds__(kParityBlock) void parity_kernel(
     const float rx = ((float)(mnx + cx) + jx) * h;
     const float ry = ((float)(mny + cy) + jy) * h;
 
-    int n = 0;
+    int n = 0, total = 0;
     bool bad = false;
     const int64_t s = (int64_t)(cy / sup) * sdx + (cx / sup);
     const int64_t e0 = start[s], e1 = start[s + 1];
@@ -126,7 +134,9 @@ __global__ __launch_bounds__(kParityBlock) void parity_kernel(
         if (fabsf(area2) < 1e-12f) continue;
         float zh = (d1 * t.az + d2 * t.bz + d0 * t.cz) / area2;
         if (n < max_hits) {
-            kbuf[n][lane] = toggle_index(zh, mnz, dz, h);
+            const int k = toggle_index(zh, mnz, dz, h);
+            kbuf[n][lane] = SIGNED ? ((k << 1) | (pos ? 1 : 0)) : k;
+            total += pos ? 1 : -1;
             ++n;
         } else {
             bad = true;
@@ -136,7 +146,7 @@ __global__ __launch_bounds__(kParityBlock) void parity_kernel(
 
     const int64_t zs = (int64_t)dx * dy;
     uint8_t* out = sign + col;
-    if (bad || (n & 1)) {
+    if (bad || (n & 1) || (SIGNED && total != 0)) {
         for (int z = 0; z < dz; ++z) out[(int64_t)z * zs] = 2;
         *flag = 1;
         return;
@@ -150,16 +160,21 @@ __global__ __launch_bounds__(kParityBlock) void parity_kernel(
         }
         kbuf[j + 1][lane] = key;
     }
+    const int shift = SIGNED ? 1 : 0;
     int p = 0;
-    int next = (n > 0) ? kbuf[0][lane] : dz;
+    int next = (n > 0) ? (kbuf[0][lane] >> shift) : dz;
     uint8_t inside = 0;
+    int below = 0;  // signed crossings below the voxel centre
     for (int z = 0; z < dz; ++z) {
         while (next <= z) {
-            inside ^= 1;
+            if (SIGNED)
+                below += (kbuf[p][lane] & 1) ? 1 : -1;
+            else
+                inside ^= 1;
             ++p;
-            next = (p < n) ? kbuf[p][lane] : dz;
+            next = (p < n) ? (kbuf[p][lane] >> shift) : dz;
         }
-        out[(int64_t)z * zs] = inside;
+        out[(int64_t)z * zs] = SIGNED ? (uint8_t)(below != 0) : inside;
     }
 }
 
@@ -192,13 +207,19 @@ hipError_t launch_csr_fill(int64_t nt, const Grid& g, int sup, const CsrBuffers&
 }
 
 hipError_t launch_parity(const float* tris, const Grid& g, int sup, const CsrBuffers& b, float jx, float jy,
-                         int max_hits, uint8_t* sign, int* flag, hipStream_t s) {
+                         int max_hits, uint8_t* sign, int* flag, hipStream_t s, bool signed_count) {
     if (max_hits > kParityMaxHits) return hipErrorInvalidValue;
     int sdx = (g.dims[0] + sup - 1) / sup;
     int64_t ncols = (int64_t)g.dims[0] * g.dims[1];
-    parity_kernel<<<(unsigned)((ncols + kParityBlock - 1) / kParityBlock), kParityBlock, 0, s>>>(
-        tris, b.xyab, b.start, b.ids, g.mn[0], g.mn[1], g.mn[2], g.dims[0], g.dims[1], g.dims[2], g.h, sup, sdx,
-        jx, jy, max_hits, sign, flag);
+    const unsigned blocks = (unsigned)((ncols + kParityBlock - 1) / kParityBlock);
+    if (signed_count)
+        parity_kernel<true><<<blocks, kParityBlock, 0, s>>>(tris, b.xyab, b.start, b.ids, g.mn[0], g.mn[1], g.mn[2],
+                                                           g.dims[0], g.dims[1], g.dims[2], g.h, sup, sdx, jx, jy,
+                                                           max_hits, sign, flag);
+    else
+        parity_kernel<false><<<blocks, kParityBlock, 0, s>>>(tris, b.xyab, b.start, b.ids, g.mn[0], g.mn[1],
+                                                            g.mn[2], g.dims[0], g.dims[1], g.dims[2], g.h, sup, sdx,
+                                                            jx, jy, max_hits, sign, flag);
     return hipGetLastError();
 }
 

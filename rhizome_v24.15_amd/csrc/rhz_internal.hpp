@@ -62,9 +62,10 @@ int64_t num_supercells(const Grid& g, int sup);
 hipError_t launch_csr_count(const float* tris, int64_t nt, const Grid& g, int sup,
                             const CsrBuffers& b, unsigned long long* total, hipStream_t s);
 hipError_t launch_csr_fill(int64_t nt, const Grid& g, int sup, const CsrBuffers& b, hipStream_t s);
+// signed_count = false: reference parity; true: signed crossing count (w != 0).
 hipError_t launch_parity(const float* tris, const Grid& g, int sup, const CsrBuffers& b,
                          float jx, float jy, int max_hits, uint8_t* sign, int* flag,
-                         hipStream_t s);
+                         hipStream_t s, bool signed_count = false);
 
 // ---- k_fsm.hip ------------------------------------------------------------
 hipError_t launch_sweep(const float* src, float* dst, const Grid& g, int axis, int dir,
@@ -100,6 +101,35 @@ size_t legacy_scratch_bytes(int64_t nt);
 // Standalone winding numbers over an indexed mesh.
 hipError_t launch_winding_points(const float* verts, const uint32_t* idx, int64_t nt,
                                  const float* pts, int64_t np, float* out, hipStream_t s);
+
+// ---- k_learned.hip --------------------------------------------------------
+// Candidate voxels: band distance d[i] <= thr (appended to list, counter += n).
+hipError_t launch_candidates(const float* d, int64_t n, float thr, int64_t* list, int64_t cap,
+                             unsigned long long* counter, hipStream_t s);
+// 12 features per candidate (row-major n x 12) from the column candidate lists.
+hipError_t launch_features(const float* tris, const CsrBuffers& b, const uint8_t* sign, const int64_t* list,
+                           int64_t n, const Grid& g, int sup, float jx, float jy, float* feats, hipStream_t s);
+hipError_t launch_margin_score(const float* feats, int64_t n, float* score, hipStream_t s);
+int score_bins();
+hipError_t launch_score_hist(const float* score, int64_t n, float lo, float hi, unsigned* hist, hipStream_t s);
+hipError_t launch_select(const float* score, const int64_t* list, int64_t n, float lo, float hi, int min_bin,
+                         int64_t* out, int64_t cap, unsigned long long* counter, hipStream_t s);
+hipError_t launch_mark_probed(const int64_t* list, int64_t n, unsigned* bits, hipStream_t s);
+// sign[v] = exact decision; voxels whose previous sign differed go to mism.
+hipError_t launch_commit(const double* w, const int64_t* list, int64_t n, uint8_t* sign, int64_t* mism,
+                         int64_t cap, unsigned long long* counter, hipStream_t s);
+// Unprobed 26-neighbours of the mismatches (marked probed as they are claimed).
+hipError_t launch_grow(const int64_t* mism, int64_t m, const int32_t dims[3], unsigned* bits, int64_t* out,
+                       int64_t cap, unsigned long long* counter, hipStream_t s);
+
+// ---- k_mlp.hip ------------------------------------------------------------
+int mlp_param_count();  // 9217 trainable
+int mlp_blob_count();   // + feature mean / inverse std
+hipError_t launch_mlp_forward(const float* P, const float* X, int64_t n, float* out, hipStream_t s);
+hipError_t launch_mlp_grad(const float* P, const float* X, const float* Y, int64_t n, float pos_weight, float* G,
+                           float* loss, hipStream_t s);
+hipError_t launch_adam(float* P, const float* G, float* M, float* V, int n, float lr, float b1, float b2, float eps,
+                       int step, float scale, float wd, hipStream_t s);
 
 // ---- k_mesh.hip -----------------------------------------------------------
 hipError_t launch_tri_valid(const float* verts, int64_t nv, const uint32_t* idx, int64_t nt,

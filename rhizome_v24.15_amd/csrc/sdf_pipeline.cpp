@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <climits>
 #include <string>
 #include <vector>
 
@@ -76,6 +77,14 @@ enum Slot {
     S_OUT,         // field staged by the host API
     S_MISC,
     S_MISC2,
+    S_BITS,        // probed-voxel bitmap (learned method)
+    S_CAND,        // candidate voxel list
+    S_FEAT,        // candidate features
+    S_SCORE,       // candidate scores
+    S_SEL,         // selected / grown probe list
+    S_MISM,        // mismatch list
+    S_HIST,        // score histogram
+    S_MLP,         // proposer parameters (device copy)
     S_COUNT
 };
 
@@ -130,7 +139,7 @@ void* ws_get(Workspace* w, int slot, size_t bytes) {
 }
 
 // counters: ull slots 0..15, int slots at byte offset 128 (32 ints)
-enum CounterId { C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4 };
+enum CounterId { C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4, C_SEL = 5, C_GROW = 6 };
 enum FlagId { F_UNKNOWN = 0, F_CHANGED = 1 };
 
 unsigned long long* cnt_ptr(Workspace* w, int id) { return w->d_cnt + id; }
@@ -366,8 +375,9 @@ bool int_range_ok(const Grid& g, int64_t nt) {
     return true;
 }
 
-// [B] candidate lists + parity; returns RHZ_DECLINED on a pathological list.
-int run_parity(Ctx& c, uint8_t* sign, int* unknown_flag) {
+// [B] candidate lists + parity (or the signed crossing count); returns
+// RHZ_DECLINED on a pathological list.  *csr receives the lists.
+int run_parity(Ctx& c, uint8_t* sign, int* unknown_flag, bool signed_count = false, CsrBuffers* csr = nullptr) {
     const rhz_sdf_params* p = c.p;
     const int sup = p->sup;
     const int64_t nsup = num_supercells(c.g, sup);
@@ -394,10 +404,11 @@ int run_parity(Ctx& c, uint8_t* sign, int* unknown_flag) {
     HIP_TRY(launch_scan_u32(b.counts, nsup, b.start, scan, c.s));
     HIP_TRY(launch_csr_fill(c.nt, c.g, sup, b, c.s));
     HIP_TRY(launch_parity(c.tris, c.g, sup, b, p->jitter_x, p->jitter_y, p->parity_max_hits, sign,
-                          flag_ptr(c.w, F_UNKNOWN), c.s));
+                          flag_ptr(c.w, F_UNKNOWN), c.s, signed_count));
     rc = read_counters(c);
     if (rc) return rc;
     *unknown_flag = hf(c, F_UNKNOWN);
+    if (csr) *csr = b;
     return RHZ_OK;
 }
 
@@ -554,6 +565,256 @@ int run_plain_exact(Ctx& c) {
     return RHZ_OK;
 }
 
+// ---------------------------------------------------------------------------
+// learned sign gate (FAST_LEARNED)
+// ---------------------------------------------------------------------------
+#if __has_include("proposer_weights.inc")
+#include "proposer_weights.inc"  // kProposerBlob[9241]: weights trained by the project's own kernels
+#define RHZ_HAVE_BUILTIN_PROPOSER 1
+#endif
+
+// Exact decisions for `n` listed voxels committed into `sign`; voxels whose
+// sign changed are appended to mism (counter C_MISM, not reset here).
+int probe_commit(Ctx& c, const int64_t* list, int64_t n, uint8_t* sign, int64_t* mism, int64_t mism_cap) {
+    const int prec = c.p->winding_precision;
+    const int64_t chunk = std::min<int64_t>(n, 8ll << 20);
+    for (int64_t off = 0; off < n; off += chunk) {
+        const int64_t m = std::min(chunk, n - off);
+        double* w = (double*)ws_get(c.w, S_W, sizeof(double) * (size_t)m);
+        void* scr = ws_get(c.w, S_WSCR, winding_scratch_bytes(m, c.nt));
+        int64_t* f64 = (int64_t*)ws_get(c.w, S_FP64, sizeof(int64_t) * (size_t)m);
+        if (!w || !scr || !f64) return set_err(RHZ_ERR_NOMEM, "winding scratch allocation failed");
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FP64), 0, sizeof(unsigned long long), c.s));
+        if (c.wind_t) c.wind_t->begin(c.s);
+        HIP_TRY(launch_winding(c.tris, c.nt, c.g, list + off, 0, m, prec, c.p->fp64_band, w, scr, f64,
+                               cnt_ptr(c.w, C_FP64), c.s));
+        if (c.wind_t) c.wind_t->end(c.s);
+        c.st->winding_pairs += m * c.nt;
+        if (prec != 0) {
+            int rc = read_counters(c);
+            if (rc) return rc;
+            const int64_t nf = (int64_t)hc(c, C_FP64);
+            c.st->fp64_voxels += nf;
+            HIP_TRY(launch_winding_fp64(c.tris, c.nt, c.g, list + off, 0, f64, nf, w, c.s));
+        }
+        HIP_TRY(launch_commit(w, list + off, m, sign, mism, mism_cap, cnt_ptr(c.w, C_MISM), c.s));
+    }
+    return RHZ_OK;
+}
+
+// Candidates + features (after splat, signed scan and fixup).  On return
+// *cand / *feats are workspace buffers holding *n_cand rows.
+int learned_candidates(Ctx& c, uint8_t* sign, const CsrBuffers& b, int64_t** cand, float** feats,
+                       int64_t* n_cand) {
+    const int64_t V = c.g.n();
+    const float thr = c.p->candidate_radius * c.g.h;
+    int64_t cap = (int64_t)(c.w->bytes[S_CAND] / sizeof(int64_t));
+    cap = std::max<int64_t>(cap, std::max<int64_t>(1 << 20, V / 512));
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        int64_t* list = (int64_t*)ws_get(c.w, S_CAND, sizeof(int64_t) * (size_t)cap);
+        if (!list) return set_err(RHZ_ERR_NOMEM, "candidate list allocation failed");
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_LIST), 0, sizeof(unsigned long long), c.s));
+        HIP_TRY(launch_candidates(c.d, V, thr, list, cap, cnt_ptr(c.w, C_LIST), c.s));
+        int rc = read_counters(c);
+        if (rc) return rc;
+        const int64_t n = (int64_t)hc(c, C_LIST);
+        if (n <= cap) {
+            float* f = (float*)ws_get(c.w, S_FEAT, sizeof(float) * 12 * (size_t)std::max<int64_t>(n, 1));
+            if (!f) return set_err(RHZ_ERR_NOMEM, "feature buffer allocation failed");
+            HIP_TRY(launch_features(c.tris, b, sign, list, n, c.g, c.p->sup, c.p->jitter_x, c.p->jitter_y, f, c.s));
+            *cand = list;
+            *feats = f;
+            *n_cand = n;
+            return RHZ_OK;
+        }
+        cap = n;
+    }
+    return set_err(RHZ_ERR_HIP, "candidate compaction did not converge");
+}
+
+// Ranks the candidates and returns the top ones (at least `k`, whole score
+// bins) in the S_SEL list.
+int learned_select(Ctx& c, const int64_t* cand, const float* feats, int64_t n, int64_t k, int64_t** sel,
+                   int64_t* n_sel) {
+    const rhz_sdf_params* p = c.p;
+    int64_t* out = (int64_t*)ws_get(c.w, S_SEL, sizeof(int64_t) * (size_t)std::max<int64_t>(n, 1));
+    if (!out) return set_err(RHZ_ERR_NOMEM, "selection buffer allocation failed");
+    *sel = out;
+    if (n == 0) {
+        *n_sel = 0;
+        return RHZ_OK;
+    }
+    if (k >= n || p->scorer == 2) {
+        // everything, or a stratified subset without ranking (every n/k-th candidate)
+        const int64_t m = (k >= n) ? n : k;
+        if (m == n) {
+            HIP_TRY(hipMemcpyAsync(out, cand, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToDevice, c.s));
+        } else {
+            float* score = (float*)ws_get(c.w, S_SCORE, sizeof(float) * (size_t)n);
+            if (!score) return set_err(RHZ_ERR_NOMEM, "score buffer allocation failed");
+            // stratified: score 1 for every (n/m)-th candidate, 0 otherwise
+            std::vector<float> hs((size_t)n, 0.0f);
+            const double stride = (double)n / (double)m;
+            for (int64_t i = 0; i < m; ++i) hs[(size_t)std::min<int64_t>(n - 1, (int64_t)((i + 0.5) * stride))] = 1.0f;
+            HIP_TRY(hipMemcpyAsync(score, hs.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c.s));
+            HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_SEL), 0, sizeof(unsigned long long), c.s));
+            HIP_TRY(launch_select(score, cand, n, 0.0f, 1.0f, score_bins() - 1, out, n, cnt_ptr(c.w, C_SEL), c.s));
+            int rc = read_counters(c);
+            if (rc) return rc;
+            *n_sel = (int64_t)hc(c, C_SEL);
+            return RHZ_OK;
+        }
+        *n_sel = n;
+        return RHZ_OK;
+    }
+    float* score = (float*)ws_get(c.w, S_SCORE, sizeof(float) * (size_t)n);
+    unsigned* hist = (unsigned*)ws_get(c.w, S_HIST, sizeof(unsigned) * (size_t)score_bins());
+    if (!score || !hist) return set_err(RHZ_ERR_NOMEM, "score buffer allocation failed");
+    float lo, hi;
+    const float* blob = p->mlp_weights;
+#ifdef RHZ_HAVE_BUILTIN_PROPOSER
+    if (!blob) blob = kProposerBlob;
+#endif
+    if (p->scorer == 0 && blob) {
+        float* dblob = (float*)ws_get(c.w, S_MLP, sizeof(float) * (size_t)mlp_blob_count());
+        if (!dblob) return set_err(RHZ_ERR_NOMEM, "proposer parameter allocation failed");
+        HIP_TRY(hipMemcpyAsync(dblob, blob, sizeof(float) * (size_t)mlp_blob_count(), hipMemcpyHostToDevice, c.s));
+        HIP_TRY(launch_mlp_forward(dblob, feats, n, score, c.s));
+        lo = -40.0f;
+        hi = 40.0f;
+    } else {
+        // margin = s_out * t - d / h lies in [-0.17, 0.063] for candidates
+        HIP_TRY(launch_margin_score(feats, n, score, c.s));
+        lo = -0.5f;
+        hi = 0.25f;
+    }
+    HIP_TRY(launch_score_hist(score, n, lo, hi, hist, c.s));
+    std::vector<unsigned> hh((size_t)score_bins());
+    HIP_TRY(hipMemcpyAsync(hh.data(), hist, sizeof(unsigned) * hh.size(), hipMemcpyDeviceToHost, c.s));
+    HIP_TRY(hipStreamSynchronize(c.s));
+    int min_bin = 0;
+    int64_t acc = 0;
+    for (int bi = score_bins() - 1; bi >= 0; --bi) {
+        acc += hh[(size_t)bi];
+        if (acc >= k) {
+            min_bin = bi;
+            break;
+        }
+    }
+    HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_SEL), 0, sizeof(unsigned long long), c.s));
+    HIP_TRY(launch_select(score, cand, n, lo, hi, min_bin, out, n, cnt_ptr(c.w, C_SEL), c.s));
+    int rc = read_counters(c);
+    if (rc) return rc;
+    *n_sel = (int64_t)hc(c, C_SEL);
+    return RHZ_OK;
+}
+
+// Fast path with the learned sign gate: signed scanline signs, exact
+// re-probes of the top-ranked candidates, growth around every mismatch,
+// then the reference far-field policy.
+int run_learned(Ctx& c) {
+    const rhz_sdf_params* p = c.p;
+    rhz_sdf_stats* st = c.st;
+    const int64_t V = c.g.n();
+    if (!int_range_ok(c.g, c.nt) || c.nt <= 0) {
+        st->decline_reason = 1;
+        return RHZ_DECLINED;
+    }
+    uint8_t* sign = (uint8_t*)ws_get(c.w, S_SIGN, (size_t)V);
+    unsigned* bits = (unsigned*)ws_get(c.w, S_BITS, sizeof(unsigned) * (size_t)((V + 31) / 32));
+    if (!sign || !bits) {
+        st->decline_reason = 4;
+        return RHZ_DECLINED;
+    }
+    Timer tm(true);
+    tm.mark(0, c.s);
+    int rc = zero_counters(c);
+    if (rc) return rc;
+    HIP_TRY(launch_fill(c.d, V, kFar, c.s));
+    HIP_TRY(launch_splat(c.tris, c.nt, c.g, (float)p->band_voxels * c.g.h, c.d, c.s));
+    tm.mark(1, c.s);
+    int unknown = 0;
+    CsrBuffers b;
+    rc = run_parity(c, sign, &unknown, true, &b);
+    if (rc) return rc;
+    tm.mark(2, c.s);
+    st->unknown_flag = unknown;
+    if (unknown) {
+        int64_t fixed = 0;
+        rc = fixup_unknown(c, sign, &fixed);
+        if (rc) return rc;
+        st->fixup_voxels = fixed;
+    }
+    tm.mark(3, c.s);
+    int64_t* cand = nullptr;
+    float* feats = nullptr;
+    int64_t n_cand = 0;
+    rc = learned_candidates(c, sign, b, &cand, &feats, &n_cand);
+    if (rc) return rc;
+    st->candidates = n_cand;
+    const int64_t k = std::max<int64_t>(1, (int64_t)((double)p->reprobe_fraction * (double)V));
+    int64_t* sel = nullptr;
+    int64_t n_sel = 0;
+    rc = learned_select(c, cand, feats, n_cand, k, &sel, &n_sel);
+    if (rc) return rc;
+    tm.mark(4, c.s);
+    HIP_TRY(hipMemsetAsync(bits, 0, sizeof(unsigned) * (size_t)((V + 31) / 32), c.s));
+    HIP_TRY(launch_mark_probed(sel, n_sel, bits, c.s));
+    int64_t mism_cap = std::max<int64_t>(n_sel, 1 << 16);
+    int64_t* mism = (int64_t*)ws_get(c.w, S_MISM, sizeof(int64_t) * (size_t)mism_cap);
+    if (!mism) return set_err(RHZ_ERR_NOMEM, "mismatch list allocation failed");
+    int64_t* cur = sel;
+    int64_t n_cur = n_sel;
+    int gens = 0;
+    while (n_cur > 0) {
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_MISM), 0, sizeof(unsigned long long), c.s));
+        rc = probe_commit(c, cur, n_cur, sign, mism, mism_cap);
+        if (rc) return rc;
+        st->reprobed_voxels += n_cur;
+        rc = read_counters(c);
+        if (rc) return rc;
+        const int64_t m = (int64_t)hc(c, C_MISM);
+        if (m > mism_cap) return set_err(RHZ_ERR_HIP, "mismatch list overflow");
+        st->mismatches_found += m;
+        if (m == 0 || gens >= p->learned_max_patch_gens) break;
+        // grow into the 26-neighbourhood of this generation's mismatches
+        const int64_t gcap = 26 * m;
+        int64_t* grown = (int64_t*)ws_get(c.w, S_SEL, sizeof(int64_t) * (size_t)std::max(gcap, n_sel));
+        if (!grown) return set_err(RHZ_ERR_NOMEM, "growth list allocation failed");
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_GROW), 0, sizeof(unsigned long long), c.s));
+        HIP_TRY(launch_grow(mism, m, c.g.dims, bits, grown, gcap, cnt_ptr(c.w, C_GROW), c.s));
+        rc = read_counters(c);
+        if (rc) return rc;
+        n_cur = std::min<int64_t>((int64_t)hc(c, C_GROW), gcap);
+        cur = grown;
+        if (n_cur > mism_cap) {
+            mism_cap = n_cur;
+            mism = (int64_t*)ws_get(c.w, S_MISM, sizeof(int64_t) * (size_t)mism_cap);
+            if (!mism) return set_err(RHZ_ERR_NOMEM, "mismatch list allocation failed");
+        }
+        ++gens;
+    }
+    st->patch_rounds = gens;
+    st->sites = (int32_t)std::min<int64_t>(st->mismatches_found, INT32_MAX);
+    st->probes = 1;
+    tm.mark(5, c.s);
+    rc = run_fsm(c, sign);
+    if (rc) return rc;
+    tm.mark(6, c.s);
+    HIP_TRY(launch_apply(c.d, sign, V, c.s));
+    tm.mark(7, c.s);
+    HIP_TRY(hipStreamSynchronize(c.s));
+    st->ms_splat = tm.ms(0, 1);
+    st->ms_parity = tm.ms(1, 2);
+    st->ms_fixup = tm.ms(2, 3);
+    st->ms_proposer = tm.ms(3, 4);
+    st->ms_verify = tm.ms(4, 5);
+    st->ms_eikonal = tm.ms(5, 6);
+    st->ms_apply = tm.ms(6, 7);
+    st->ms_total = tm.ms(0, 7);
+    return RHZ_OK;
+}
+
 int check_params(const rhz_sdf_params* p) {
     if (!p || p->struct_size != (int32_t)sizeof(rhz_sdf_params))
         return set_err(RHZ_ERR_ARG, "rhz_sdf_params.struct_size mismatch (rebuild against rhizome_sdf.h)");
@@ -574,6 +835,8 @@ int dispatch(Ctx& c) {
     if (rc >= 0 && hipStreamSynchronize(c.s) == hipSuccess) {
         c.st->ms_sweeps = sw.total_ms();
         c.st->ms_winding = wd.total_ms();
+        c.st->sweep_calls = (int64_t)(sw.used / 2);
+        c.st->winding_calls = (int64_t)(wd.used / 2);
     }
     c.sweeps_t = nullptr;
     c.wind_t = nullptr;
@@ -605,6 +868,8 @@ int dispatch_inner(Ctx& c) {
             return run_legacy(c);
         case RHZ_METHOD_PLAIN_EXACT:
             return run_plain_exact(c);
+        case RHZ_METHOD_FAST_LEARNED:
+            return run_learned(c);
         default:
             return set_err(RHZ_ERR_ARG, "method %d is not available", method);
     }
@@ -664,6 +929,8 @@ void rhz_sdf_params_default(rhz_sdf_params* p) {
     p->fp64_band = 1e-3f;
     p->reprobe_fraction = 1e-3f;
     p->learned_max_patch_gens = 64;
+    p->candidate_radius = 0.1f;
+    p->scorer = 0;
     p->mlp_weights = nullptr;
 }
 
@@ -1073,6 +1340,91 @@ int rhz_stage_winding_voxels(const rhz_triangle* d_tris, int64_t n_tris, float h
         HIP_TRY(launch_winding_fp64(c.tris, n_tris, g, d_voxels, 0, f64, (int64_t)hc(c, C_FP64), d_w, c.s));
     }
     HIP_TRY(hipStreamSynchronize(c.s));
+    return RHZ_OK;
+}
+
+
+int rhz_mlp_param_count(void) { return mlp_param_count(); }
+int rhz_mlp_blob_count(void) { return mlp_blob_count(); }
+
+int rhz_mlp_builtin(float* out) {
+#ifdef RHZ_HAVE_BUILTIN_PROPOSER
+    if (out) std::memcpy(out, kProposerBlob, sizeof(float) * (size_t)mlp_blob_count());
+    return RHZ_OK;
+#else
+    (void)out;
+    return RHZ_DECLINED;
+#endif
+}
+
+int rhz_mlp_forward(const float* d_blob, const float* d_feats, int64_t n, float* d_logits, void* stream) {
+    if (n < 0 || (n > 0 && (!d_blob || !d_feats || !d_logits))) return set_err(RHZ_ERR_ARG, "null buffer");
+    HIP_TRY(launch_mlp_forward(d_blob, d_feats, n, d_logits, (hipStream_t)stream));
+    return RHZ_OK;
+}
+
+int rhz_mlp_grad(const float* d_blob, const float* d_feats, const float* d_labels, int64_t n, float pos_weight,
+                 float* d_grads, float* d_loss, void* stream) {
+    if (n < 0 || (n > 0 && (!d_blob || !d_feats || !d_labels || !d_grads || !d_loss)))
+        return set_err(RHZ_ERR_ARG, "null buffer");
+    HIP_TRY(launch_mlp_grad(d_blob, d_feats, d_labels, n, pos_weight, d_grads, d_loss, (hipStream_t)stream));
+    return RHZ_OK;
+}
+
+int rhz_mlp_adam(float* d_blob, const float* d_grads, float* d_m, float* d_v, int32_t n_params, float lr,
+                 float beta1, float beta2, float eps, int32_t step, float grad_scale, float weight_decay,
+                 void* stream) {
+    if (n_params <= 0 || n_params > mlp_param_count() || step < 1) return set_err(RHZ_ERR_ARG, "invalid Adam step");
+    HIP_TRY(launch_adam(d_blob, d_grads, d_m, d_v, n_params, lr, beta1, beta2, eps, step, grad_scale, weight_decay,
+                        (hipStream_t)stream));
+    return RHZ_OK;
+}
+
+int rhz_learned_candidates(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],
+                           const int32_t max_c[3], const rhz_sdf_params* params, int64_t* d_voxels, float* d_feats,
+                           int64_t cap, int64_t* n_out, void* stream) {
+    int rc = need_device();
+    if (rc) return rc;
+    rc = check_params(params);
+    if (rc) return rc;
+    Grid g;
+    rc = make_grid(min_c, max_c, h, &g);
+    if (rc) return rc;
+    if (n_tris <= 0 || !d_tris || !n_out) return set_err(RHZ_ERR_ARG, "null buffer");
+    if (!int_range_ok(g, n_tris)) return set_err(RHZ_ERR_ARG, "lattice out of range");
+    rhz_sdf_stats st;
+    std::memset(&st, 0, sizeof(st));
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    const int64_t V = g.n();
+    float* d = (float*)ws_get(w, S_OUT, sizeof(float) * (size_t)V);
+    uint8_t* sign = (uint8_t*)ws_get(w, S_SIGN, (size_t)V);
+    if (!d || !sign) return set_err(RHZ_ERR_NOMEM, "grid allocation failed");
+    Ctx c{reinterpret_cast<const float*>(d_tris), n_tris, g, params, d, (hipStream_t)stream, &st, w};
+    rc = zero_counters(c);
+    if (rc) return rc;
+    HIP_TRY(launch_fill(d, V, kFar, c.s));
+    HIP_TRY(launch_splat(c.tris, n_tris, g, (float)params->band_voxels * h, d, c.s));
+    int unknown = 0;
+    CsrBuffers b;
+    rc = run_parity(c, sign, &unknown, true, &b);
+    if (rc) return rc;
+    if (unknown) {
+        int64_t fixed = 0;
+        rc = fixup_unknown(c, sign, &fixed);
+        if (rc) return rc;
+    }
+    int64_t* cand = nullptr;
+    float* feats = nullptr;
+    int64_t n = 0;
+    rc = learned_candidates(c, sign, b, &cand, &feats, &n);
+    if (rc) return rc;
+    const int64_t k = std::min(n, cap);
+    if (k > 0 && d_voxels) HIP_TRY(hipMemcpyAsync(d_voxels, cand, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToDevice, c.s));
+    if (k > 0 && d_feats) HIP_TRY(hipMemcpyAsync(d_feats, feats, sizeof(float) * 12 * (size_t)k, hipMemcpyDeviceToDevice, c.s));
+    HIP_TRY(hipStreamSynchronize(c.s));
+    *n_out = n;
     return RHZ_OK;
 }
 

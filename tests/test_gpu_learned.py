@@ -1,0 +1,199 @@
+"""The learned sign gate (FAST_LEARNED) and the proposer's HIP kernels.
+
+* MLP forward / gradient / Adam kernels against the CPU oracle (oracle/mlp.py,
+  which emulates the kernels' BF16 operand rounding; sums are FP32 on the GPU
+  and FP64 in the oracle, so logits agree to within accumulation order and
+  rare one-ulp BF16 flips of an activation: max |dz| <= 2e-2 (1 + |z|),
+  median <= 1e-5 (1 + |z|); gradients per parameter block within 2e-2
+  relative L2, cosine >= 0.9995).
+* The learned method against the plain exact-winding method on the same
+  lattice: signs identical (0 mismatches; only voxels lying on the surface
+  to within 1e-5 voxel may differ, where the exact winding number is 1/2 to
+  rounding), distance magnitudes bit-identical (same band and sweeps).
+"""
+import numpy as np
+import pytest
+
+from conftest import dims_of
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _blob(seed):
+    rng = np.random.default_rng(seed)
+    b = mlp.init_blob(seed, mean=rng.normal(size=12) * 0.1, istd=1.0 + rng.random(12))
+    b[mlp.OFF_B1:mlp.OFF_W2] = rng.normal(size=64) * 0.1
+    b[mlp.OFF_B2:mlp.OFF_W3] = rng.normal(size=64) * 0.1
+    b[mlp.OFF_B3:mlp.OFF_W4] = rng.normal(size=64) * 0.1
+    b[mlp.OFF_B4] = -0.3
+    return b
+
+
+@pytest.mark.parametrize("n", [1, 16, 1003, 70001])
+def test_mlp_forward_matches_oracle(rhz, torch_gpu, n):
+    torch = torch_gpu
+    blob = _blob(11)
+    X = np.random.default_rng(n).normal(size=(n, 12)).astype(np.float32)
+    d_b = torch.from_numpy(blob).cuda()
+    d_x = torch.from_numpy(X).cuda()
+    d_z = torch.full((n,), np.nan, dtype=torch.float32, device="cuda")
+    rhz.Proposer.forward(d_b, d_x, n, d_z)
+    torch.cuda.synchronize()
+    z = d_z.cpu().numpy().astype(np.float64)
+    ref, _ = mlp.forward(blob, X, bf16=True)
+    err = np.abs(z - ref) / (1 + np.abs(ref))
+    assert np.all(np.isfinite(z))
+    assert err.max() <= 2e-2, err.max()
+    assert np.median(err) <= 1e-5, np.median(err)
+
+
+@pytest.mark.parametrize("n", [37, 4099])
+def test_mlp_grad_matches_oracle(rhz, torch_gpu, n):
+    torch = torch_gpu
+    blob = _blob(5)
+    rng = np.random.default_rng(n)
+    X = rng.normal(size=(n, 12)).astype(np.float32)
+    Y = (rng.random(n) < 0.2).astype(np.float32)
+    d_b = torch.from_numpy(blob).cuda()
+    d_x = torch.from_numpy(X).cuda()
+    d_y = torch.from_numpy(Y).cuda()
+    d_g = torch.zeros(mlp.N_PARAM, dtype=torch.float32, device="cuda")
+    d_l = torch.zeros(1, dtype=torch.float32, device="cuda")
+    rhz.Proposer.grad(d_b, d_x, d_y, n, 4.0, d_g, d_l)
+    torch.cuda.synchronize()
+    g = d_g.cpu().numpy().astype(np.float64)
+    loss = float(d_l.cpu().numpy()[0])
+    ref_loss, ref_g = mlp.loss_and_grad(blob, X, Y, 4.0, bf16=True)
+    assert abs(loss - ref_loss) <= 1e-3 * abs(ref_loss)
+    blocks = [(mlp.OFF_W1, mlp.OFF_B1), (mlp.OFF_B1, mlp.OFF_W2), (mlp.OFF_W2, mlp.OFF_B2), (mlp.OFF_B2, mlp.OFF_W3),
+              (mlp.OFF_W3, mlp.OFF_B3), (mlp.OFF_B3, mlp.OFF_W4), (mlp.OFF_W4, mlp.OFF_B4),
+              (mlp.OFF_B4, mlp.N_PARAM)]
+    for a, b in blocks:
+        ga, ra = g[a:b], ref_g[a:b]
+        rel = np.linalg.norm(ga - ra) / max(np.linalg.norm(ra), 1e-12)
+        assert rel <= 2e-2, (a, b, rel)
+    cos = np.dot(g, ref_g) / (np.linalg.norm(g) * np.linalg.norm(ref_g))
+    assert cos >= 0.9995, cos
+
+
+def test_mlp_grad_accumulates(rhz, torch_gpu):
+    """Two calls add: grad(A) + grad(B) == grad(A u B) up to summation order."""
+    torch = torch_gpu
+    blob = _blob(7)
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(512, 12)).astype(np.float32)
+    Y = (rng.random(512) < 0.5).astype(np.float32)
+    d_b = torch.from_numpy(blob).cuda()
+    d_x = torch.from_numpy(X).cuda()
+    d_y = torch.from_numpy(Y).cuda()
+    g1 = torch.zeros(mlp.N_PARAM, dtype=torch.float32, device="cuda")
+    g2 = torch.zeros_like(g1)
+    l1 = torch.zeros(1, dtype=torch.float32, device="cuda")
+    l2 = torch.zeros_like(l1)
+    rhz.Proposer.grad(d_b, d_x, d_y, 512, 1.0, g1, l1)
+    rhz.Proposer.grad(d_b, d_x[:200], d_y[:200], 200, 1.0, g2, l2)
+    rhz.Proposer.grad(d_b, d_x[200:], d_y[200:], 312, 1.0, g2, l2)
+    torch.cuda.synchronize()
+    a, b = g1.cpu().numpy(), g2.cpu().numpy()
+    assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(a)
+
+
+def test_adam_matches_oracle(rhz, torch_gpu):
+    torch = torch_gpu
+    rng = np.random.default_rng(2)
+    blob = _blob(3)
+    m = np.zeros(mlp.N_PARAM, np.float32)
+    v = np.zeros(mlp.N_PARAM, np.float32)
+    d_b = torch.from_numpy(blob.copy()).cuda()
+    d_m = torch.zeros(mlp.N_PARAM, dtype=torch.float32, device="cuda")
+    d_v = torch.zeros_like(d_m)
+    ref = blob.copy()
+    for step in range(1, 4):
+        g = rng.normal(size=mlp.N_PARAM).astype(np.float32)
+        d_g = torch.from_numpy(g).cuda()
+        rhz.Proposer.adam(d_b, d_g, d_m, d_v, mlp.N_PARAM, lr=1e-3, step=step, grad_scale=0.5, weight_decay=1e-4)
+        mlp.adam(ref, g, m, v, mlp.N_PARAM, lr=1e-3, step=step, grad_scale=0.5, weight_decay=1e-4)
+    torch.cuda.synchronize()
+    got = d_b.cpu().numpy()
+    assert np.allclose(got[:mlp.N_PARAM], ref[:mlp.N_PARAM], rtol=1e-5, atol=1e-7)
+    # the normalisation statistics behind the parameters are not touched
+    assert np.array_equal(got[mlp.N_PARAM:], blob[mlp.N_PARAM:])
+
+
+def _compare_learned(rhz, tris, h, mn, mx, **kw):
+    pl = rhz.default_params(method=rhz.METHOD_FAST_LEARNED, **kw)
+    rc, got, st = rhz.compute_sdf(tris, h, mn, mx, pl)
+    assert rc == rhz.RHZ_OK, st.as_dict()
+    assert st.method_used == rhz.METHOD_FAST_LEARNED
+    pe = rhz.default_params(method=rhz.METHOD_PLAIN_EXACT)
+    rc, ref, est = rhz.compute_sdf(tris, h, mn, mx, pe)
+    assert rc == rhz.RHZ_OK
+    got = got.reshape(-1)
+    ref = ref.reshape(-1)
+    assert np.array_equal(np.abs(got).view(np.uint32), np.abs(ref).view(np.uint32))
+    wrong = np.signbit(got) != np.signbit(ref)
+    on_surface = np.abs(ref) <= 1e-5 * h
+    assert not np.any(wrong & ~on_surface), f"{int(np.sum(wrong & ~on_surface))} sign mismatches; {st.as_dict()}"
+    return st, est, int(wrong.sum())
+
+
+@pytest.mark.parametrize("scorer", [0, 1])
+def test_learned_small_sphere(rhz, small_sphere, scorer):
+    tris, h, mn, mx = small_sphere
+    st, est, _ = _compare_learned(rhz, tris, h, mn, mx, scorer=scorer)
+    assert st.candidates > 0
+    assert st.reprobed_voxels < est.reprobed_voxels
+
+
+def test_learned_config1(rhz, config1):
+    tris, h, mn, mx = config1
+    st, est, _ = _compare_learned(rhz, tris, h, mn, mx)
+    V = int(np.prod(dims_of(mn, mx).astype(np.int64)))
+    # the exact kernel probes a small fraction of what the plain path probes
+    assert st.reprobed_voxels <= 0.02 * V
+    print(st.log_line(), "reprobed", st.reprobed_voxels, "mismatches", st.mismatches_found)
+
+
+def test_learned_overlapping_solids(rhz, geom, orc):
+    """Two overlapping closed cubes: winding number 2 in the overlap; the
+    signed crossing count keeps it inside where parity would flip it."""
+    v, f = geom.two_cubes()
+    tris = orc.build_triangles(v, f)
+    h = 0.25
+    mn, mx = geom.frame_grid(v, h)
+    _compare_learned(rhz, tris, h, mn, mx)
+
+
+def test_learned_vessel_coarse(rhz, geom):
+    v, f = geom.vessel()
+    tris = rhz.build_triangles(v, f)
+    h = 0.5
+    mn, mx = geom.frame_grid(v, h)
+    st, est, _ = _compare_learned(rhz, tris, h, mn, mx)
+    print(st.log_line(), "reprobed", st.reprobed_voxels, "mismatches", st.mismatches_found)
+
+
+def test_learned_candidates_api(rhz, torch_gpu, small_sphere):
+    torch = torch_gpu
+    tris, h, mn, mx = small_sphere
+    d_t = torch.from_numpy(tris).cuda()
+    n = rhz.Proposer.candidates(d_t, len(tris), h, mn, mx, None, None, 0)
+    assert n > 0
+    d_v = torch.zeros(n, dtype=torch.int64, device="cuda")
+    d_f = torch.zeros((n, 12), dtype=torch.float32, device="cuda")
+    n2 = rhz.Proposer.candidates(d_t, len(tris), h, mn, mx, d_v, d_f, n)
+    torch.cuda.synchronize()
+    assert n2 == n
+    vox = d_v.cpu().numpy()
+    feats = d_f.cpu().numpy()
+    V = int(np.prod(dims_of(mn, mx).astype(np.int64)))
+    assert vox.min() >= 0 and vox.max() < V
+    assert len(np.unique(vox)) == n
+    # feature 0 is the distance (voxels) to the nearest triangle of the voxel's
+    # supercell list: candidates lie within 0.1 voxel of the surface, and the
+    # list holds the nearest triangle unless it only grazes a neighbouring cell
+    assert np.all(np.isfinite(feats))
+    assert np.all(feats[:, 0] >= 0)
+    assert np.mean(feats[:, 0] <= 0.1 + 1e-5) >= 0.99
+    assert set(np.unique(feats[:, 1])) <= {0.0, 1.0}
