@@ -86,11 +86,19 @@ typedef struct rhz_sdf_params {
     int32_t fsm_round_cap;        /* cap for RHZ_FSM_CONVERGE */
     int32_t winding_precision;    /* enum rhz_winding_precision */
     float fp64_band;              /* 1e-3 */
-    float reprobe_fraction;       /* FAST_LEARNED: top-ranked fraction of all voxels re-probed exactly */
-    int32_t learned_max_patch_gens; /* FAST_LEARNED: mismatch-region growth generations */
-    float candidate_radius;       /* FAST_LEARNED: candidates have band distance <= this many voxels */
+    float reprobe_fraction;       /* FAST_LEARNED: top-ranked fraction of all voxels re-probed
+                                     exactly (1e-3); < 0: every candidate the scorer calls
+                                     positive (logit >= 0, margin >= 0) */
+    int32_t learned_max_patch_gens; /* FAST_LEARNED: generations of 26-neighbour growth around
+                                     mismatches (0: off; the certificate below makes it redundant) */
+    float candidate_radius;       /* FAST_LEARNED: candidates have band distance <= this many
+                                     voxels; 0 = the jitter reach |(jitter_x, jitter_y)| (+0.1%),
+                                     beyond which a scanline sign cannot be wrong */
     int32_t scorer;               /* FAST_LEARNED: 0 proposer network, 1 planar-margin rule,
                                      2 stratified (no ranking: every k-th candidate) */
+    int32_t learned_certify;      /* FAST_LEARNED: 1 = also probe every candidate whose voxel-to-ray
+                                     segment may cross a triangle and that the ranking missed
+                                     (0 sign errors on closed meshes); 0 = ranked probes only */
     const float* mlp_weights;     /* FAST_LEARNED: proposer parameters, rhz_mlp_blob_count()
                                      floats (host memory); NULL = the built-in trained set */
 } rhz_sdf_params;
@@ -134,6 +142,11 @@ typedef struct rhz_sdf_stats {
     double ms_proposer;           /* FAST_LEARNED: features + network + selection */
     int64_t winding_calls;        /* bracketed exact-winding launches behind ms_winding */
     int64_t sweep_calls;          /* bracketed sweep rounds behind ms_sweeps (6 sweeps each) */
+    int64_t ranked_probes;        /* FAST_LEARNED: exact probes chosen by the scorer */
+    int64_t ranked_mismatches;    /* FAST_LEARNED: scanline signs those probes corrected */
+    int64_t risky_candidates;     /* FAST_LEARNED: candidates the segment certificate flags */
+    int64_t fallback_probes;      /* FAST_LEARNED: flagged candidates the ranking missed */
+    int64_t fallback_mismatches;  /* FAST_LEARNED: scanline signs the fallback corrected */
 } rhz_sdf_stats;
 
 void rhz_sdf_params_default(rhz_sdf_params* p);
@@ -208,12 +221,16 @@ int rhz_mlp_grad(const float* d_blob, const float* d_feats, const float* d_label
 int rhz_mlp_adam(float* d_blob, const float* d_grads, float* d_m, float* d_v, int32_t n_params, float lr,
                  float beta1, float beta2, float eps, int32_t step, float grad_scale, float weight_decay,
                  void* stream);
-/* Candidate voxels and their 12 features as the FAST_LEARNED method sees
- * them (after splat, signed scanline pass and fixup): up to cap voxel ids and
- * cap x 12 features into device buffers; *n_out = total count (may exceed cap). */
+/* Candidate voxels, their 12 features and certificate flags as the
+ * FAST_LEARNED method sees them (after splat, signed scanline pass and
+ * fixup): up to cap voxel ids, cap x 12 features and cap flags (1 = the
+ * voxel-to-ray segment may cross a triangle) into device buffers (any may be
+ * NULL); d_scan (optional, V bytes) receives the scanline signs (1 inside);
+ * *n_out = total count (may exceed cap). */
 int rhz_learned_candidates(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],
                            const int32_t max_c[3], const rhz_sdf_params* params, int64_t* d_voxels,
-                           float* d_feats, int64_t cap, int64_t* n_out, void* stream);
+                           float* d_feats, uint8_t* d_risk, uint8_t* d_scan, int64_t cap, int64_t* n_out,
+                           void* stream);
 
 /* ---- stage-level entry points (device pointers, one stream) -------------
  * Each runs one stage of the fast path on device buffers; the pipeline above

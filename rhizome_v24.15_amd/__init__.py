@@ -68,6 +68,7 @@ class Params(C.Structure):
         ("learned_max_patch_gens", C.c_int32),
         ("candidate_radius", C.c_float),
         ("scorer", C.c_int32),
+        ("learned_certify", C.c_int32),
         ("mlp_weights", C.c_void_p),
     ]
 
@@ -110,6 +111,11 @@ class Stats(C.Structure):
         ("ms_proposer", C.c_double),
         ("winding_calls", C.c_int64),
         ("sweep_calls", C.c_int64),
+        ("ranked_probes", C.c_int64),
+        ("ranked_mismatches", C.c_int64),
+        ("risky_candidates", C.c_int64),
+        ("fallback_probes", C.c_int64),
+        ("fallback_mismatches", C.c_int64),
     ]
 
     def as_dict(self):
@@ -183,7 +189,7 @@ def lib():
     L.rhz_mlp_forward.argtypes = [vp, vp, i64, vp, vp]
     L.rhz_mlp_grad.argtypes = [vp, vp, vp, i64, f32, vp, vp, vp]
     L.rhz_mlp_adam.argtypes = [vp, vp, vp, vp, i32, f32, f32, f32, f32, i32, f32, f32, vp]
-    L.rhz_learned_candidates.argtypes = [vp, i64, f32, vp, vp, C.POINTER(Params), vp, vp, i64,
+    L.rhz_learned_candidates.argtypes = [vp, i64, f32, vp, vp, C.POINTER(Params), vp, vp, vp, vp, i64,
                                          C.POINTER(C.c_int64), vp]
     _lib = L
     return L
@@ -434,13 +440,15 @@ class Proposer:
                                  float(weight_decay), stream), "rhz_mlp_adam")
 
     @staticmethod
-    def candidates(d_tris, n_tris, h, min_c, max_c, d_voxels, d_feats, cap, params: Params | None = None,
-                   stream=None, **kw):
+    def candidates(d_tris, n_tris, h, min_c, max_c, d_voxels, d_feats, cap, d_risk=None, d_scan=None,
+                   params: Params | None = None, stream=None, **kw):
         p = params if params is not None else default_params(**kw)
         n = C.c_int64(0)
-        check(lib().rhz_learned_candidates(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c).ctypes.data,
-                                           _i32x3(max_c).ctypes.data, C.byref(p),
-                                           _ptr(d_voxels) if d_voxels is not None else None,
-                                           _ptr(d_feats) if d_feats is not None else None, int(cap), C.byref(n),
-                                           stream), "rhz_learned_candidates")
+        opt = lambda a: _ptr(a) if a is not None else None  # noqa: E731
+        rc = check(lib().rhz_learned_candidates(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c).ctypes.data,
+                                                _i32x3(max_c).ctypes.data, C.byref(p), opt(d_voxels), opt(d_feats),
+                                                opt(d_risk), opt(d_scan), int(cap), C.byref(n), stream),
+                   "rhz_learned_candidates")
+        if rc != RHZ_OK:
+            raise RhzError(f"rhz_learned_candidates declined ({rc})")
         return n.value

@@ -76,8 +76,32 @@ __device__ __forceinline__ int closest_region(float px, float py, float pz, cons
     return 0;
 }
 
+// Does the horizontal segment from the voxel centre P0 to its jittered ray
+// point P1 = P0 + (jx, jy, 0) h possibly cross triangle t?  The scanline
+// pass classified P1; the winding number of P0 equals that of P1 unless the
+// segment crosses the surface.  Conservative: "no" only when both ends lie
+// beyond eps on one side of t's plane, or the segment meets the plane more
+// than eps away from t.  Zero-area triangles (zero normal) bound no volume.
+__device__ __forceinline__ bool segment_may_cross(float px, float py, float pz, float rx, float ry, const Tri9& t,
+                                                  float nx, float ny, float nz, float eps, float half_len) {
+    if (nx == 0.0f && ny == 0.0f && nz == 0.0f) return false;
+    const float s0 = nx * (px - t.ax) + ny * (py - t.ay) + nz * (pz - t.az);
+    const float s1 = nx * (rx - t.ax) + ny * (ry - t.ay) + nz * (pz - t.az);
+    if ((s0 > eps && s1 > eps) || (s0 < -eps && s1 < -eps)) return false;
+    const bool flat = fabsf(s0) <= eps && fabsf(s1) <= eps;  // segment (nearly) in the plane
+    float lam = 0.5f;
+    if (!flat) {
+        const float den = s0 - s1;
+        lam = (fabsf(den) > 0.0f) ? fminf(fmaxf(s0 / den, 0.0f), 1.0f) : 0.5f;
+    }
+    const float qx = px + lam * (rx - px), qy = py + lam * (ry - py);
+    const float r = flat ? (half_len + eps) : eps;
+    return point_tri_dist2(qx, qy, pz, t) <= r * r;
+}
+
 // Features of candidate voxel v (nearest triangles searched in the voxel's
-// scanline supercell list):
+// scanline supercell list), and risk[i] = 1 when the segment certificate
+// above cannot rule out a scanline sign error:
 //  0 d/h  1 scan sign  2-4 nearest-triangle normal  5 jitter offset along the
 //  normal (voxels)  6 planar margin  7 second-nearest distance/h
 //  8 log1p(#triangles within 1 h)  9 normal spread within 1.5 h
@@ -87,7 +111,8 @@ __global__ void features_kernel(const float* __restrict__ tris, const float4* __
                                 const int64_t* __restrict__ start, const int32_t* __restrict__ ids,
                                 const uint8_t* __restrict__ sign, const int64_t* __restrict__ list, int64_t n,
                                 int mnx, int mny, int mnz, int dx, int dy, float h, int sup, int sdx, float jx,
-                                float jy, float* __restrict__ feats) {
+                                float jy, float* __restrict__ feats, uint8_t* __restrict__ risk,
+                                unsigned long long* __restrict__ n_risky) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t v = list[i];
@@ -105,11 +130,18 @@ __global__ void features_kernel(const float* __restrict__ tris, const float4* __
     float sx = 0.f, sy = 0.f, sz = 0.f;
     int spread_cnt = 0;
     float zhit = 2.0f * h;
+    bool risky = false;
+    const float eps = 2e-3f * h;
+    const float half_len = 0.5f * sqrtf(jx * jx + jy * jy) * h;
     const int64_t s = (int64_t)(y / sup) * sdx + (x / sup);
     for (int64_t e = start[s]; e < start[s + 1]; ++e) {
         const int ti = ids[e];
         const Tri9 t = load_tri(tris, ti);
         const float d2 = point_tri_dist2(px, py, pz, t);
+        if (!risky && d2 <= (2.0f * half_len + eps) * (2.0f * half_len + eps)) {
+            const float4 tn = reinterpret_cast<const float4*>(tris + 12 * (int64_t)ti)[2];
+            risky = segment_may_cross(px, py, pz, rx, ry, t, tn.y, tn.z, tn.w, eps, half_len);
+        }
         if (d2 < best) {
             second = best;
             best = d2;
@@ -168,6 +200,29 @@ __global__ void features_kernel(const float* __restrict__ tris, const float4* __
     f[9] = spread;
     f[10] = 0.5f * (float)region;
     f[11] = fminf(zhit / h, 2.0f);
+    risk[i] = risky ? 1 : 0;
+    wave_append(n_risky, risky);
+}
+
+// Risky candidates the ranked probes did not cover (the certificate's
+// fallback list).
+__global__ void unprobed_risky_kernel(const int64_t* __restrict__ list, const uint8_t* __restrict__ risk, int64_t n,
+                                      const unsigned* __restrict__ bits, int64_t* __restrict__ out, int64_t cap,
+                                      unsigned long long* counter) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t iters = (n + stride - 1) / stride;
+    for (int64_t it = 0; it < iters; ++it) {
+        const int64_t i = tid + it * stride;
+        bool want = false;
+        int64_t v = 0;
+        if (i < n && risk[i]) {
+            v = list[i];
+            want = ((bits[v >> 5] >> (v & 31)) & 1u) == 0;
+        }
+        long long slot = wave_append(counter, want);
+        if (want && slot < cap) out[slot] = v;
+    }
 }
 
 // score = planar margin (feature 6): the rule the network starts from
@@ -281,12 +336,20 @@ hipError_t launch_candidates(const float* d, int64_t n, float thr, int64_t* list
 }
 
 hipError_t launch_features(const float* tris, const CsrBuffers& b, const uint8_t* sign, const int64_t* list,
-                           int64_t n, const Grid& g, int sup, float jx, float jy, float* feats, hipStream_t s) {
+                           int64_t n, const Grid& g, int sup, float jx, float jy, float* feats, uint8_t* risk,
+                           unsigned long long* n_risky, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int sdx = (g.dims[0] + sup - 1) / sup;
     features_kernel<<<(unsigned)((n + 127) / 128), 128, 0, s>>>(tris, b.xyab, b.start, b.ids, sign, list, n, g.mn[0],
                                                               g.mn[1], g.mn[2], g.dims[0], g.dims[1], g.h, sup, sdx,
-                                                              jx, jy, feats);
+                                                              jx, jy, feats, risk, n_risky);
+    return hipGetLastError();
+}
+
+hipError_t launch_unprobed_risky(const int64_t* list, const uint8_t* risk, int64_t n, const unsigned* bits,
+                                 int64_t* out, int64_t cap, unsigned long long* counter, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    unprobed_risky_kernel<<<grid_for(n), kBlock, 0, s>>>(list, risk, n, bits, out, cap, counter);
     return hipGetLastError();
 }
 

@@ -108,7 +108,10 @@ hipError_t launch_candidates(const float* d, int64_t n, float thr, int64_t* list
                              unsigned long long* counter, hipStream_t s);
 // 12 features per candidate (row-major n x 12) from the column candidate lists.
 hipError_t launch_features(const float* tris, const CsrBuffers& b, const uint8_t* sign, const int64_t* list,
-                           int64_t n, const Grid& g, int sup, float jx, float jy, float* feats, hipStream_t s);
+                           int64_t n, const Grid& g, int sup, float jx, float jy, float* feats, uint8_t* risk,
+                           unsigned long long* n_risky, hipStream_t s);
+hipError_t launch_unprobed_risky(const int64_t* list, const uint8_t* risk, int64_t n, const unsigned* bits,
+                                 int64_t* out, int64_t cap, unsigned long long* counter, hipStream_t s);
 hipError_t launch_margin_score(const float* feats, int64_t n, float* score, hipStream_t s);
 int score_bins();
 hipError_t launch_score_hist(const float* score, int64_t n, float lo, float hi, unsigned* hist, hipStream_t s);

@@ -85,6 +85,7 @@ enum Slot {
     S_MISM,        // mismatch list
     S_HIST,        // score histogram
     S_MLP,         // proposer parameters (device copy)
+    S_RISK,        // per-candidate segment-certificate flag
     S_COUNT
 };
 
@@ -139,7 +140,7 @@ void* ws_get(Workspace* w, int slot, size_t bytes) {
 }
 
 // counters: ull slots 0..15, int slots at byte offset 128 (32 ints)
-enum CounterId { C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4, C_SEL = 5, C_GROW = 6 };
+enum CounterId { C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4, C_SEL = 5, C_GROW = 6, C_RISKY = 7 };
 enum FlagId { F_UNKNOWN = 0, F_CHANGED = 1 };
 
 unsigned long long* cnt_ptr(Workspace* w, int id) { return w->d_cnt + id; }
@@ -602,12 +603,22 @@ int probe_commit(Ctx& c, const int64_t* list, int64_t n, uint8_t* sign, int64_t*
     return RHZ_OK;
 }
 
-// Candidates + features (after splat, signed scan and fixup).  On return
-// *cand / *feats are workspace buffers holding *n_cand rows.
-int learned_candidates(Ctx& c, uint8_t* sign, const CsrBuffers& b, int64_t** cand, float** feats,
-                       int64_t* n_cand) {
+// Band distance (voxels) beyond which a scanline sign cannot be wrong: the
+// scanline pass classifies the point (jx, jy) h away from the voxel centre,
+// and the two can only differ if the segment between them meets the surface.
+float candidate_radius(const rhz_sdf_params* p) {
+    if (p->candidate_radius > 0.0f) return p->candidate_radius;
+    const double reach = std::sqrt((double)p->jitter_x * p->jitter_x + (double)p->jitter_y * p->jitter_y);
+    return (float)(reach * 1.001 + 1e-4);
+}
+
+// Candidates + features + certificate flags (after splat, signed scan and
+// fixup).  On return *cand / *feats / *risk are workspace buffers holding
+// *n_cand rows; *n_risky counts the flagged ones.
+int learned_candidates(Ctx& c, uint8_t* sign, const CsrBuffers& b, int64_t** cand, float** feats, uint8_t** risk,
+                       int64_t* n_cand, int64_t* n_risky) {
     const int64_t V = c.g.n();
-    const float thr = c.p->candidate_radius * c.g.h;
+    const float thr = candidate_radius(c.p) * c.g.h;
     int64_t cap = (int64_t)(c.w->bytes[S_CAND] / sizeof(int64_t));
     cap = std::max<int64_t>(cap, std::max<int64_t>(1 << 20, V / 512));
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -620,11 +631,18 @@ int learned_candidates(Ctx& c, uint8_t* sign, const CsrBuffers& b, int64_t** can
         const int64_t n = (int64_t)hc(c, C_LIST);
         if (n <= cap) {
             float* f = (float*)ws_get(c.w, S_FEAT, sizeof(float) * 12 * (size_t)std::max<int64_t>(n, 1));
-            if (!f) return set_err(RHZ_ERR_NOMEM, "feature buffer allocation failed");
-            HIP_TRY(launch_features(c.tris, b, sign, list, n, c.g, c.p->sup, c.p->jitter_x, c.p->jitter_y, f, c.s));
+            uint8_t* r = (uint8_t*)ws_get(c.w, S_RISK, (size_t)std::max<int64_t>(n, 1));
+            if (!f || !r) return set_err(RHZ_ERR_NOMEM, "feature buffer allocation failed");
+            HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_RISKY), 0, sizeof(unsigned long long), c.s));
+            HIP_TRY(launch_features(c.tris, b, sign, list, n, c.g, c.p->sup, c.p->jitter_x, c.p->jitter_y, f, r,
+                                    cnt_ptr(c.w, C_RISKY), c.s));
+            rc = read_counters(c);
+            if (rc) return rc;
             *cand = list;
             *feats = f;
+            *risk = r;
             *n_cand = n;
+            *n_risky = (int64_t)hc(c, C_RISKY);
             return RHZ_OK;
         }
         cap = n;
@@ -644,7 +662,7 @@ int learned_select(Ctx& c, const int64_t* cand, const float* feats, int64_t n, i
         *n_sel = 0;
         return RHZ_OK;
     }
-    if (k >= n || p->scorer == 2) {
+    if (k >= n || (p->scorer == 2 && k >= 0)) {
         // everything, or a stratified subset without ranking (every n/k-th candidate)
         const int64_t m = (k >= n) ? n : k;
         if (m == n) {
@@ -688,17 +706,22 @@ int learned_select(Ctx& c, const int64_t* cand, const float* feats, int64_t n, i
         lo = -0.5f;
         hi = 0.25f;
     }
-    HIP_TRY(launch_score_hist(score, n, lo, hi, hist, c.s));
-    std::vector<unsigned> hh((size_t)score_bins());
-    HIP_TRY(hipMemcpyAsync(hh.data(), hist, sizeof(unsigned) * hh.size(), hipMemcpyDeviceToHost, c.s));
-    HIP_TRY(hipStreamSynchronize(c.s));
     int min_bin = 0;
-    int64_t acc = 0;
-    for (int bi = score_bins() - 1; bi >= 0; --bi) {
-        acc += hh[(size_t)bi];
-        if (acc >= k) {
-            min_bin = bi;
-            break;
+    if (k < 0) {
+        // classifier mode: every candidate scored positive (bin of score 0)
+        min_bin = (int)std::floor((0.0 - lo) * score_bins() / (hi - lo));
+    } else {
+        HIP_TRY(launch_score_hist(score, n, lo, hi, hist, c.s));
+        std::vector<unsigned> hh((size_t)score_bins());
+        HIP_TRY(hipMemcpyAsync(hh.data(), hist, sizeof(unsigned) * hh.size(), hipMemcpyDeviceToHost, c.s));
+        HIP_TRY(hipStreamSynchronize(c.s));
+        int64_t acc = 0;
+        for (int bi = score_bins() - 1; bi >= 0; --bi) {
+            acc += hh[(size_t)bi];
+            if (acc >= k) {
+                min_bin = bi;
+                break;
+            }
         }
     }
     HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_SEL), 0, sizeof(unsigned long long), c.s));
@@ -748,11 +771,15 @@ int run_learned(Ctx& c) {
     tm.mark(3, c.s);
     int64_t* cand = nullptr;
     float* feats = nullptr;
-    int64_t n_cand = 0;
-    rc = learned_candidates(c, sign, b, &cand, &feats, &n_cand);
+    uint8_t* risk = nullptr;
+    int64_t n_cand = 0, n_risky = 0;
+    rc = learned_candidates(c, sign, b, &cand, &feats, &risk, &n_cand, &n_risky);
     if (rc) return rc;
     st->candidates = n_cand;
-    const int64_t k = std::max<int64_t>(1, (int64_t)((double)p->reprobe_fraction * (double)V));
+    st->risky_candidates = n_risky;
+    const int64_t k = (p->reprobe_fraction < 0.0f)
+                          ? -1
+                          : std::max<int64_t>(1, (int64_t)((double)p->reprobe_fraction * (double)V));
     int64_t* sel = nullptr;
     int64_t n_sel = 0;
     rc = learned_select(c, cand, feats, n_cand, k, &sel, &n_sel);
@@ -766,6 +793,7 @@ int run_learned(Ctx& c) {
     int64_t* cur = sel;
     int64_t n_cur = n_sel;
     int gens = 0;
+    st->ranked_probes = n_sel;
     while (n_cur > 0) {
         HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_MISM), 0, sizeof(unsigned long long), c.s));
         rc = probe_commit(c, cur, n_cur, sign, mism, mism_cap);
@@ -776,6 +804,7 @@ int run_learned(Ctx& c) {
         const int64_t m = (int64_t)hc(c, C_MISM);
         if (m > mism_cap) return set_err(RHZ_ERR_HIP, "mismatch list overflow");
         st->mismatches_found += m;
+        if (gens == 0) st->ranked_mismatches = m;
         if (m == 0 || gens >= p->learned_max_patch_gens) break;
         // grow into the 26-neighbourhood of this generation's mismatches
         const int64_t gcap = 26 * m;
@@ -795,6 +824,32 @@ int run_learned(Ctx& c) {
         ++gens;
     }
     st->patch_rounds = gens;
+    if (p->learned_certify && n_risky > 0) {
+        // certificate fallback: every flagged candidate not probed yet
+        int64_t* fb = (int64_t*)ws_get(c.w, S_SEL, sizeof(int64_t) * (size_t)n_risky);
+        if (!fb) return set_err(RHZ_ERR_NOMEM, "fallback list allocation failed");
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_GROW), 0, sizeof(unsigned long long), c.s));
+        HIP_TRY(launch_unprobed_risky(cand, risk, n_cand, bits, fb, n_risky, cnt_ptr(c.w, C_GROW), c.s));
+        rc = read_counters(c);
+        if (rc) return rc;
+        const int64_t n_fb = std::min<int64_t>((int64_t)hc(c, C_GROW), n_risky);
+        st->fallback_probes = n_fb;
+        if (n_fb > 0) {
+            if (n_fb > mism_cap) {
+                mism_cap = n_fb;
+                mism = (int64_t*)ws_get(c.w, S_MISM, sizeof(int64_t) * (size_t)mism_cap);
+                if (!mism) return set_err(RHZ_ERR_NOMEM, "mismatch list allocation failed");
+            }
+            HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_MISM), 0, sizeof(unsigned long long), c.s));
+            rc = probe_commit(c, fb, n_fb, sign, mism, mism_cap);
+            if (rc) return rc;
+            rc = read_counters(c);
+            if (rc) return rc;
+            st->fallback_mismatches = (int64_t)hc(c, C_MISM);
+            st->mismatches_found += st->fallback_mismatches;
+            st->reprobed_voxels += n_fb;
+        }
+    }
     st->sites = (int32_t)std::min<int64_t>(st->mismatches_found, INT32_MAX);
     st->probes = 1;
     tm.mark(5, c.s);
@@ -928,9 +983,10 @@ void rhz_sdf_params_default(rhz_sdf_params* p) {
     p->winding_precision = RHZ_WINDING_FP32_KAHAN;
     p->fp64_band = 1e-3f;
     p->reprobe_fraction = 1e-3f;
-    p->learned_max_patch_gens = 64;
-    p->candidate_radius = 0.1f;
+    p->learned_max_patch_gens = 0;
+    p->candidate_radius = 0.0f;
     p->scorer = 0;
+    p->learned_certify = 1;
     p->mlp_weights = nullptr;
 }
 
@@ -1382,7 +1438,7 @@ int rhz_mlp_adam(float* d_blob, const float* d_grads, float* d_m, float* d_v, in
 
 int rhz_learned_candidates(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],
                            const int32_t max_c[3], const rhz_sdf_params* params, int64_t* d_voxels, float* d_feats,
-                           int64_t cap, int64_t* n_out, void* stream) {
+                           uint8_t* d_risk, uint8_t* d_scan, int64_t cap, int64_t* n_out, void* stream) {
     int rc = need_device();
     if (rc) return rc;
     rc = check_params(params);
@@ -1417,12 +1473,15 @@ int rhz_learned_candidates(const rhz_triangle* d_tris, int64_t n_tris, float h, 
     }
     int64_t* cand = nullptr;
     float* feats = nullptr;
-    int64_t n = 0;
-    rc = learned_candidates(c, sign, b, &cand, &feats, &n);
+    uint8_t* risk = nullptr;
+    int64_t n = 0, n_risky = 0;
+    rc = learned_candidates(c, sign, b, &cand, &feats, &risk, &n, &n_risky);
     if (rc) return rc;
     const int64_t k = std::min(n, cap);
     if (k > 0 && d_voxels) HIP_TRY(hipMemcpyAsync(d_voxels, cand, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToDevice, c.s));
     if (k > 0 && d_feats) HIP_TRY(hipMemcpyAsync(d_feats, feats, sizeof(float) * 12 * (size_t)k, hipMemcpyDeviceToDevice, c.s));
+    if (k > 0 && d_risk) HIP_TRY(hipMemcpyAsync(d_risk, risk, (size_t)k, hipMemcpyDeviceToDevice, c.s));
+    if (d_scan) HIP_TRY(hipMemcpyAsync(d_scan, sign, (size_t)V, hipMemcpyDeviceToDevice, c.s));
     HIP_TRY(hipStreamSynchronize(c.s));
     *n_out = n;
     return RHZ_OK;

@@ -174,26 +174,73 @@ def test_learned_vessel_coarse(rhz, geom):
     print(st.log_line(), "reprobed", st.reprobed_voxels, "mismatches", st.mismatches_found)
 
 
-def test_learned_candidates_api(rhz, torch_gpu, small_sphere):
+def _plain_signs(rhz, tris, h, mn, mx):
+    rc, ref, _ = rhz.compute_sdf(tris, h, mn, mx, rhz.default_params(method=rhz.METHOD_PLAIN_EXACT))
+    assert rc == rhz.RHZ_OK
+    return ref.reshape(-1)
+
+
+@pytest.mark.parametrize("case", ["small_sphere", "config1", "vessel_1mm"])
+def test_certificate_covers_every_scanline_error(rhz, geom, torch_gpu, request, case):
+    """Every voxel whose scanline sign differs from the exact winding sign is
+    a candidate (band distance within the jitter reach) and is flagged by the
+    segment certificate; the flag is a small superset of the errors."""
     torch = torch_gpu
-    tris, h, mn, mx = small_sphere
+    if case == "vessel_1mm":
+        v, f = geom.vessel()
+        tris = rhz.build_triangles(v, f)
+        h = 1.0
+        mn, mx = geom.frame_grid(v, h)
+    else:
+        tris, h, mn, mx = request.getfixturevalue(case)
+    V = int(np.prod(dims_of(mn, mx).astype(np.int64)))
     d_t = torch.from_numpy(tris).cuda()
     n = rhz.Proposer.candidates(d_t, len(tris), h, mn, mx, None, None, 0)
     assert n > 0
     d_v = torch.zeros(n, dtype=torch.int64, device="cuda")
     d_f = torch.zeros((n, 12), dtype=torch.float32, device="cuda")
-    n2 = rhz.Proposer.candidates(d_t, len(tris), h, mn, mx, d_v, d_f, n)
+    d_r = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    d_s = torch.zeros(V, dtype=torch.uint8, device="cuda")
+    n2 = rhz.Proposer.candidates(d_t, len(tris), h, mn, mx, d_v, d_f, n, d_risk=d_r, d_scan=d_s)
     torch.cuda.synchronize()
     assert n2 == n
     vox = d_v.cpu().numpy()
     feats = d_f.cpu().numpy()
-    V = int(np.prod(dims_of(mn, mx).astype(np.int64)))
-    assert vox.min() >= 0 and vox.max() < V
-    assert len(np.unique(vox)) == n
-    # feature 0 is the distance (voxels) to the nearest triangle of the voxel's
-    # supercell list: candidates lie within 0.1 voxel of the surface, and the
-    # list holds the nearest triangle unless it only grazes a neighbouring cell
-    assert np.all(np.isfinite(feats))
-    assert np.all(feats[:, 0] >= 0)
-    assert np.mean(feats[:, 0] <= 0.1 + 1e-5) >= 0.99
+    risk = d_r.cpu().numpy().astype(bool)
+    scan = d_s.cpu().numpy()
+    assert vox.min() >= 0 and vox.max() < V and len(np.unique(vox)) == n
+    assert np.all(np.isfinite(feats)) and np.all(feats[:, 0] >= 0)
     assert set(np.unique(feats[:, 1])) <= {0.0, 1.0}
+    assert set(np.unique(scan)) <= {0, 1}
+    ref = _plain_signs(rhz, tris, h, mn, mx)
+    exact = (~np.signbit(ref)).astype(np.uint8) ^ 1  # 1 inside
+    on_surface = np.abs(ref) <= 1e-5 * h
+    wrong = (scan != exact) & ~on_surface
+    is_cand = np.zeros(V, bool)
+    is_cand[vox] = True
+    flagged = np.zeros(V, bool)
+    flagged[vox[risk]] = True
+    assert not np.any(wrong & ~is_cand), "scanline error outside the candidate set"
+    assert not np.any(wrong & ~flagged), f"{int(np.sum(wrong & ~flagged))} errors not flagged"
+    # the flag is selective: at most 2x the actual errors plus a few
+    print(case, "candidates", n, "flagged", int(risk.sum()), "errors", int(wrong.sum()))
+    assert risk.sum() <= 2 * wrong.sum() + 16
+
+
+def test_ranked_probes_alone_can_miss(rhz, small_sphere):
+    """Without the certificate a blind (stratified) probe budget leaves
+    scanline errors behind; with it the result is exact again."""
+    tris, h, mn, mx = small_sphere
+    ref = _plain_signs(rhz, tris, h, mn, mx)
+    on_surface = np.abs(ref) <= 1e-5 * h
+    res = {}
+    for scorer in (1, 2):
+        p = rhz.default_params(method=rhz.METHOD_FAST_LEARNED, scorer=scorer, learned_certify=0,
+                               reprobe_fraction=5e-4)
+        rc, got, st = rhz.compute_sdf(tris, h, mn, mx, p)
+        assert rc == rhz.RHZ_OK
+        res[scorer] = int(np.sum((np.signbit(got.reshape(-1)) != np.signbit(ref)) & ~on_surface))
+        assert st.fallback_probes == 0
+    # the planar-margin ranking leaves fewer errors than blind stratified probes
+    assert res[2] > 0
+    assert res[1] < res[2], res
