@@ -87,8 +87,8 @@ typedef struct rhz_sdf_params {
     int32_t winding_precision;    /* enum rhz_winding_precision */
     float fp64_band;              /* 1e-3 */
     float reprobe_fraction;       /* FAST_LEARNED: top-ranked fraction of all voxels re-probed
-                                     exactly (1e-3); < 0: every candidate the scorer calls
-                                     positive (logit >= 0, margin >= 0) */
+                                     exactly (e.g. 1e-3); < 0 (default): every candidate the
+                                     scorer calls positive (logit >= 0, margin >= 0) */
     int32_t learned_max_patch_gens; /* FAST_LEARNED: generations of 26-neighbour growth around
                                      mismatches (0: off; the certificate below makes it redundant) */
     float candidate_radius;       /* FAST_LEARNED: candidates have band distance <= this many

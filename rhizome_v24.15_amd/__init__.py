@@ -213,6 +213,14 @@ def default_params(**kw) -> Params:
 
 
 def _i32x3(v):
+    """Three int32 values as a ctypes array: ctypes keeps it alive for the
+    duration of the call it is passed to (an address taken from a temporary
+    numpy array would not be)."""
+    a = np.asarray(v, dtype=np.int64).reshape(3)
+    return (C.c_int32 * 3)(int(a[0]), int(a[1]), int(a[2]))
+
+
+def _np3(v):
     return np.ascontiguousarray(np.asarray(v, dtype=np.int32).reshape(3))
 
 
@@ -236,7 +244,7 @@ def compute_sdf(tris, h, min_c, max_c, params: Params | None = None, **kw):
     RHZ_OK, None when the method declined."""
     p = params if params is not None else default_params(**kw)
     t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 12)
-    mn, mx = _i32x3(min_c), _i32x3(max_c)
+    mn, mx = _np3(min_c), _np3(max_c)  # locals: alive across the call
     dims = mx - mn + 1
     out = np.empty(int(np.prod(dims.astype(np.int64))), dtype=np.float32)
     st = Stats()
@@ -250,7 +258,7 @@ def compute_sdf_device(d_tris, n_tris, h, min_c, max_c, d_out, params: Params | 
     """Device-resident variant: ``d_tris``/``d_out`` are device pointers or
     torch tensors on the GPU."""
     p = params if params is not None else default_params(**kw)
-    mn, mx = _i32x3(min_c), _i32x3(max_c)
+    mn, mx = _np3(min_c), _np3(max_c)  # locals: alive across the call
     st = Stats()
     s = stream.cuda_stream if stream is not None and hasattr(stream, "cuda_stream") else stream
     rc = check(lib().rhz_sdf_compute_device(_ptr(d_tris), int(n_tris), float(h), mn.ctypes.data, mx.ctypes.data,
@@ -330,7 +338,7 @@ class Stages:
     @staticmethod
     def splat(d_tris, n_tris, h, band_width, min_c, dims, d, stream=None):
         check(lib().rhz_stage_splat(_ptr(d_tris), int(n_tris), float(h), float(band_width),
-                                    _i32x3(min_c).ctypes.data, _i32x3(dims).ctypes.data, _ptr(d), stream),
+                                    _i32x3(min_c), _i32x3(dims), _ptr(d), stream),
               "rhz_stage_splat")
 
     @staticmethod
@@ -339,27 +347,27 @@ class Stages:
         flag = C.c_int32(0)
         ent = C.c_int64(0)
         rc = check(lib().rhz_stage_parity(_ptr(d_tris), int(n_tris), float(h), float(jitter[0]), float(jitter[1]),
-                                          int(max_hits), int(sup), int(csr_max_entries), _i32x3(min_c).ctypes.data,
-                                          _i32x3(dims).ctypes.data, _ptr(d_sign), C.byref(flag), C.byref(ent),
+                                          int(max_hits), int(sup), int(csr_max_entries), _i32x3(min_c),
+                                          _i32x3(dims), _ptr(d_sign), C.byref(flag), C.byref(ent),
                                           stream), "rhz_stage_parity")
         return rc, flag.value, ent.value
 
     @staticmethod
     def fixup(d_tris, n_tris, h, min_c, dims, d_sign, precision=WINDING_FP32_KAHAN, stream=None):
         k = C.c_int64(0)
-        check(lib().rhz_stage_fixup(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c).ctypes.data,
-                                    _i32x3(dims).ctypes.data, int(precision), _ptr(d_sign), C.byref(k), stream),
+        check(lib().rhz_stage_fixup(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c),
+                                    _i32x3(dims), int(precision), _ptr(d_sign), C.byref(k), stream),
               "rhz_stage_fixup")
         return k.value
 
     @staticmethod
     def sweep(src, dst, dims, h, axis, direction, d_changed, stream=None):
-        check(lib().rhz_stage_sweep(_ptr(src), _ptr(dst), _i32x3(dims).ctypes.data, float(h), int(axis),
+        check(lib().rhz_stage_sweep(_ptr(src), _ptr(dst), _i32x3(dims), float(h), int(axis),
                                     int(direction), _ptr(d_changed), stream), "rhz_stage_sweep")
 
     @staticmethod
     def sweep_round(a, b, dims, h, d_changed, stream=None):
-        check(lib().rhz_stage_sweep_round(_ptr(a), _ptr(b), _i32x3(dims).ctypes.data, float(h), _ptr(d_changed),
+        check(lib().rhz_stage_sweep_round(_ptr(a), _ptr(b), _i32x3(dims), float(h), _ptr(d_changed),
                                           stream), "rhz_stage_sweep_round")
 
     @staticmethod
@@ -377,15 +385,15 @@ class Stages:
     def verify(d_tris, n_tris, h, min_c, dims, samples, gen, d_sign, d_sites, precision=WINDING_FP32_KAHAN,
                stream=None):
         m = C.c_int32(0)
-        check(lib().rhz_stage_verify(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c).ctypes.data,
-                                     _i32x3(dims).ctypes.data, int(samples), int(gen), int(precision), _ptr(d_sign),
+        check(lib().rhz_stage_verify(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c),
+                                     _i32x3(dims), int(samples), int(gen), int(precision), _ptr(d_sign),
                                      _ptr(d_sites), C.byref(m), stream), "rhz_stage_verify")
         return m.value
 
     @staticmethod
     def mark_patch(d_sign, d_sites, n_sites, radius, dims, stream=None):
         check(lib().rhz_stage_mark_patch(_ptr(d_sign), _ptr(d_sites), int(n_sites), int(radius),
-                                         _i32x3(dims).ctypes.data, stream), "rhz_stage_mark_patch")
+                                         _i32x3(dims), stream), "rhz_stage_mark_patch")
 
     @staticmethod
     def apply(d, sign, n, stream=None):
@@ -394,8 +402,8 @@ class Stages:
     @staticmethod
     def winding_voxels(d_tris, n_tris, h, min_c, dims, d_voxels, n_voxels, d_w, precision=WINDING_FP32_KAHAN,
                        stream=None):
-        check(lib().rhz_stage_winding_voxels(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c).ctypes.data,
-                                             _i32x3(dims).ctypes.data, _ptr(d_voxels), int(n_voxels),
+        check(lib().rhz_stage_winding_voxels(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c),
+                                             _i32x3(dims), _ptr(d_voxels), int(n_voxels),
                                              int(precision), _ptr(d_w), stream), "rhz_stage_winding_voxels")
 
 
@@ -445,8 +453,8 @@ class Proposer:
         p = params if params is not None else default_params(**kw)
         n = C.c_int64(0)
         opt = lambda a: _ptr(a) if a is not None else None  # noqa: E731
-        rc = check(lib().rhz_learned_candidates(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c).ctypes.data,
-                                                _i32x3(max_c).ctypes.data, C.byref(p), opt(d_voxels), opt(d_feats),
+        rc = check(lib().rhz_learned_candidates(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c),
+                                                _i32x3(max_c), C.byref(p), opt(d_voxels), opt(d_feats),
                                                 opt(d_risk), opt(d_scan), int(cap), C.byref(n), stream),
                    "rhz_learned_candidates")
         if rc != RHZ_OK:

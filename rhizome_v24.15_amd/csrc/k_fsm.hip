@@ -248,7 +248,17 @@ __global__ __launch_bounds__(64 * kYZRows) void sweep_lane_kernel(const float* _
             g[SF] = a_f;
             g[SW] = a_w;
             g[SC] = a_c;
-            const float nv = godunov_min(cc[k], g[0], g[1], g[2], K.h, K.h2x2, K.h2x3, K.h057, K.hbig);
+            // The Godunov solve runs only when some lane of the wave can improve:
+            // a uniform branch, so converged stretches (most of the far field
+            // after the first round) cost a handful of instructions per cell.
+            // Lanes that skip keep cc[k], which is what godunov_min's own exit
+            // returns for them: bit-identical either way.
+            const float lo = fminf(g[0], fminf(g[1], g[2]));
+            const float hi = fmaxf(g[0], fmaxf(g[1], g[2]));
+            const bool need = (cc[k] > lo + K.h057) || !(hi < K.hbig);
+            float nv = cc[k];
+            if (__ballot(need) != 0)
+                nv = godunov_min(cc[k], g[0], g[1], g[2], K.h, K.h2x2, K.h2x3, K.h057, K.hbig);
             chg |= nv < cc[k];
             if (live) dst[row + fl] = nv;
             behind = nv;

@@ -27,30 +27,72 @@ inline unsigned grid_for(int64_t n, int per = kBlock) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
-__global__ void candidates_kernel(const float* __restrict__ d, int64_t n, float thr, int64_t* __restrict__ list,
-                                  int64_t cap, unsigned long long* counter) {
+// Candidate compaction: a block handles chunks of 256 x 16 float4 (16,384
+// voxels), reads them coalesced, ranks its hits with a block-wide prefix sum
+// and reserves output slots with ONE atomic per chunk (a per-wave atomic on a
+// single counter serialises at the L2 for a sparse but surface-wide list).
+constexpr int kCandBlock = 256;
+constexpr int kCandPer = 16;  // float4 per thread per chunk
+
+__global__ __launch_bounds__(kCandBlock) void candidates_kernel(const float* __restrict__ d, int64_t n, float thr,
+                                                                int64_t* __restrict__ list, int64_t cap,
+                                                                unsigned long long* counter) {
+    __shared__ unsigned wave_tot[kCandBlock / 64];
+    __shared__ unsigned long long chunk_base;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t n4 = n >> 2;
     const float4* d4 = reinterpret_cast<const float4*>(d);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t iters = (n4 + stride - 1) / stride;
-    for (int64_t it = 0; it < iters; ++it) {
-        const int64_t i = tid + it * stride;
-        float4 v = make_float4(1e30f, 1e30f, 1e30f, 1e30f);
-        if (i < n4) v = d4[i];
-        const bool any = (v.x <= thr) | (v.y <= thr) | (v.z <= thr) | (v.w <= thr);
-        if (__ballot(any) == 0) continue;
-        const float vals[4] = {v.x, v.y, v.z, v.w};
-        for (int j = 0; j < 4; ++j) {
-            const bool want = vals[j] <= thr;
-            long long slot = wave_append(counter, want);
-            if (want && slot < cap) list[slot] = i * 4 + j;
+    const int64_t per_chunk = (int64_t)kCandBlock * kCandPer;
+    const int64_t nchunks = (n4 + per_chunk - 1) / per_chunk;
+    for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const int64_t base4 = ch * per_chunk;
+        unsigned long long m = 0;
+#pragma unroll
+        for (int i = 0; i < kCandPer; ++i) {
+            const int64_t q = base4 + (int64_t)i * kCandBlock + tid;
+            if (q < n4) {
+                const float4 v = d4[q];
+                m |= (unsigned long long)((v.x <= thr) | ((v.y <= thr) << 1) | ((v.z <= thr) << 2) |
+                                          ((v.w <= thr) << 3))
+                     << (4 * i);
+            }
         }
+        const unsigned cnt = (unsigned)__popcll(m);
+        // inclusive wave scan
+        unsigned incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wave_tot[wv] = incl;
+        __syncthreads();
+        unsigned before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < kCandBlock / 64; ++w) {
+            const unsigned t = wave_tot[w];
+            before += (w < wv) ? t : 0u;
+            total += t;
+        }
+        if (tid == 0) chunk_base = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
+        __syncthreads();
+        unsigned long long pos = chunk_base + before + incl - cnt;
+        while (m) {
+            const int b = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const int i = b >> 2, j = b & 3;
+            if (pos < (unsigned long long)cap) list[pos] = (base4 + (int64_t)i * kCandBlock + tid) * 4 + j;
+            ++pos;
+        }
+        __syncthreads();  // wave_tot / chunk_base are reused by the next chunk
     }
-    const int64_t t = (n4 << 2) + tid;
-    const bool want = (t < n) && (d[t] <= thr);
-    long long slot = wave_append(counter, want);
-    if (want && slot < cap) list[slot] = t;
+    // the n % 4 trailing voxels
+    if (blockIdx.x == 0 && tid < 64) {
+        const int64_t t = (n4 << 2) + tid;
+        const bool want = (t < n) && (d[t] <= thr);
+        long long slot = wave_append(counter, want);
+        if (want && slot < cap) list[slot] = t;
+    }
 }
 
 // Closest-point region of p on triangle t: 0 face interior, 1 edge, 2 vertex
@@ -331,7 +373,9 @@ __global__ void grow_kernel(const int64_t* __restrict__ mism, int64_t m, int dx,
 
 hipError_t launch_candidates(const float* d, int64_t n, float thr, int64_t* list, int64_t cap,
                              unsigned long long* counter, hipStream_t s) {
-    candidates_kernel<<<grid_for(n / 4 + 1), kBlock, 0, s>>>(d, n, thr, list, cap, counter);
+    const int64_t chunks = (n / 4 + (int64_t)kCandBlock * kCandPer - 1) / ((int64_t)kCandBlock * kCandPer);
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(chunks, 8192));
+    candidates_kernel<<<blocks, kCandBlock, 0, s>>>(d, n, thr, list, cap, counter);
     return hipGetLastError();
 }
 

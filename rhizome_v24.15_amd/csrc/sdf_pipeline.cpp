@@ -658,7 +658,8 @@ int learned_select(Ctx& c, const int64_t* cand, const float* feats, int64_t n, i
     int64_t* out = (int64_t*)ws_get(c.w, S_SEL, sizeof(int64_t) * (size_t)std::max<int64_t>(n, 1));
     if (!out) return set_err(RHZ_ERR_NOMEM, "selection buffer allocation failed");
     *sel = out;
-    if (n == 0) {
+    if (n == 0 || (k < 0 && p->scorer == 2)) {
+        // nothing ranked (a stratified scorer has no positives)
         *n_sel = 0;
         return RHZ_OK;
     }
@@ -982,7 +983,7 @@ void rhz_sdf_params_default(rhz_sdf_params* p) {
     p->fsm_round_cap = 512;
     p->winding_precision = RHZ_WINDING_FP32_KAHAN;
     p->fp64_band = 1e-3f;
-    p->reprobe_fraction = 1e-3f;
+    p->reprobe_fraction = -1.0f;
     p->learned_max_patch_gens = 0;
     p->candidate_radius = 0.0f;
     p->scorer = 0;
