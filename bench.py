@@ -105,6 +105,9 @@ def main():
     ap.add_argument("--h", type=float, default=0.10)
     ap.add_argument("--method", default="learned", choices=["fast", "learned", "plain", "legacy"])
     ap.add_argument("--fsm", default="reference", choices=["reference", "converge"])
+    ap.add_argument("--scorer", default="network", choices=["network", "margin", "certificate"],
+                    help="learned method: rank with the proposer network, the planar-margin rule, or "
+                         "nothing (exact probes of the certificate-flagged voxels only)")
     ap.add_argument("--cpu-layers", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-layers", type=int, default=12,
@@ -133,7 +136,8 @@ def main():
     d_tris = torch.from_numpy(tris).to("cuda")
     d_out = torch.empty(V, dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
-    params = rhz.default_params(method=method, fsm_policy=fsm)
+    scorer = {"network": 0, "margin": 1, "certificate": 2}[args.scorer]
+    params = rhz.default_params(method=method, fsm_policy=fsm, scorer=scorer)
 
     def step():
         rc, st = rhz.compute_sdf_device(d_tris, len(tris), args.h, mn, mx, d_out, params, stream=stream)
@@ -216,7 +220,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"vessel mesh {len(tris)} tris @ {args.h} mm, {dims[0]}x{dims[1]}x{dims[2]}",
-                   "method": args.method, "fsm_policy": args.fsm, "voxels": V,
+                   "method": args.method, "scorer": args.scorer if method == rhz.METHOD_FAST_LEARNED else None,
+                   "fsm_policy": args.fsm, "voxels": V,
                    "eikonal_rounds": st.eikonal_rounds, "sweeps_per_step": st.sweeps,
                    "patched_voxels": st.patched_voxels, "reprobed_voxels": st.reprobed_voxels},
         "roofline": dominant,

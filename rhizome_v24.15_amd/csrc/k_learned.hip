@@ -95,29 +95,6 @@ __global__ __launch_bounds__(kCandBlock) void candidates_kernel(const float* __r
     }
 }
 
-// Closest-point region of p on triangle t: 0 face interior, 1 edge, 2 vertex
-// (same region tests as point_tri_dist2).
-__device__ __forceinline__ int closest_region(float px, float py, float pz, const Tri9& t) {
-    float abx = t.bx - t.ax, aby = t.by - t.ay, abz = t.bz - t.az;
-    float acx = t.cx - t.ax, acy = t.cy - t.ay, acz = t.cz - t.az;
-    float apx = px - t.ax, apy = py - t.ay, apz = pz - t.az;
-    float d1 = dot3(abx, aby, abz, apx, apy, apz), d2 = dot3(acx, acy, acz, apx, apy, apz);
-    if (d1 <= 0.0f && d2 <= 0.0f) return 2;
-    float bpx = px - t.bx, bpy = py - t.by, bpz = pz - t.bz;
-    float d3 = dot3(abx, aby, abz, bpx, bpy, bpz), d4 = dot3(acx, acy, acz, bpx, bpy, bpz);
-    if (d3 >= 0.0f && d4 <= d3) return 2;
-    float vc = d1 * d4 - d3 * d2;
-    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) return 1;
-    float cpx = px - t.cx, cpy = py - t.cy, cpz = pz - t.cz;
-    float d5 = dot3(abx, aby, abz, cpx, cpy, cpz), d6 = dot3(acx, acy, acz, cpx, cpy, cpz);
-    if (d6 >= 0.0f && d5 <= d6) return 2;
-    float vb = d5 * d2 - d1 * d6;
-    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) return 1;
-    float va = d3 * d6 - d5 * d4;
-    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) return 1;
-    return 0;
-}
-
 // Does the horizontal segment from the voxel centre P0 to its jittered ray
 // point P1 = P0 + (jx, jy, 0) h possibly cross triangle t?  The scanline
 // pass classified P1; the winding number of P0 equals that of P1 unless the
@@ -147,8 +124,8 @@ __device__ __forceinline__ bool segment_may_cross(float px, float py, float pz, 
 //  0 d/h  1 scan sign  2-4 nearest-triangle normal  5 jitter offset along the
 //  normal (voxels)  6 planar margin  7 second-nearest distance/h
 //  8 log1p(#triangles within 1 h)  9 normal spread within 1.5 h
-//  10 closest-point region  11 distance from the voxel centre to the nearest
-//  crossing of its jittered column ray (voxels)
+//  10 the certificate flag itself  11 distance from the voxel centre to the
+//  nearest crossing of its jittered column ray (voxels)
 __global__ void features_kernel(const float* __restrict__ tris, const float4* __restrict__ xyab,
                                 const int64_t* __restrict__ start, const int32_t* __restrict__ ids,
                                 const uint8_t* __restrict__ sign, const int64_t* __restrict__ list, int64_t n,
@@ -215,13 +192,11 @@ __global__ void features_kernel(const float* __restrict__ tris, const float4* __
         }
     }
     float nx = 0.f, ny = 0.f, nz = 0.f;
-    int region = 0;
     if (best_t >= 0) {
         const float4 nn = reinterpret_cast<const float4*>(tris + 12 * (int64_t)best_t)[2];
         nx = nn.y;
         ny = nn.z;
         nz = nn.w;
-        region = closest_region(px, py, pz, load_tri(tris, best_t));
     }
     const float dist = sqrtf(best) / h;
     const uint8_t sg = sign[v];
@@ -240,7 +215,7 @@ __global__ void features_kernel(const float* __restrict__ tris, const float4* __
     f[7] = fminf(sqrtf(second) / h, 4.0f);
     f[8] = log1pf((float)near_cnt);
     f[9] = spread;
-    f[10] = 0.5f * (float)region;
+    f[10] = risky ? 1.0f : 0.0f;
     f[11] = fminf(zhit / h, 2.0f);
     risk[i] = risky ? 1 : 0;
     wave_append(n_risky, risky);
