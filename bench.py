@@ -112,7 +112,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-layers", type=int, default=12,
                     help="z-layers of the plain exact-winding comparison run (0 = skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 2 GiB device buffer 3 times (reference kernel for rocprofv3 --pmc "
+                         "FETCH_SIZE / WRITE_SIZE scaling) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = 1 << 29  # 2 GiB of float32
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding is a design note only)"}))
         sys.exit(2)
@@ -164,21 +178,27 @@ def main():
     # dominant-kernel roofline from the library's own HIP events (recorded on
     # the launch stream around every sweep round / exact-winding launch)
     sweeps = sum(s.sweeps for s in stats)
-    rounds = sum(s.sweep_calls for s in stats)
+    sweep_launches = sum(s.sweep_calls for s in stats)
     ms_sweeps = sum(s.ms_sweeps for s in stats)
+    ms_trans = sum(s.ms_transposes for s in stats)
     pairs = sum(s.winding_pairs for s in stats)
     wcalls = sum(s.winding_calls for s in stats)
     ms_wind = sum(s.ms_winding for s in stats)
-    rl_fsm = None
-    if sweeps and rounds:
-        # a round = 6 directional sweeps (8 B/voxel each: one read, one write)
-        # plus the two x<->y plane transposes, which are timed but not counted
-        round_ms = ms_sweeps / rounds
-        achieved = 6 * FSM_BYTES_PER_VOXEL * V / (round_ms * 1e-3) / 1e9
-        rl_fsm = {"bound": "hbm", "kernel": "fsm sweep round", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                  "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                  "avg_launch_ms": round_ms, "launches": rounds,
-                  "algorithmic_bytes_per_launch": 6 * FSM_BYTES_PER_VOXEL * V}
+    rl_fsm = rl_round = None
+    if sweeps and sweep_launches:
+        # one directional sweep reads and writes every voxel once: 8 B/voxel
+        per_ms = ms_sweeps / sweep_launches
+        achieved = FSM_BYTES_PER_VOXEL * V / (per_ms * 1e-3) / 1e9
+        rl_fsm = {"bound": "hbm", "kernel": "sweep_lane_kernel (one directional sweep)", "achieved": achieved,
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                  "avg_launch_ms": per_ms, "launches": sweep_launches,
+                  "algorithmic_bytes_per_launch": FSM_BYTES_PER_VOXEL * V}
+        rounds = sweeps / 6
+        round_ms = (ms_sweeps + ms_trans) / rounds
+        ach_r = 6 * FSM_BYTES_PER_VOXEL * V / (round_ms * 1e-3) / 1e9
+        rl_round = {"kernel": "FSM round: 6 sweeps + 2 x<->y plane transposes", "achieved": ach_r,
+                    "unit": "GB/s", "frac": ach_r / HBM_PEAK_GBS, "avg_round_ms": round_ms,
+                    "transpose_ms_per_round": ms_trans / rounds}
     rl_wind = None
     if pairs and ms_wind > 0:
         tflops = WINDING_OPS_PER_PAIR * pairs / (ms_wind * 1e-3) / 1e12
@@ -189,7 +209,7 @@ def main():
                    "note": "FP32 vector ALU work priced against MI355X FP32 157.3 TFLOP/s "
                            f"({WINDING_OPS_PER_PAIR:.0f} FP32 ops per voxel-triangle pair)"}
     dominant = rl_fsm
-    if rl_wind and (not rl_fsm or ms_wind > ms_sweeps):
+    if rl_wind and (not rl_fsm or ms_wind > ms_sweeps + ms_trans):
         dominant = rl_wind
     traffic = load_traffic(REPO, "fsm" if dominant is rl_fsm else "winding")
     if dominant is not None and traffic is not None:
@@ -226,6 +246,7 @@ def main():
                    "patched_voxels": st.patched_voxels, "reprobed_voxels": st.reprobed_voxels},
         "roofline": dominant,
         "roofline_fsm": rl_fsm,
+        "roofline_fsm_round": rl_round,
         "roofline_winding": rl_wind,
         "stage_ms": {k: getattr(st, k) for k in ("ms_splat", "ms_parity", "ms_fixup", "ms_eikonal", "ms_verify",
                                                   "ms_apply", "ms_total")},

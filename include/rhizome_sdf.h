@@ -135,13 +135,14 @@ typedef struct rhz_sdf_stats {
     double ms_apply;
     double ms_host_total;         /* host wall time of the call (the reference's out_time_ms) */
     double ms_sweeps;             /* device time of the directional sweep kernels alone */
+    double ms_transposes;         /* device time of the x<->y plane transposes around x-sweeps */
     double ms_winding;            /* device time of the exact winding kernels alone */
     int64_t winding_pairs;        /* (voxel, triangle) pairs those kernels evaluated */
     int64_t candidates;           /* FAST_LEARNED: candidate voxels ranked */
     int64_t mismatches_found;     /* FAST_LEARNED: scanline signs the exact kernel corrected */
     double ms_proposer;           /* FAST_LEARNED: features + network + selection */
     int64_t winding_calls;        /* bracketed exact-winding launches behind ms_winding */
-    int64_t sweep_calls;          /* bracketed sweep rounds behind ms_sweeps (6 sweeps each) */
+    int64_t sweep_calls;          /* sweep launches behind ms_sweeps (each timed on its own) */
     int64_t ranked_probes;        /* FAST_LEARNED: exact probes chosen by the scorer */
     int64_t ranked_mismatches;    /* FAST_LEARNED: scanline signs those probes corrected */
     int64_t risky_candidates;     /* FAST_LEARNED: candidates the segment certificate flags */

@@ -104,6 +104,7 @@ class Stats(C.Structure):
         ("ms_apply", C.c_double),
         ("ms_host_total", C.c_double),
         ("ms_sweeps", C.c_double),
+        ("ms_transposes", C.c_double),
         ("ms_winding", C.c_double),
         ("winding_pairs", C.c_int64),
         ("candidates", C.c_int64),

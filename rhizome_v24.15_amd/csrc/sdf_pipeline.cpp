@@ -212,6 +212,7 @@ struct Ctx {
     Workspace* w;
     EventAcc* sweeps_t = nullptr;
     EventAcc* wind_t = nullptr;
+    EventAcc* trans_t = nullptr;
 };
 
 // Reads `count` ull counters starting at `first` into h_cnt (synchronises).
@@ -293,25 +294,41 @@ int sweep_round(Ctx& c, float* a, float* b, int* changed) {
     const int dirs[6] = {+1, -1, +1, -1, +1, -1};
     float* src = a;
     float* dst = b;
-    if (c.sweeps_t) c.sweeps_t->begin(c.s);
+    // every sweep launch is bracketed by its own events (per-launch timing);
+    // the plane transposes are timed apart from the sweeps
+    auto sw0 = [&]() { if (c.sweeps_t) c.sweeps_t->begin(c.s); };
+    auto sw1 = [&]() { if (c.sweeps_t) c.sweeps_t->end(c.s); };
+    auto tr0 = [&]() { if (c.trans_t) c.trans_t->begin(c.s); };
+    auto tr1 = [&]() { if (c.trans_t) c.trans_t->end(c.s); };
     if (lane_sweeps_ok(c.g)) {
         // x-sweeps run on a y-fastest copy: a -> b (transpose), x+ b -> a,
         // x- a -> b, b -> a (transpose back); then y and z on x-fastest.
+        tr0();
         HIP_TRY(launch_transpose_xy(a, b, c.g, true, c.s));
+        tr1();
+        sw0();
         HIP_TRY(launch_sweep_yx(b, a, c.g, +1, changed, c.s));
+        sw1();
+        sw0();
         HIP_TRY(launch_sweep_yx(a, b, c.g, -1, changed, c.s));
+        sw1();
+        tr0();
         HIP_TRY(launch_transpose_xy(b, a, c.g, false, c.s));
+        tr1();
         for (int k = 2; k < 6; ++k) {
+            sw0();
             HIP_TRY(launch_sweep(src, dst, c.g, axes[k], dirs[k], changed, c.s));
+            sw1();
             std::swap(src, dst);
         }
     } else {
         for (int k = 0; k < 6; ++k) {
+            sw0();
             HIP_TRY(launch_sweep(src, dst, c.g, axes[k], dirs[k], changed, c.s));
+            sw1();
             std::swap(src, dst);
         }
     }
-    if (c.sweeps_t) c.sweeps_t->end(c.s);
     c.st->sweeps += 6;
     return RHZ_OK;
 }
@@ -884,18 +901,21 @@ int check_params(const rhz_sdf_params* p) {
 int dispatch_inner(Ctx& c);
 
 int dispatch(Ctx& c) {
-    EventAcc sw, wd;
+    EventAcc sw, wd, tr;
     c.sweeps_t = &sw;
     c.wind_t = &wd;
+    c.trans_t = &tr;
     int rc = dispatch_inner(c);
     if (rc >= 0 && hipStreamSynchronize(c.s) == hipSuccess) {
         c.st->ms_sweeps = sw.total_ms();
         c.st->ms_winding = wd.total_ms();
+        c.st->ms_transposes = tr.total_ms();
         c.st->sweep_calls = (int64_t)(sw.used / 2);
         c.st->winding_calls = (int64_t)(wd.used / 2);
     }
     c.sweeps_t = nullptr;
     c.wind_t = nullptr;
+    c.trans_t = nullptr;
     return rc;
 }
 
