@@ -108,7 +108,7 @@ def main():
     ap.add_argument("--scorer", default="network", choices=["network", "margin", "certificate"],
                     help="learned method: rank with the proposer network, the planar-margin rule, or "
                          "nothing (exact probes of the certificate-flagged voxels only)")
-    ap.add_argument("--cpu-layers", type=int, default=8)
+    ap.add_argument("--cpu-layers", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-layers", type=int, default=12,
                     help="z-layers of the plain exact-winding comparison run (0 = skip)")

@@ -1,0 +1,173 @@
+// C++ drop-in for the mesh-to-SDF entry points of rhizome::CudaRasterizer
+// (native/src/engine/sdf/cuda_rasterizer.hpp:15-35, :280-291), implemented
+// over the C-ABI of rhizome_sdf.h (link with -lrhizome_sdf).  Header-only:
+// nothing of the C++ standard library crosses the shared-library boundary.
+//
+// Semantics kept from the reference (cuda_rasterizer.cu:1163-1265, :1340-1435):
+//  * compute_mesh_sdf returns false on failure (message on stderr); on
+//    success out_distances is resized to the lattice size (new elements
+//    1e6, as the reference's resize) and every element is written: signed
+//    distance, negative inside (|w| >= 0.5), x fastest, lattice point
+//    (min + i) * h.  Lattices of >= 32Mi voxels take the fast path, which
+//    falls back to the exact legacy kernel when it declines; smaller ones go
+//    to the legacy kernel directly.
+//  * out_time_ms is host wall time of the call, transfers included.
+//  * build_from_mesh frames the mesh (bbox padded by 10 h, floor/ceil),
+//    builds the triangles (degenerates dropped) and returns nullptr on
+//    failure; every grid it returns carries a new version number.
+//  * Calls are synchronous.  Concurrent calls from several threads are
+//    serialised inside the library (one cached device workspace per device).
+//
+// glm is not a dependency: ivec3 / vec3 are plain structs with glm's layout
+// (three 32-bit members x, y, z), so a caller's glm values convert member-wise.
+#pragma once
+
+#include <atomic>
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <vector>
+
+#include "rhizome_sdf.h"
+
+namespace rhizome {
+
+struct ivec3 {
+    int32_t x = 0, y = 0, z = 0;
+};
+struct vec3 {
+    float x = 0.f, y = 0.f, z = 0.f;
+};
+
+// cuda_types.hpp:18-21: {v0, v1, v2, normal}, 48 bytes, the layout of rhz_triangle.
+struct CudaTriangle {
+    vec3 v0, v1, v2;
+    vec3 normal;
+};
+static_assert(sizeof(CudaTriangle) == sizeof(rhz_triangle), "CudaTriangle must match rhz_triangle");
+
+// engine/io/mesh.hpp:14-47: flat xyz vertices and uint32 triangle indices (mm).
+struct TriangleMesh {
+    std::vector<float> vertices;
+    std::vector<uint32_t> indices;
+    size_t vertex_count() const noexcept { return vertices.size() / 3; }
+    size_t triangle_count() const noexcept { return indices.size() / 3; }
+};
+
+// cuda_types.hpp:155-175 (the fields build_from_mesh fills).
+class CudaVolumeGrid {
+public:
+    std::vector<float> data;
+    ivec3 min_coord{};
+    ivec3 max_coord{};
+    ivec3 dims{};
+    float voxel_size = 0.5f;
+    uint64_t version = 1;
+    bool is_valid() const { return !data.empty() && dims.x > 0 && dims.y > 0 && dims.z > 0; }
+};
+
+class CudaRasterizer {
+public:
+    static bool is_cuda_available() { return rhz_device_count() > 0; }
+
+    static bool query_gpu_memory(size_t& out_free_bytes, size_t& out_total_bytes) {
+        uint64_t f = 0, t = 0;
+        if (rhz_device_memory(&f, &t) != RHZ_OK) return false;
+        out_free_bytes = (size_t)f;
+        out_total_bytes = (size_t)t;
+        return true;
+    }
+
+    static bool compute_mesh_sdf(const std::vector<CudaTriangle>& triangles, float voxel_size, ivec3 min_coord,
+                                 ivec3 max_coord, std::vector<float>& out_distances, float& out_time_ms) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const int64_t dx = (int64_t)max_coord.x - min_coord.x + 1, dy = (int64_t)max_coord.y - min_coord.y + 1,
+                      dz = (int64_t)max_coord.z - min_coord.z + 1;
+        if (dx <= 0 || dy <= 0 || dz <= 0) {
+            std::fprintf(stderr, "[CudaRasterizer] compute_mesh_sdf: empty lattice\n");
+            return false;
+        }
+        out_distances.resize((size_t)(dx * dy * dz), 1e6f);
+        rhz_sdf_params p;
+        rhz_sdf_params_default(&p);
+        p.method = RHZ_METHOD_AUTO;
+        rhz_sdf_stats st;
+        const int32_t mn[3] = {min_coord.x, min_coord.y, min_coord.z};
+        const int32_t mx[3] = {max_coord.x, max_coord.y, max_coord.z};
+        const int rc = rhz_sdf_compute(reinterpret_cast<const rhz_triangle*>(triangles.data()),
+                                       (int64_t)triangles.size(), voxel_size, mn, mx, &p, out_distances.data(), &st);
+        out_time_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (rc != RHZ_OK) {
+            std::fprintf(stderr, "[CudaRasterizer] compute_mesh_sdf failed (%d): %s\n", rc, rhz_last_error());
+            return false;
+        }
+        return true;
+    }
+
+    static std::shared_ptr<CudaVolumeGrid> build_from_mesh(const TriangleMesh& mesh, float voxel_size,
+                                                           float& out_time_ms) {
+        const auto t0 = std::chrono::steady_clock::now();
+        out_time_ms = 0.f;
+        const int64_t nv = (int64_t)mesh.vertex_count(), nt = (int64_t)mesh.triangle_count();
+        if (nv == 0 || nt == 0 || !(voxel_size > 0.f)) return nullptr;
+        int32_t mn[3], mx[3];
+        if (rhz_frame_grid(mesh.vertices.data(), nv, voxel_size, mn, mx) != RHZ_OK) return nullptr;
+        std::vector<CudaTriangle> tris((size_t)nt);
+        int64_t kept = 0;
+        if (rhz_build_triangles(mesh.vertices.data(), nv, mesh.indices.data(), nt,
+                                reinterpret_cast<rhz_triangle*>(tris.data()), &kept) != RHZ_OK || kept == 0)
+            return nullptr;
+        tris.resize((size_t)kept);
+        auto grid = std::make_shared<CudaVolumeGrid>();
+        grid->min_coord = {mn[0], mn[1], mn[2]};
+        grid->max_coord = {mx[0], mx[1], mx[2]};
+        grid->dims = {mx[0] - mn[0] + 1, mx[1] - mn[1] + 1, mx[2] - mn[2] + 1};
+        grid->voxel_size = voxel_size;
+        float ms = 0.f;
+        if (!compute_mesh_sdf(tris, voxel_size, grid->min_coord, grid->max_coord, grid->data, ms)) return nullptr;
+        grid->version = next_version();
+        out_time_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return grid;
+    }
+
+    static bool compute_winding_numbers_gpu(const TriangleMesh& mesh, const std::vector<vec3>& query_points,
+                                            std::vector<float>& out_winding_numbers, float& out_time_ms) {
+        const auto t0 = std::chrono::steady_clock::now();
+        out_winding_numbers.assign(query_points.size(), 0.f);
+        const int rc = rhz_winding_numbers(mesh.vertices.data(), (int64_t)mesh.vertex_count(), mesh.indices.data(),
+                                           (int64_t)mesh.triangle_count(),
+                                           reinterpret_cast<const float*>(query_points.data()),
+                                           (int64_t)query_points.size(), out_winding_numbers.data());
+        out_time_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (rc != RHZ_OK) {
+            std::fprintf(stderr, "[CudaRasterizer] compute_winding_numbers_gpu failed (%d): %s\n", rc,
+                         rhz_last_error());
+            return false;
+        }
+        return true;
+    }
+
+    static bool fast_sweeping_eikonal_gpu(CudaVolumeGrid& grid, float& out_time_ms) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!grid.is_valid()) return false;
+        const int32_t dims[3] = {grid.dims.x, grid.dims.y, grid.dims.z};
+        const int rc = rhz_eikonal_redblack(grid.data.data(), dims, grid.voxel_size, 8);
+        out_time_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (rc != RHZ_OK) {
+            std::fprintf(stderr, "[CudaRasterizer] fast_sweeping_eikonal_gpu failed (%d): %s\n", rc,
+                         rhz_last_error());
+            return false;
+        }
+        grid.version = next_version();
+        return true;
+    }
+
+private:
+    static uint64_t next_version() {
+        static std::atomic<uint64_t> v{1};
+        return ++v;
+    }
+};
+
+}  // namespace rhizome

@@ -154,6 +154,9 @@ void rhz_sdf_params_default(rhz_sdf_params* p);
 const char* rhz_last_error(void);
 /* Number of visible devices (0 when no GPU). */
 int rhz_device_count(void);
+/* Free and total memory of the current device.  Replaces
+ * CudaRasterizer::query_gpu_memory (cuda_rasterizer.hpp:18). */
+int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
 /* Frees the cached device workspace of the current device. */
 void rhz_release_workspace(void);
 

@@ -23,8 +23,16 @@
  *   orc_godunov                      closed forms (s,inf,inf)->s+h,
  *                                    (s,s,inf)->s+h/sqrt2, (s,s,s)->s+h/sqrt3
  *   orc_fsm_*                        planar seeds give the exact linear field,
- *                                    idempotent on a converged field, 6-axis and
- *                                    8-ordering sweeps reach the same fixed point
+ *                                    idempotent on a converged field, only
+ *                                    lowers values, converged far field within
+ *                                    a voxel of the Euclidean distance.  The
+ *                                    fixed point depends on the sweep schedule
+ *                                    (the halved 3D discriminant makes the
+ *                                    update non-monotone in s2): 6-axis and
+ *                                    8-ordering schedules differ by up to about
+ *                                    a voxel, so the schedule itself is pinned
+ *                                    (double-buffered lines, Gauss-Seidel
+ *                                    along each line)
  *   orc_parity_signs                 brute-force ray/triangle counting; equals
  *                                    exact winding sign away from the surface on
  *                                    watertight meshes
@@ -36,6 +44,9 @@
  *   orc_splat_band                   equals the brute-force min over all
  *                                    triangles wherever that min <= band
  *   orc_legacy_sdf                   |p|-R on a sphere (Test 12 bound)
+ *   oracle/mlp.py (proposer network) central finite differences of the FP64
+ *                                    loss, torch's bfloat16 rounding, a planted
+ *                                    mask error that must fail
  * The nvcc FMA-contracted bit patterns of the reference build are not
  * reproducible here (no nvcc, no glm): parity unpinned at the ulp level,
  * absorbed by the 1e-3-voxel distance tolerance.

@@ -87,9 +87,28 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_dropin_example()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+DROPIN_SRC = os.path.join(REPO, "tests", "cxx", "dropin_main.cpp")
+DROPIN_BIN = os.path.join(REPO, "tests", "cxx", "dropin_main")
+
+
+def build_dropin_example() -> str | None:
+    """Host-only C++ program over include/rhizome/cuda_rasterizer.hpp (g++,
+    linked against the library by a relative rpath so the tree can move)."""
+    if not os.path.exists(DROPIN_SRC):
+        return None
+    rel = os.path.relpath(HERE, os.path.dirname(DROPIN_BIN))
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{os.path.join(REPO, 'include')}", DROPIN_SRC,
+           f"-L{HERE}", "-lrhizome_sdf", f"-Wl,-rpath,$ORIGIN/{rel}", "-o", DROPIN_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"drop-in example build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return DROPIN_BIN
 
 
 if __name__ == "__main__":

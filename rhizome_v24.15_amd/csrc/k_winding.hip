@@ -3,9 +3,10 @@
 // launch_winding evaluates w(p) = sum_t Omega_t(p) / (4 pi) over ALL
 // triangles for a list of lattice voxels.  Work is a 2-D grid: 256 points per
 // block x `splits` triangle ranges, so a few thousand probe points still fill
-// 256 CUs.  Each block streams its triangle range through LDS (structure of
-// arrays, every lane reads the same element: broadcast, no bank conflicts) and
-// keeps a compensated (Kahan) FP32 partial per point.  A finalize pass adds the
+// 256 CUs.  A lane owns one point; every lane of a wave reads the same
+// triangle at the same time, so triangle loads are wave-uniform scalar loads
+// (no LDS staging, no barriers).  Each lane keeps compensated (Kahan) FP32
+// partials (two interleaved accumulators).  A finalize pass adds the
 // partials in split order (deterministic), and every point whose ||w| - 0.5| is
 // below fp64_band is queued for a binary64 recompute, which decides its sign.
 //

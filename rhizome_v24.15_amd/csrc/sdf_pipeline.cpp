@@ -1022,6 +1022,16 @@ int rhz_device_count(void) {
     return n;
 }
 
+int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
+    int rc = need_device();
+    if (rc) return rc;
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = (uint64_t)f;
+    if (total_bytes) *total_bytes = (uint64_t)t;
+    return RHZ_OK;
+}
+
 void rhz_release_workspace(void) {
     std::lock_guard<std::mutex> lk(g_ws_mu);
     Workspace* w = workspace();

@@ -114,3 +114,16 @@ def test_no_cpu_fallback_without_gpu(rhz):
     tris[0, 3] = tris[0, 7] = 1.0
     with pytest.raises(rhz.RhzError, match="no HIP device"):
         rhz.compute_sdf(tris, 1.0, (0, 0, 0), (3, 3, 3))
+
+
+def test_cxx_dropin_header_builds_and_runs_without_gpu():
+    """include/rhizome/cuda_rasterizer.hpp compiles against the C-ABI, links
+    the library and reports no device on a machine without one."""
+    import subprocess
+
+    binary = os.path.join(REPO, "tests", "cxx", "dropin_main")
+    assert os.path.exists(binary), "built by __graft_entry__.build()"
+    r = subprocess.run([binary, "info"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    out = r.stdout.split()
+    assert out[0] == "available" and out[1] in ("0", "1")
