@@ -157,6 +157,13 @@ int rhz_device_count(void);
 /* Free and total memory of the current device.  Replaces
  * CudaRasterizer::query_gpu_memory (cuda_rasterizer.hpp:18). */
 int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
+/* Launch-shape tuning of the kernels (no counterpart in the reference; used by
+ * the autotuner): "sweep_step" (2, 4, 8: steps whose loads a sweep wave issues
+ * together), "sweep_rows" (4, 8, 16: waves per block) and "sweep_fmajor" (1: a
+ * block's waves take adjacent segments of one row, 0: adjacent rows).  Process-wide;
+ * results are identical for every setting. */
+int rhz_tuning_set(const char* key, int64_t value);
+int rhz_tuning_get(const char* key, int64_t* value);
 /* Frees the cached device workspace of the current device. */
 void rhz_release_workspace(void);
 

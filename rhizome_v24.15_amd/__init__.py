@@ -184,6 +184,8 @@ def lib():
     L.rhz_stage_mark_patch.argtypes = [vp, vp, i32, i32, vp, vp]
     L.rhz_stage_apply.argtypes = [vp, vp, i64, vp]
     L.rhz_stage_winding_voxels.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, vp, vp]
+    L.rhz_tuning_set.argtypes = [C.c_char_p, i64]
+    L.rhz_tuning_get.argtypes = [C.c_char_p, C.POINTER(C.c_int64)]
     L.rhz_mlp_param_count.restype = C.c_int
     L.rhz_mlp_blob_count.restype = C.c_int
     L.rhz_mlp_builtin.argtypes = [vp]
@@ -229,6 +231,16 @@ def _ptr(a) -> int:
     if hasattr(a, "data_ptr"):
         return a.data_ptr()
     return a.ctypes.data
+
+
+def tuning_set(key: str, value: int) -> None:
+    check(lib().rhz_tuning_set(key.encode(), int(value)), "rhz_tuning_set")
+
+
+def tuning_get(key: str) -> int:
+    v = C.c_int64(0)
+    check(lib().rhz_tuning_get(key.encode(), C.byref(v)), "rhz_tuning_get")
+    return v.value
 
 
 def device_count() -> int:

@@ -11,14 +11,13 @@
 // Godunov update is not monotone in the largest neighbour and therefore
 // schedule-dependent.
 //
-// Memory layout is x fastest.  y- and z-sweeps put 64 consecutive x in one
-// wave (every step is a coalesced 256-byte row access) and prefetch the
-// inputs of the next kStep steps before walking them, so each wave keeps
-// several rows in flight.  The x-sweep stages a 64-line x 32-column tile of
-// the plane (with one halo row/column on each side) and the matching tiles of
-// the planes above and below through LDS, so global traffic stays
-// row-coalesced while each lane walks its own line.  Block order is remapped
-// so neighbouring tiles share an XCD's L2.
+// Memory layout is x fastest.  The y- and z-sweeps put 64 consecutive x in
+// one wave, so every step is a coalesced 256-byte row access, and issue the
+// loads of kStep steps together so each wave keeps several rows in flight.
+// The x-sweeps run the same kernel on a y-fastest copy of the field (plane
+// transposes through LDS before and after).  Block order is remapped so
+// neighbouring tiles share an XCD's L2.  A generic x-sweep kernel covers
+// lattices too large for 32-bit lane offsets.
 #include "rhz_device.hpp"
 #include "rhz_internal.hpp"
 
@@ -194,20 +193,114 @@ __global__ __launch_bounds__(kXLines) void sweep_x_kernel(const float* __restric
 // walks its line along axis W; C is the remaining cross axis.  Each axis
 // feeds one Godunov argument slot (0 = x, 1 = y, 2 = z) so the same kernel
 // serves the y- and z-sweeps on the x-fastest layout and the x-sweep on a
-// y-fastest copy.  The wave's row pointer is uniform (scalar registers);
-// lanes add 32-bit offsets.  A missing neighbour at a grid face reads the
-// opposite neighbour instead: fminf(v, v) = fminf(1e30, v) for every stored
-// value (all are <= 1e30), so no per-step select is needed (F and C must have
-// at least 2 cells).
-template <int SF, int SW, int SC, int DIR>
-__global__ __launch_bounds__(64 * kYZRows) void sweep_lane_kernel(const float* __restrict__ src,
-                                                                   float* __restrict__ dst, int nf, int nw, int nc,
-                                                                   int64_t ws, int64_t cs, int nbf, unsigned nblocks,
-                                                                   float h, int* __restrict__ changed) {
+// y-fastest copy.
+//
+// Addressing costs no vector instructions: every step builds a buffer
+// resource (scalar registers) whose base is the wave's current row minus
+// A = cs + ws + 1 cells, so the five reads of a step (the two fast-axis
+// neighbours, the two cross-line neighbours, the next cell on the line) are
+// buffer loads at per-lane byte offsets fixed for the whole walk.  Stores go
+// through a resource bounded by the row length: dead lanes (f >= nf) store
+// at an offset past the bound, which the hardware drops.  A missing
+// neighbour at a grid face reads the opposite neighbour instead:
+// fminf(v, v) = fminf(1e30, v) for every stored value (all are <= 1e30), so no
+// per-step select is needed (F and C must have at least 2 cells).
+//
+// The walk runs in groups of STEP steps whose loads are issued together;
+// groups never touch a row past the end of the line, and the last < STEP + 1
+// steps run one at a time with the end-of-line value 1e30.  ROWS waves (ROWS
+// adjacent lines) form a block.  Both are launch-time choices (sweep tuning).
+constexpr int kBufWord3 = 0x00020000;  // raw 32-bit buffer access (gfx9 family)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_at(const float* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, (int)bytes, kBufWord3);
+}
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, 0, 0));
+}
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, 0, 0);
+}
+
+// Correctly rounded binary32 square root for 2^-96 <= x < inf: the hardware
+// estimate (within 1 ulp) corrected by the signs of x - s'^2 for the two
+// neighbours s' of s (the same rounding test the compiler's full sequence
+// uses, without its rescaling of tiny inputs and its zero / infinity cases).
+__device__ __forceinline__ float sqrt_rn_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __int_as_float(__float_as_int(s) - 1);
+    const float su = __int_as_float(__float_as_int(s) + 1);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
+    float r = (rd <= 0.0f) ? sd : s;
+    r = (ru > 0.0f) ? su : r;
+    return r;
+}
+
+// Wave-uniform choice of square root: the short form when every active lane's
+// argument is in its range, the full library form otherwise (zero, tiny).
+__device__ __forceinline__ float sqrt_rn_wave(float x) {
+    if (__ballot(!(x >= 0x1.0p-96f)) == 0) return sqrt_rn_normal(x);
+    return sqrtf(x);
+}
+
+// godunov_min with the uniform square roots above; bit-identical results.
+__device__ __forceinline__ float godunov_min_w(float cur, float ax, float ay, float az, const SweepConst& K) {
+    const float s0 = fminf(ax, fminf(ay, az));
+    const float s2 = fmaxf(ax, fmaxf(ay, az));
+    if (!(cur > s0 + K.h057) && s2 < K.hbig) return cur;
+    const float s1 = ax + ay + az - s0 - s2;
+    float u = s0 + K.h;
+    if (u > s1) {
+        const float d10 = s1 - s0;
+        const float disc = K.h2x2 - d10 * d10;
+        if (disc >= 0.0f) {
+            u = (s0 + s1 + sqrt_rn_wave(disc)) * 0.5f;
+            if (u > s2) {
+                const float d21 = s2 - s1, d20 = s2 - s0;
+                const float disc3 = K.h2x3 - (d10 * d10 + d21 * d21 + d20 * d20) * 0.5f;
+                if (disc3 >= 0.0f) u = div3(s0 + s1 + s2 + sqrt_rn_wave(disc3));
+            }
+        }
+    }
+    return (u < cur) ? u : cur;
+}
+
+template <int SF, int SW, int SC>
+__device__ __forceinline__ float lane_step(float cur, float fm, float fp, float cm, float cp, float behind, float next,
+                                           const SweepConst& K) {
+    float g[3];
+    g[SF] = fminf(fm, fp);
+    g[SW] = fminf(behind, next);
+    g[SC] = fminf(cm, cp);
+    // The solve runs only when some lane of the wave can improve (a uniform
+    // branch); lanes that skip keep cur, which is what the solve's own exit
+    // returns for them.
+    const float lo = fminf(g[0], fminf(g[1], g[2]));
+    const float hi = fmaxf(g[0], fmaxf(g[1], g[2]));
+    const bool need = (cur > lo + K.h057) || !(hi < K.hbig);
+    if (__ballot(need) == 0) return cur;
+    return godunov_min_w(cur, g[0], g[1], g[2], K);
+}
+
+template <int SF, int SW, int SC, int DIR, int STEP, int ROWS, bool FMAJ>
+__global__ __launch_bounds__(64 * ROWS) void sweep_lane_kernel(const float* __restrict__ src,
+                                                               float* __restrict__ dst, int nf, int nw, int nc,
+                                                               int64_t ws, int64_t cs, int nbf, unsigned nblocks,
+                                                               float h, int* __restrict__ changed) {
+    // FMAJ: the ROWS waves of a block take adjacent 64-cell segments of ONE
+    // line row (long contiguous accesses); otherwise ROWS adjacent rows.
     const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
-    const int bf = (int)(lb % (unsigned)nbf);
-    const int bc = (int)(lb / (unsigned)nbf);
-    const int c = __builtin_amdgcn_readfirstlane(bc * kYZRows + (int)threadIdx.y);
+    int bf, c;
+    if (FMAJ) {
+        const int nbg = (nbf + ROWS - 1) / ROWS;
+        bf = __builtin_amdgcn_readfirstlane((int)(lb % (unsigned)nbg) * ROWS + (int)threadIdx.y);
+        c = (int)(lb / (unsigned)nbg);
+        if (bf >= nbf) return;
+    } else {
+        bf = (int)(lb % (unsigned)nbf);
+        c = __builtin_amdgcn_readfirstlane((int)(lb / (unsigned)nbf) * ROWS + (int)threadIdx.y);
+    }
     if (c >= nc) return;
     const int lane = threadIdx.x;
     const int f = bf * 64 + lane;
@@ -217,54 +310,56 @@ __global__ __launch_bounds__(64 * kYZRows) void sweep_lane_kernel(const float* _
     const int ofp = (fl < nf - 1) ? 1 : -1;
     const int64_t ocm = (c > 0) ? -cs : cs;
     const int64_t ocp = (c < nc - 1) ? cs : -cs;
-    const int om = (int)ocm, op = (int)ocp;  // |cs| < 2^29 (checked on the host)
     const SweepConst K = sweep_const(h);
-    const int64_t wave_base = (int64_t)c * cs;  // lane offsets fl carry the fast coordinate
+    // byte offsets from the step's resource base (row - A), all >= 0
+    const int64_t A = cs + ws + 1;
+    const uint32_t v_fm = (uint32_t)((fl + ofm + A) * 4);
+    const uint32_t v_fp = (uint32_t)((fl + ofp + A) * 4);
+    const uint32_t v_cm = (uint32_t)((fl + ocm + A) * 4);
+    const uint32_t v_cp = (uint32_t)((fl + ocp + A) * 4);
+    const uint32_t v_nx = (uint32_t)((fl + (int64_t)DIR * ws + A) * 4);
+    const uint32_t v_st = live ? (uint32_t)(f * 4) : 0x7ffffff0u;
+    const uint32_t nrec = (uint32_t)((2 * A + nf + 64) * 4);
+    const uint32_t nrow = (uint32_t)(nf * 4);
     const int64_t step = (int64_t)DIR * ws;
-    int64_t row = wave_base + (int64_t)((DIR > 0) ? 0 : nw - 1) * ws;
+    int64_t row = (int64_t)c * cs + (int64_t)((DIR > 0) ? 0 : nw - 1) * ws;
     float behind = kFar;
-    float cur = src[row + fl];
+    float cur = bload(rsrc_at(src + row - A, nrec), v_nx - (uint32_t)(step * 4));  // the first cell
     bool chg = false;
-    for (int s0 = 0; s0 < nw; s0 += kStep) {
-        float cc[kStep + 1], fm[kStep], fp[kStep], cm[kStep], cp[kStep];
-        cc[0] = cur;
+    int s0 = 0;
+    // full groups: every step of the group has a next cell on the line
+    for (; s0 + STEP < nw; s0 += STEP) {
+        float fm[STEP], fp[STEP], cm[STEP], cp[STEP], nx[STEP];
 #pragma unroll
-        for (int k = 0; k < kStep; ++k) {
-            const int64_t rk = (s0 + k < nw) ? row + k * step : row;
-            const float* __restrict__ rp = src + rk;
-            fm[k] = rp[fl + ofm];
-            fp[k] = rp[fl + ofp];
-            cm[k] = rp[fl + om];
-            cp[k] = rp[fl + op];
-            cc[k + 1] = (s0 + k + 1 < nw) ? src[rk + step + fl] : kFar;
+        for (int k = 0; k < STEP; ++k) {
+            const __amdgpu_buffer_rsrc_t r = rsrc_at(src + (row + k * step) - A, nrec);
+            fm[k] = bload(r, v_fm);
+            fp[k] = bload(r, v_fp);
+            cm[k] = bload(r, v_cm);
+            cp[k] = bload(r, v_cp);
+            nx[k] = bload(r, v_nx);
         }
 #pragma unroll
-        for (int k = 0; k < kStep; ++k) {
-            if (s0 + k >= nw) break;
-            const float a_f = fminf(fm[k], fp[k]);
-            const float a_c = fminf(cm[k], cp[k]);
-            const float a_w = fminf(behind, cc[k + 1]);
-            float g[3];
-            g[SF] = a_f;
-            g[SW] = a_w;
-            g[SC] = a_c;
-            // The Godunov solve runs only when some lane of the wave can improve:
-            // a uniform branch, so converged stretches (most of the far field
-            // after the first round) cost a handful of instructions per cell.
-            // Lanes that skip keep cc[k], which is what godunov_min's own exit
-            // returns for them: bit-identical either way.
-            const float lo = fminf(g[0], fminf(g[1], g[2]));
-            const float hi = fmaxf(g[0], fmaxf(g[1], g[2]));
-            const bool need = (cc[k] > lo + K.h057) || !(hi < K.hbig);
-            float nv = cc[k];
-            if (__ballot(need) != 0)
-                nv = godunov_min(cc[k], g[0], g[1], g[2], K.h, K.h2x2, K.h2x3, K.h057, K.hbig);
-            chg |= nv < cc[k];
-            if (live) dst[row + fl] = nv;
+        for (int k = 0; k < STEP; ++k) {
+            const float nv = lane_step<SF, SW, SC>(cur, fm[k], fp[k], cm[k], cp[k], behind, nx[k], K);
+            chg |= nv < cur;
+            bstore(rsrc_at(dst + row, nrow), v_st, nv);
             behind = nv;
+            cur = nx[k];
             row += step;
         }
-        cur = cc[kStep];
+    }
+    // the last steps, one at a time
+    for (; s0 < nw; ++s0) {
+        const __amdgpu_buffer_rsrc_t r = rsrc_at(src + row - A, nrec);
+        const float a = bload(r, v_fm), b = bload(r, v_fp), cmv = bload(r, v_cm), cpv = bload(r, v_cp);
+        const float next = (s0 + 1 < nw) ? bload(r, v_nx) : kFar;
+        const float nv = lane_step<SF, SW, SC>(cur, a, b, cmv, cpv, behind, next, K);
+        chg |= nv < cur;
+        bstore(rsrc_at(dst + row, nrow), v_st, nv);
+        behind = nv;
+        cur = next;
+        row += step;
     }
     if (__ballot(chg) != 0 && lane == 0) atomicOr(changed, 1);
 }
@@ -319,21 +414,61 @@ __global__ void redblack_kernel(float* __restrict__ g, int dx, int dy, int dz, f
 
 }  // namespace
 
+// Launch shape of the lane sweeps (steps per load group, waves per block);
+// set by the tuning entry point, read at every launch.
+static SweepTuning g_sweep_tuning = {8, 4, 0};
+SweepTuning sweep_tuning() { return g_sweep_tuning; }
+bool set_sweep_tuning(int step, int rows, int fmajor) {
+    if ((step != 2 && step != 4 && step != 8) || (rows != 4 && rows != 8 && rows != 16) ||
+        (fmajor != 0 && fmajor != 1))
+        return false;
+    g_sweep_tuning = {step, rows, fmajor};
+    return true;
+}
+
 namespace {
+
+template <int SF, int SW, int SC, int DIR, int STEP, int ROWS>
+hipError_t launch_lane_v(const float* src, float* dst, int nf, int nw, int nc, int64_t ws, int64_t cs, float h,
+                         int* changed, hipStream_t s) {
+    const int nbf = (nf + 63) / 64;
+    dim3 blk(64, ROWS);
+    if (sweep_tuning().fmajor) {
+        const int64_t blocks = (int64_t)((nbf + ROWS - 1) / ROWS) * nc;
+        sweep_lane_kernel<SF, SW, SC, DIR, STEP, ROWS, true><<<(unsigned)blocks, blk, 0, s>>>(
+            src, dst, nf, nw, nc, ws, cs, nbf, (unsigned)blocks, h, changed);
+    } else {
+        const int64_t blocks = (int64_t)nbf * ((nc + ROWS - 1) / ROWS);
+        sweep_lane_kernel<SF, SW, SC, DIR, STEP, ROWS, false><<<(unsigned)blocks, blk, 0, s>>>(
+            src, dst, nf, nw, nc, ws, cs, nbf, (unsigned)blocks, h, changed);
+    }
+    return hipGetLastError();
+}
+
+template <int SF, int SW, int SC, int DIR>
+hipError_t launch_lane_d(const float* src, float* dst, int nf, int nw, int nc, int64_t ws, int64_t cs, float h,
+                         int* changed, hipStream_t s) {
+    const SweepTuning t = sweep_tuning();
+    if (t.step == 8) {
+        if (t.rows == 4) return launch_lane_v<SF, SW, SC, DIR, 8, 4>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+        if (t.rows == 16) return launch_lane_v<SF, SW, SC, DIR, 8, 16>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+        return launch_lane_v<SF, SW, SC, DIR, 8, 8>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+    }
+    if (t.step == 2) {
+        if (t.rows == 4) return launch_lane_v<SF, SW, SC, DIR, 2, 4>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+        if (t.rows == 16) return launch_lane_v<SF, SW, SC, DIR, 2, 16>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+        return launch_lane_v<SF, SW, SC, DIR, 2, 8>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+    }
+    if (t.rows == 4) return launch_lane_v<SF, SW, SC, DIR, 4, 4>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+    if (t.rows == 16) return launch_lane_v<SF, SW, SC, DIR, 4, 16>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+    return launch_lane_v<SF, SW, SC, DIR, 4, 8>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+}
 
 template <int SF, int SW, int SC>
 hipError_t launch_lane(const float* src, float* dst, int nf, int nw, int nc, int64_t ws, int64_t cs, float h, int dir,
                        int* changed, hipStream_t s) {
-    const int nbf = (nf + 63) / 64;
-    const int64_t blocks = (int64_t)nbf * ((nc + kYZRows - 1) / kYZRows);
-    dim3 blk(64, kYZRows);
-    if (dir > 0)
-        sweep_lane_kernel<SF, SW, SC, 1><<<(unsigned)blocks, blk, 0, s>>>(src, dst, nf, nw, nc, ws, cs, nbf,
-                                                                         (unsigned)blocks, h, changed);
-    else
-        sweep_lane_kernel<SF, SW, SC, -1><<<(unsigned)blocks, blk, 0, s>>>(src, dst, nf, nw, nc, ws, cs, nbf,
-                                                                          (unsigned)blocks, h, changed);
-    return hipGetLastError();
+    if (dir > 0) return launch_lane_d<SF, SW, SC, 1>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+    return launch_lane_d<SF, SW, SC, -1>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
 }
 
 hipError_t launch_generic_sweep(const float* src, float* dst, const Grid& g, int axis, int dir, int* changed,
@@ -363,7 +498,7 @@ hipError_t launch_generic_sweep(const float* src, float* dst, const Grid& g, int
 
 bool lane_sweeps_ok(const Grid& g) {
     const int64_t dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
-    return dx >= 2 && dy >= 2 && dz >= 2 && dx * dy < (int64_t(1) << 29);
+    return dx >= 2 && dy >= 2 && dz >= 2 && dx * dy < (int64_t(1) << 28);
 }
 
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s) {

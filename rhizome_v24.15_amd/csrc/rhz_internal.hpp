@@ -73,6 +73,13 @@ hipError_t launch_sweep(const float* src, float* dst, const Grid& g, int axis, i
 // True when the lane-walker sweeps apply (every axis has >= 2 cells and a
 // plane has < 2^29 cells); otherwise launch_sweep uses generic kernels.
 bool lane_sweeps_ok(const Grid& g);
+struct SweepTuning {
+    int step;  // walk steps whose loads are issued together (2, 4, 8)
+    int rows;  // waves per block (4, 8, 16)
+    int fmajor;  // 1: a block's waves are adjacent segments of one row; 0: adjacent rows
+};
+SweepTuning sweep_tuning();
+bool set_sweep_tuning(int step, int rows, int fmajor);
 // Per-plane transpose between the x-fastest and the y-fastest layouts.
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s);
 // x-sweep on the y-fastest layout (src/dst both y-fastest).

@@ -1022,6 +1022,42 @@ int rhz_device_count(void) {
     return n;
 }
 
+int rhz_tuning_set(const char* key, int64_t value) {
+    if (!key) return set_err(RHZ_ERR_ARG, "null tuning key");
+    const SweepTuning t = sweep_tuning();
+    if (!std::strcmp(key, "sweep_step")) {
+        if (!set_sweep_tuning((int)value, t.rows, t.fmajor)) return set_err(RHZ_ERR_ARG, "sweep_step must be 2, 4 or 8");
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "sweep_rows")) {
+        if (!set_sweep_tuning(t.step, (int)value, t.fmajor)) return set_err(RHZ_ERR_ARG, "sweep_rows must be 4, 8 or 16");
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "sweep_fmajor")) {
+        if (!set_sweep_tuning(t.step, t.rows, (int)value)) return set_err(RHZ_ERR_ARG, "sweep_fmajor must be 0 or 1");
+        return RHZ_OK;
+    }
+    return set_err(RHZ_ERR_ARG, "unknown tuning key '%s'", key);
+}
+
+int rhz_tuning_get(const char* key, int64_t* value) {
+    if (!key || !value) return set_err(RHZ_ERR_ARG, "null argument");
+    const SweepTuning t = sweep_tuning();
+    if (!std::strcmp(key, "sweep_step")) {
+        *value = t.step;
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "sweep_rows")) {
+        *value = t.rows;
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "sweep_fmajor")) {
+        *value = t.fmajor;
+        return RHZ_OK;
+    }
+    return set_err(RHZ_ERR_ARG, "unknown tuning key '%s'", key);
+}
+
 int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
     int rc = need_device();
     if (rc) return rc;

@@ -1,0 +1,47 @@
+"""Wavefront OBJ loading for the mesh-to-SDF entry points.
+
+Reads ``v x y z`` and ``f`` records (1-based or negative indices, ``a/b/c``
+forms, polygons fanned into triangles), as the reference's ObjParser does
+(native/src/engine/io/obj_parser.cpp:36-120), and applies the outward
+orientation check of its winding pre-pass (obj_parser.cpp:8-34): a closed
+mesh with negative signed volume has every face flipped.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def load_obj(path: str):
+    """Returns (vertices float32[N, 3], faces uint32[M, 3])."""
+    verts = []
+    faces = []
+    with open(path, "r", encoding="utf-8", errors="replace") as fh:
+        for line in fh:
+            if line.startswith("v "):
+                p = line.split()
+                verts.append((float(p[1]), float(p[2]), float(p[3])))
+            elif line.startswith("f "):
+                idx = []
+                for tok in line.split()[1:]:
+                    k = int(tok.split("/")[0])
+                    idx.append(k - 1 if k > 0 else len(verts) + k)
+                for i in range(1, len(idx) - 1):
+                    faces.append((idx[0], idx[i], idx[i + 1]))
+    v = np.asarray(verts, dtype=np.float32).reshape(-1, 3)
+    f = np.asarray(faces, dtype=np.int64).reshape(-1, 3)
+    if f.size and (f.min() < 0 or f.max() >= len(v)):
+        raise ValueError(f"{path}: face index out of range")
+    return v, f.astype(np.uint32)
+
+
+def signed_volume(v, f) -> float:
+    v = np.asarray(v, dtype=np.float64)
+    a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    return float(np.einsum("ij,ij->i", a, np.cross(b, c)).sum() / 6.0)
+
+
+def orient_outward(v, f):
+    """Flips every face of a mesh whose signed volume is negative."""
+    if signed_volume(v, f) < 0:
+        return v, f[:, ::-1].copy()
+    return v, f

@@ -127,3 +127,29 @@ def test_cxx_dropin_header_builds_and_runs_without_gpu():
     assert r.returncode == 0
     out = r.stdout.split()
     assert out[0] == "available" and out[1] in ("0", "1")
+
+
+def test_mug_obj_specimen():
+    """The reference's own specimen (native/assets/mug.obj, held as a fixture):
+    3,106 triangles on 1,553 vertices, a closed shell of 75,913 mm^3, framed at
+    0.10 mm to 1021 x 787 x 871 = 699.9M voxels (cuda_rasterizer.cu:2126)."""
+    import importlib.util
+
+    from conftest import entry
+
+    spec = importlib.util.spec_from_file_location(
+        "rhz_obj", os.path.join(REPO, "rhizome_v24.15_amd", "obj.py"))
+    obj = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(obj)
+    v, f = obj.load_obj(os.path.join(REPO, "tests", "golden", "mug.obj"))
+    assert v.shape == (1553, 3) and f.shape == (3106, 3)
+    vol = obj.signed_volume(v, f)
+    assert abs(abs(vol) - 75913) < 1.0
+    geom = entry.load_geometry()
+    mn, mx = geom.frame_grid(v, 0.1)
+    dims = [mx[i] - mn[i] + 1 for i in range(3)]
+    assert dims == [1021, 787, 871]
+    # every edge is shared by exactly two faces (closed single shell)
+    e = np.sort(np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]), axis=1)
+    _, counts = np.unique(e, axis=0, return_counts=True)
+    assert np.all(counts == 2)
