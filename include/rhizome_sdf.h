@@ -185,6 +185,26 @@ int rhz_sdf_compute_device(const rhz_triangle* d_tris, int64_t n_tris, float h,
                            const rhz_sdf_params* params, float* d_out,
                            void* stream, rhz_sdf_stats* stats);
 
+/* One mesh of a batched bake: device triangles, lattice, device output. */
+typedef struct rhz_bake_item {
+    const rhz_triangle* d_tris;
+    int64_t n_tris;
+    float h;
+    int32_t min_c[3];
+    int32_t max_c[3];
+    float* d_out;                 /* (max-min+1) product floats */
+} rhz_bake_item;
+
+/* Batched bake (no counterpart in the reference, whose callers build one
+ * mesh per call): every item's field with the same parameters, largest
+ * lattices first, on n_lanes concurrent device streams (one host thread and
+ * one cached workspace per lane, 1..16).  stats[i] / ms_item[i] (host wall
+ * time of item i, start to completion) are filled when non-NULL.  Returns 0
+ * when every item succeeded; a declined item reports RHZ_DECLINED in
+ * stats[i].status.  Results are identical to rhz_sdf_compute_device per item. */
+int rhz_sdf_bake(const rhz_bake_item* items, int32_t n_items, const rhz_sdf_params* params, int32_t n_lanes,
+                 rhz_sdf_stats* stats, double* ms_item);
+
 /* Indexed mesh -> triangles, dropping |(v1-v0)x(v2-v0)|^2 <= 1e-12 and
  * out-of-range indices, in index order.  Replaces gpu_build_cuda_triangles_kernel
  * + dev_try_build_cuda_triangle (cuda_rasterizer.cu:1321-1337,

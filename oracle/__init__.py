@@ -124,6 +124,7 @@ def lib():
         L.orc_count_far.restype = C.c_int64
         L.orc_clamp_far.argtypes = [_f32p, _u8p, C.c_int64, C.c_float, C.c_int]
         L.orc_verify_indices.argtypes = [C.c_int64, C.c_int, C.c_int, _i64p]
+        L.orc_band_f64.argtypes = [_f32p, C.c_int64, C.c_float, C.c_double, _i32p, _i32p, _f64p]
         L.orc_mark_patch.argtypes = [_u8p, _i64p, C.c_int, C.c_int, _i32p]
         L.orc_apply_signs.argtypes = [_f32p, _u8p, C.c_int64]
         L.orc_fast_sdf.argtypes = [_f32p, C.c_int64, C.c_float, _i32p, _i32p, C.POINTER(Params),
@@ -182,6 +183,14 @@ def splat_band(tris, h, mn, dims, band_voxels=8):
     lib().orc_splat_band(_f32(tris).reshape(-1), len(tris), h, np.float32(band_voxels) * np.float32(h),
                          _i32(mn), _i32(dims), d)
     return d
+
+
+def band_f64(tris, h, mn, dims, band_voxels=8):
+    """Binary64 band distances from the binary32 lattice points (1e30 outside)."""
+    out = np.empty(int(np.prod(dims)), dtype=np.float64)
+    lib().orc_band_f64(_f32(tris).reshape(-1), len(tris), h, band_voxels * float(np.float32(h)), _i32(mn),
+                       _i32(dims), out)
+    return out
 
 
 def parity_signs(tris, h, mn, dims, jx=0.037, jy=0.051, max_hits=128, with_hits=False):

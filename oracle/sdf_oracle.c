@@ -429,6 +429,77 @@ void orc_splat_band(const orc_tri* t, int64_t nt, float h, float bw,
     orc_parallel_for(dims[2], 1, splat_range, &c);
 }
 
+/* Binary64 reference of the band: for every voxel, the minimum over all
+ * triangles of the binary64 point-triangle distance from its binary32 lattice
+ * point (float)(min + i) * h to the binary32 vertices, wherever that minimum
+ * is <= bw; 1e30 elsewhere.  Not part of the reference algorithm: the
+ * yardstick of the precision study. */
+typedef struct {
+    const orc_tri* t;
+    int64_t nt;
+    float h;
+    double bw;
+    const int32_t* mn;
+    const int32_t* dm;
+    double* d;
+} band64_ctx;
+
+static void band64_range(void* vctx, int64_t zb, int64_t ze, int tid) {
+    (void)tid;
+    band64_ctx* c = (band64_ctx*)vctx;
+    const float h = c->h;
+    const double bw = c->bw * (1.0 + 1e-6), band_sq = bw * bw;
+    const int dx = c->dm[0], dy = c->dm[1];
+    const size_t zstride = (size_t)dy * (size_t)dx;
+    for (int64_t ti = 0; ti < c->nt; ++ti) {
+        const orc_tri* tr = &c->t[ti];
+        double a[3], b[3], e[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = tr->v0[k];
+            b[k] = tr->v1[k];
+            e[k] = tr->v2[k];
+        }
+        double lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin(a[k], fmin(b[k], e[k])) - bw;
+            hi[k] = fmax(a[k], fmax(b[k], e[k])) + bw;
+        }
+        int x0 = (int)floor(lo[0] / h) - c->mn[0] - 1, x1 = (int)ceil(hi[0] / h) - c->mn[0] + 1;
+        int y0 = (int)floor(lo[1] / h) - c->mn[1] - 1, y1 = (int)ceil(hi[1] / h) - c->mn[1] + 1;
+        if (x0 < 0) x0 = 0;
+        if (y0 < 0) y0 = 0;
+        if (x1 > dx - 1) x1 = dx - 1;
+        if (y1 > dy - 1) y1 = dy - 1;
+        for (int64_t z = zb; z < ze; ++z) {
+            const double pz = (double)((float)(c->mn[2] + (int)z) * h);
+            if (pz < lo[2] || pz > hi[2]) continue;
+            for (int y = y0; y <= y1; ++y) {
+                const double py = (double)((float)(c->mn[1] + y) * h);
+                if (py < lo[1] || py > hi[1]) continue;
+                size_t row = (size_t)z * zstride + (size_t)y * (size_t)dx;
+                for (int x = x0; x <= x1; ++x) {
+                    const double px = (double)((float)(c->mn[0] + x) * h);
+                    if (px < lo[0] || px > hi[0]) continue;
+                    const double p[3] = {px, py, pz};
+                    const double d2 = orc_point_triangle_distance_sq_f64(p, a, b, e);
+                    if (d2 <= band_sq) {
+                        const double sd = sqrt(d2);
+                        if (sd < c->d[row + x]) c->d[row + x] = sd;
+                    }
+                }
+            }
+        }
+    }
+}
+
+void orc_band_f64(const orc_tri* t, int64_t nt, float h, double bw, const int32_t min_c[3],
+                  const int32_t dims[3], double* d) {
+    const int64_t n = (int64_t)dims[0] * dims[1] * dims[2];
+    for (int64_t i = 0; i < n; ++i) d[i] = 1e30;
+    band64_ctx c = {t, nt, h, bw, min_c, dims, d};
+    orc_parallel_for(dims[2], 1, band64_range, &c);
+}
+
 /* Supercell coverage of one triangle; cuda_rasterizer.cu:836-859. */
 static void csr_cover(const orc_tri* tr, float h, int sup, const int32_t min_c[3],
                       int sdx, int sdy, int* sx0, int* sx1, int* sy0, int* sy1,

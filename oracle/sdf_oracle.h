@@ -134,6 +134,8 @@ void orc_frame_grid(const float* verts, int64_t nv, float h,
 
 /* ---- stages ([init]..[F]); dims = max-min+1, x fastest ---- */
 void orc_fill(float* d, int64_t n, float v);
+void orc_band_f64(const orc_tri* t, int64_t nt, float h, double bw, const int32_t min_c[3],
+                  const int32_t dims[3], double* d);
 void orc_splat_band(const orc_tri* t, int64_t nt, float h, float bw,
                     const int32_t min_c[3], const int32_t dims[3], float* d);
 /* Supercell CSR entry count only (decline check, :824-869). */

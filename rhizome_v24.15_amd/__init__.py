@@ -42,6 +42,17 @@ class RhzError(RuntimeError):
     pass
 
 
+class BakeItem(C.Structure):
+    _fields_ = [
+        ("d_tris", C.c_void_p),
+        ("n_tris", C.c_int64),
+        ("h", C.c_float),
+        ("min_c", C.c_int32 * 3),
+        ("max_c", C.c_int32 * 3),
+        ("d_out", C.c_void_p),
+    ]
+
+
 class Params(C.Structure):
     _fields_ = [
         ("struct_size", C.c_int32),
@@ -184,6 +195,8 @@ def lib():
     L.rhz_stage_mark_patch.argtypes = [vp, vp, i32, i32, vp, vp]
     L.rhz_stage_apply.argtypes = [vp, vp, i64, vp]
     L.rhz_stage_winding_voxels.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, vp, vp]
+    L.rhz_sdf_bake.argtypes = [C.POINTER(BakeItem), i32, C.POINTER(Params), i32, C.POINTER(Stats),
+                               C.POINTER(C.c_double)]
     L.rhz_tuning_set.argtypes = [C.c_char_p, i64]
     L.rhz_tuning_get.argtypes = [C.c_char_p, C.POINTER(C.c_int64)]
     L.rhz_mlp_param_count.restype = C.c_int
@@ -277,6 +290,26 @@ def compute_sdf_device(d_tris, n_tris, h, min_c, max_c, d_out, params: Params | 
     rc = check(lib().rhz_sdf_compute_device(_ptr(d_tris), int(n_tris), float(h), mn.ctypes.data, mx.ctypes.data,
                                             C.byref(p), _ptr(d_out), s, C.byref(st)), "rhz_sdf_compute_device")
     return rc, st
+
+
+def bake(items, params: Params | None = None, n_lanes: int = 4, **kw):
+    """Batched bake.  ``items``: sequence of (d_tris, n_tris, h, min_c, max_c,
+    d_out) with device tensors / pointers.  Returns (rc, [Stats], ms per item)."""
+    p = params if params is not None else default_params(**kw)
+    n = len(items)
+    arr = (BakeItem * max(n, 1))()
+    for i, (t, nt, h, mn, mx, out) in enumerate(items):
+        arr[i].d_tris = _ptr(t)
+        arr[i].n_tris = int(nt)
+        arr[i].h = float(h)
+        for j in range(3):
+            arr[i].min_c[j] = int(mn[j])
+            arr[i].max_c[j] = int(mx[j])
+        arr[i].d_out = _ptr(out)
+    st = (Stats * max(n, 1))()
+    ms = (C.c_double * max(n, 1))()
+    rc = check(lib().rhz_sdf_bake(arr, n, C.byref(p), int(n_lanes), st, ms), "rhz_sdf_bake")
+    return rc, [st[i] for i in range(n)], [ms[i] for i in range(n)]
 
 
 def build_triangles(verts, idx):

@@ -21,7 +21,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
+#include <thread>
 #include <climits>
 #include <string>
 #include <vector>
@@ -118,6 +120,31 @@ Workspace* workspace() {
         w.init = true;
     }
     return &w;
+}
+
+// Batched bake lanes: each lane owns a workspace and a stream (default
+// priority, non-blocking) and is driven by one host thread.
+constexpr int kMaxLanes = 16;
+struct Lane {
+    Workspace ws;
+    hipStream_t stream = nullptr;
+    bool ready = false;
+};
+std::mutex g_lane_mu;
+Lane g_lanes[64][kMaxLanes];
+
+Lane* bake_lane(int lane) {
+    int d = current_device();
+    if (d < 0 || d >= 64 || lane < 0 || lane >= kMaxLanes) return nullptr;
+    Lane& L = g_lanes[d][lane];
+    if (!L.ready) {
+        if (hipMalloc(&L.ws.d_cnt, 512) != hipSuccess) return nullptr;
+        if (hipHostMalloc((void**)&L.ws.h_cnt, 512, hipHostMallocDefault) != hipSuccess) return nullptr;
+        if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        L.ws.init = true;
+        L.ready = true;
+    }
+    return &L;
 }
 
 // Returns a device buffer of at least `bytes` for `slot` (grow-only).
@@ -1106,6 +1133,74 @@ int rhz_sdf_compute_device(const rhz_triangle* d_tris, int64_t n_tris, float h, 
     st->ms_host_total =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+}
+
+int rhz_sdf_bake(const rhz_bake_item* items, int32_t n_items, const rhz_sdf_params* params, int32_t n_lanes,
+                 rhz_sdf_stats* stats, double* ms_item) {
+    int rc = need_device();
+    if (rc) return rc;
+    rc = check_params(params);
+    if (rc) return rc;
+    if (n_items < 0 || (n_items > 0 && !items)) return set_err(RHZ_ERR_ARG, "null item list");
+    if (n_lanes < 1 || n_lanes > kMaxLanes) return set_err(RHZ_ERR_ARG, "n_lanes must be in [1, %d]", kMaxLanes);
+    std::vector<Grid> grids((size_t)n_items);
+    for (int i = 0; i < n_items; ++i) {
+        rc = make_grid(items[i].min_c, items[i].max_c, items[i].h, &grids[(size_t)i]);
+        if (rc) return rc;
+        if (items[i].n_tris <= 0 || !items[i].d_tris || !items[i].d_out)
+            return set_err(RHZ_ERR_ARG, "item %d: null or empty buffer", i);
+    }
+    // largest lattices first: the long items start early, the short ones fill in
+    std::vector<int> order((size_t)n_items);
+    for (int i = 0; i < n_items; ++i) order[(size_t)i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return grids[(size_t)a].n() > grids[(size_t)b].n(); });
+    std::lock_guard<std::mutex> lk(g_lane_mu);
+    int dev = current_device();
+    std::vector<Lane*> lanes((size_t)n_lanes);
+    for (int l = 0; l < n_lanes; ++l) {
+        lanes[(size_t)l] = bake_lane(l);
+        if (!lanes[(size_t)l]) return set_err(RHZ_ERR_NOMEM, "bake lane %d initialisation failed", l);
+    }
+    std::vector<rhz_sdf_stats> st((size_t)n_items);
+    std::vector<double> lat((size_t)n_items, 0.0);
+    std::vector<int> rcs((size_t)n_items, RHZ_OK);
+    std::atomic<int> next{0};
+    auto worker = [&](int l) {
+        (void)hipSetDevice(dev);
+        Lane* L = lanes[(size_t)l];
+        for (;;) {
+            const int k = next.fetch_add(1);
+            if (k >= n_items) break;
+            const int i = order[(size_t)k];
+            const auto t0 = std::chrono::steady_clock::now();
+            rhz_sdf_stats& si = st[(size_t)i];
+            std::memset(&si, 0, sizeof(si));
+            const Grid& g = grids[(size_t)i];
+            for (int j = 0; j < 3; ++j) si.dims[j] = g.dims[j];
+            si.n_triangles = items[i].n_tris;
+            Ctx c{reinterpret_cast<const float*>(items[i].d_tris), items[i].n_tris, g, params, items[i].d_out,
+                  L->stream, &si, &L->ws};
+            int r = dispatch(c);
+            if (r >= 0 && hipStreamSynchronize(L->stream) != hipSuccess) r = set_err(RHZ_ERR_HIP, "bake sync failed");
+            si.status = r;
+            rcs[(size_t)i] = r;
+            lat[(size_t)i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            si.ms_host_total = lat[(size_t)i];
+        }
+    };
+    std::vector<std::thread> th;
+    for (int l = 1; l < n_lanes; ++l) th.emplace_back(worker, l);
+    worker(0);
+    for (auto& t : th) t.join();
+    int first = RHZ_OK;
+    for (int i = 0; i < n_items; ++i) {
+        if (stats) stats[i] = st[(size_t)i];
+        if (ms_item) ms_item[i] = lat[(size_t)i];
+        if (rcs[(size_t)i] < 0 && first == RHZ_OK) first = rcs[(size_t)i];
+    }
+    if (first < 0) return set_err(first, "a bake item failed (status in its stats)");
+    return RHZ_OK;
 }
 
 int rhz_sdf_compute(const rhz_triangle* tris, int64_t n_tris, float h, const int32_t min_c[3],

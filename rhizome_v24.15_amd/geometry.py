@@ -36,6 +36,7 @@ __all__ = [
     "random_vessel",
     "open_box",
     "two_cubes",
+    "bake_batch",
     "uv_sphere",
     "merge",
     "signed_volume",
@@ -253,6 +254,27 @@ def random_vessel(seed: int, target_tris: int):
     v[:, 1] *= wob
     v = jitter(v, 0.02, seed + 7919)
     return _as_mesh(v, f)
+
+
+def bake_batch(n: int = 64, seed: int = 5000, tri_range=(2_000, 500_000), side_range=(64, 384),
+               h_range=(0.25, 0.5)):
+    """The batched-bake workload: ``n`` seeded random vessels with log-uniform
+    triangle counts, each scaled so that, framed by the reference rule at a
+    voxel size drawn from ``h_range``, its longest lattice side is drawn
+    uniformly from ``side_range``.  Returns [(vertices, faces, h)]."""
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng(seed + i)
+        tri = int(math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1]))))
+        side = int(rng.integers(side_range[0], side_range[1] + 1))
+        h = float(np.float32(rng.uniform(*h_range)))
+        v, f = random_vessel(seed + i, tri)
+        ext = float(np.max(v.max(0) - v.min(0)))
+        scale = (side - 22) * h / ext  # frame_grid pads 10 h on each side
+        c = (v.max(0) + v.min(0)) / 2
+        v = ((v - c) * scale).astype(np.float32)
+        out.append((v, f, h))
+    return out
 
 
 def with_slivers(v, f, frac: float = 0.01, eps: float = 1e-4, seed: int = 3):

@@ -1,0 +1,44 @@
+"""Batched bake (rhz_sdf_bake): every item's field and counters equal the
+single-mesh entry point's, whatever the number of concurrent lanes."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_bake_matches_single_builds(rhz, geom, torch_gpu):
+    torch = torch_gpu
+    meshes = geom.bake_batch(n=8, seed=77, tri_range=(2_000, 20_000), side_range=(32, 96))
+    items, singles = [], []
+    for v, f, h in meshes:
+        tris = rhz.build_triangles(v, f)
+        mn, mx = geom.frame_grid(v, h)
+        V = int(np.prod([mx[i] - mn[i] + 1 for i in range(3)]))
+        d_t = torch.from_numpy(tris).cuda()
+        out = torch.full((V,), np.nan, dtype=torch.float32, device="cuda")
+        ref = torch.empty(V, dtype=torch.float32, device="cuda")
+        items.append((d_t, len(tris), h, mn, mx, out))
+        singles.append(ref)
+    p = rhz.default_params(method=rhz.METHOD_FAST_LEARNED)
+    ref_stats = []
+    for (d_t, nt, h, mn, mx, _), ref in zip(items, singles):
+        rc, st = rhz.compute_sdf_device(d_t, nt, h, mn, mx, ref, p, stream=torch.cuda.current_stream())
+        assert rc == rhz.RHZ_OK
+        ref_stats.append(st)
+    torch.cuda.synchronize()
+    for lanes in (1, 3):
+        for it in items:
+            it[5].fill_(float("nan"))
+        torch.cuda.synchronize()
+        rc, stats, ms = rhz.bake(items, p, n_lanes=lanes)
+        assert rc == rhz.RHZ_OK
+        for it, ref, st, rs, t in zip(items, singles, stats, ref_stats, ms):
+            assert torch.equal(it[5].view(torch.int32), ref.view(torch.int32))
+            assert st.status == rhz.RHZ_OK and t > 0
+            for k in ("eikonal_rounds", "candidates", "risky_candidates", "reprobed_voxels", "mismatches_found"):
+                assert getattr(st, k) == getattr(rs, k), k
+
+
+def test_bake_rejects_bad_lane_count(rhz, geom, torch_gpu):
+    with pytest.raises(rhz.RhzError):
+        rhz.bake([], rhz.default_params(), n_lanes=0)

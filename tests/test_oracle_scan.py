@@ -144,3 +144,33 @@ def test_csr_decline(orc, small_sphere):
     tris, h, mn, mx = small_sphere
     rc, _, _, st = orc.fast_sdf(tris, h, mn, mx, orc.default_params(csr_max_entries=10))
     assert rc == 1 and st.decline_reason == 2
+
+
+def test_band_f64_vs_binary32_band_and_brute_force(orc, geom):
+    """The binary64 band (precision-study yardstick) agrees with the binary32
+    splat to binary32 rounding, covers the same voxels, and equals a direct
+    binary64 brute-force minimum at sampled voxels."""
+    v, f = geom.icosphere(2)
+    v = v * 5.0
+    tris = orc.build_triangles(v, f)
+    h = 0.5
+    mn, mx = (-14, -14, -14), (13, 13, 13)
+    dims = orc.dims_of(mn, mx)
+    d32 = orc.splat_band(tris, h, mn, dims, band_voxels=3)
+    d64 = orc.band_f64(tris, h, mn, dims, band_voxels=3)
+    inb = d32 < 1e29
+    assert np.array_equal(inb, d64 < 1e29) or np.sum(inb != (d64 < 1e29)) <= 2
+    both = inb & (d64 < 1e29)
+    assert np.max(np.abs(d32[both] - d64[both])) <= 1e-5 * h
+    rng = np.random.default_rng(0)
+    for i in rng.choice(np.flatnonzero(both), 50, replace=False):
+        x = i % dims[0]
+        y = (i // dims[0]) % dims[1]
+        z = i // (dims[0] * dims[1])
+        p = np.array([np.float32(mn[0] + x) * np.float32(h), np.float32(mn[1] + y) * np.float32(h),
+                      np.float32(mn[2] + z) * np.float32(h)], np.float64)
+        t = tris.reshape(-1, 12).astype(np.float64)
+        L = orc.lib()
+        best = min(L.orc_point_triangle_distance_sq_f64(p, r[0:3].copy(), r[3:6].copy(), r[6:9].copy())
+                   for r in t)
+        assert abs(np.sqrt(best) - d64[i]) <= 1e-12
