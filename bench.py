@@ -152,8 +152,7 @@ def batched_bake(rhz, torch, params, n_meshes=64, log=None):
                         f"at 0.25-0.5 mm, {total} voxels", "status": rc,
             "value": total / (wall * 1e-3) / 1e9, "unit": "Gvoxel/s", "batch_ms": wall,
             "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
-            "n_lanes": best["n_lanes"], "sweep_step": best["sweep_step"], "sweep_rows": best["sweep_rows"],
-            "autotune_table": table}
+            "tuning": best, "autotune_table": table}
 
 
 def certify(rhz, torch, d_tris, n_tris, h, mn, mx, d_out, params):
@@ -325,9 +324,11 @@ def main():
     batch = None
     if not args.no_batch:
         try:
+            saved = {k: rhz.tuning_get(k) for k in ("sweep_lines", "sweep_step", "sweep_rows",
+                                                    "fsm_batch_rounds_max_voxels", "fsm_graph")}
             batch = batched_bake(rhz, torch, params, log=log)
-            rhz.tuning_set("sweep_step", 8)
-            rhz.tuning_set("sweep_rows", 4)
+            for k, v in saved.items():
+                rhz.tuning_set(k, v)
         except Exception as e:  # the extra line must not hide the main number
             batch = {"error": str(e)}
     cert = None

@@ -1,7 +1,11 @@
-"""Autotuner for the batched bake: times candidate launch configurations
-(concurrent lanes, sweep load-group length and waves per block) on a sample
-of the batch and keeps the fastest.  Every configuration computes identical
-results; only the time differs.
+"""Autotuner for the batched bake: times candidate launch configurations on a
+sample of the batch and keeps the fastest.  Every configuration computes
+identical results; only the time differs.
+
+Searched, coordinate-wise: the number of concurrent lanes; the sweep kernel
+shape (lines per lane, steps whose loads are issued together); whether the
+FSM rounds of small lattices are enqueued in one go and replayed as a
+hipGraph.
 
     best, table = autotune_bake(rhz, torch, items, params)
 """
@@ -10,7 +14,8 @@ from __future__ import annotations
 import time
 
 LANES = (1, 2, 4, 8)
-SWEEP_SHAPES = ((8, 4), (4, 8), (2, 4))  # (sweep_step, sweep_rows)
+SWEEP_SHAPES = ((2, 4), (1, 8), (3, 8))          # (sweep_lines, sweep_step)
+FSM_MODES = ((16 << 20, 0), (16 << 20, 1), (0, 0))  # (fsm_batch_rounds_max_voxels, fsm_graph)
 
 
 def _time_bake(rhz, torch, items, params, n_lanes):
@@ -23,21 +28,48 @@ def _time_bake(rhz, torch, items, params, n_lanes):
     return (time.perf_counter() - t0) * 1e3
 
 
-def autotune_bake(rhz, torch, items, params, lanes=LANES, sweep_shapes=SWEEP_SHAPES, sample=16, log=None):
-    """Returns ({"n_lanes", "sweep_step", "sweep_rows", "ms"}, [timings]) and
-    leaves the library's sweep tuning set to the winner."""
+def _apply(rhz, cfg):
+    rhz.tuning_set("sweep_lines", cfg["sweep_lines"])
+    rhz.tuning_set("sweep_step", cfg["sweep_step"])
+    rhz.tuning_set("fsm_batch_rounds_max_voxels", cfg["fsm_batch_rounds_max_voxels"])
+    rhz.tuning_set("fsm_graph", cfg["fsm_graph"])
+
+
+def autotune_bake(rhz, torch, items, params, sample=16, log=None):
+    """Returns (best config with its time, [all timings]) and leaves the
+    library tuned to the winner."""
     sub = list(items[:sample])
+    cfg = {"n_lanes": 4, "sweep_lines": rhz.tuning_get("sweep_lines"), "sweep_step": rhz.tuning_get("sweep_step"),
+           "fsm_batch_rounds_max_voxels": rhz.tuning_get("fsm_batch_rounds_max_voxels"),
+           "fsm_graph": rhz.tuning_get("fsm_graph")}
     table = []
-    _time_bake(rhz, torch, sub, params, max(lanes))  # warm the lanes' workspaces
-    for step, rows in sweep_shapes:
-        rhz.tuning_set("sweep_step", step)
-        rhz.tuning_set("sweep_rows", rows)
-        for nl in lanes:
-            ms = _time_bake(rhz, torch, sub, params, nl)
-            table.append({"n_lanes": nl, "sweep_step": step, "sweep_rows": rows, "ms": ms})
-            if log:
-                log(f"autotune: lanes {nl} step {step} rows {rows}: {ms:.1f} ms")
-    best = min(table, key=lambda r: r["ms"])
-    rhz.tuning_set("sweep_step", best["sweep_step"])
-    rhz.tuning_set("sweep_rows", best["sweep_rows"])
-    return dict(best), table
+
+    def trial(c):
+        _apply(rhz, c)
+        ms = _time_bake(rhz, torch, sub, params, c["n_lanes"])
+        row = dict(c, ms=ms)
+        table.append(row)
+        if log:
+            log(f"autotune: {row}")
+        return ms
+
+    _apply(rhz, cfg)
+    _time_bake(rhz, torch, sub, params, max(LANES))  # warm the lanes' workspaces
+    best_ms = None
+    for nl in LANES:
+        c = dict(cfg, n_lanes=nl)
+        ms = trial(c)
+        if best_ms is None or ms < best_ms:
+            best_ms, cfg = ms, c
+    for lines, step in SWEEP_SHAPES:
+        c = dict(cfg, sweep_lines=lines, sweep_step=step)
+        ms = trial(c)
+        if ms < best_ms:
+            best_ms, cfg = ms, c
+    for mv, g in FSM_MODES:
+        c = dict(cfg, fsm_batch_rounds_max_voxels=mv, fsm_graph=g)
+        ms = trial(c)
+        if ms < best_ms:
+            best_ms, cfg = ms, c
+    _apply(rhz, cfg)
+    return dict(cfg, ms=best_ms), table

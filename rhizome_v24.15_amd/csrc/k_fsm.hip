@@ -283,6 +283,127 @@ __device__ __forceinline__ float lane_step(float cur, float fm, float fp, float 
     return godunov_min_w(cur, g[0], g[1], g[2], K);
 }
 
+// Branch-free form of the same update (every candidate root computed, then
+// selected): two independent lines per lane interleave without a branch
+// between them.  Bit-identical to godunov_min_w: the roots that the branchy
+// form would not reach are discarded by the selects, and where its early exit
+// returns cur the full formula returns cur too.
+template <int SF, int SW, int SC>
+__device__ __forceinline__ float lane_step_nb(float cur, float fm, float fp, float cm, float cp, float behind,
+                                              float next, const SweepConst& K) {
+    float g[3];
+    g[SF] = fminf(fm, fp);
+    g[SW] = fminf(behind, next);
+    g[SC] = fminf(cm, cp);
+    const float s0 = fminf(g[0], fminf(g[1], g[2]));
+    const float s2 = fmaxf(g[0], fmaxf(g[1], g[2]));
+    const float s1 = g[0] + g[1] + g[2] - s0 - s2;
+    const float u1 = s0 + K.h;
+    const float d10 = s1 - s0;
+    const float disc = K.h2x2 - d10 * d10;
+    const float u2 = (s0 + s1 + sqrtf(fmaxf(disc, 0.0f))) * 0.5f;
+    const float d21 = s2 - s1, d20 = s2 - s0;
+    const float disc3 = K.h2x3 - (d10 * d10 + d21 * d21 + d20 * d20) * 0.5f;
+    const float u3 = div3(s0 + s1 + s2 + sqrtf(fmaxf(disc3, 0.0f)));
+    const bool two = (u1 > s1) && (disc >= 0.0f);
+    const bool three = two && (u2 > s2) && (disc3 >= 0.0f);
+    const float u = three ? u3 : (two ? u2 : u1);
+    return (u < cur) ? u : cur;
+}
+
+// LPL lines per lane: lane l walks lines f = 64 LPL bf + l + 64 j, j < LPL.
+template <int SF, int SW, int SC, int DIR, int STEP, int LPL>
+__global__ __launch_bounds__(256) void sweep_lane2_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                                          int nf, int nw, int nc, int64_t ws, int64_t cs, int nbf,
+                                                          unsigned nblocks, float h, int* __restrict__ changed) {
+    const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
+    const int bf = (int)(lb % (unsigned)nbf);
+    const int c = __builtin_amdgcn_readfirstlane((int)(lb / (unsigned)nbf) * 4 + (int)threadIdx.y);
+    if (c >= nc) return;
+    const int lane = threadIdx.x;
+    const int64_t ocm = (c > 0) ? -cs : cs;
+    const int64_t ocp = (c < nc - 1) ? cs : -cs;
+    const SweepConst K = sweep_const(h);
+    const int64_t A = cs + ws + 1;
+    uint32_t v_fm[LPL], v_fp[LPL], v_cm[LPL], v_cp[LPL], v_nx[LPL], v_st[LPL];
+#pragma unroll
+    for (int j = 0; j < LPL; ++j) {
+        const int f = (bf * LPL + j) * 64 + lane;
+        const bool live = f < nf;
+        const int fl = live ? f : nf - 1;
+        const int ofm = (fl > 0) ? -1 : 1;
+        const int ofp = (fl < nf - 1) ? 1 : -1;
+        v_fm[j] = (uint32_t)((fl + ofm + A) * 4);
+        v_fp[j] = (uint32_t)((fl + ofp + A) * 4);
+        v_cm[j] = (uint32_t)((fl + ocm + A) * 4);
+        v_cp[j] = (uint32_t)((fl + ocp + A) * 4);
+        v_nx[j] = (uint32_t)((fl + (int64_t)DIR * ws + A) * 4);
+        v_st[j] = live ? (uint32_t)(f * 4) : 0x7ffffff0u;
+    }
+    const uint32_t nrec = (uint32_t)((2 * A + nf + 64 * LPL) * 4);
+    const uint32_t nrow = (uint32_t)(nf * 4);
+    const int64_t step = (int64_t)DIR * ws;
+    int64_t row = (int64_t)c * cs + (int64_t)((DIR > 0) ? 0 : nw - 1) * ws;
+    float behind[LPL], cur[LPL];
+    bool chg = false;
+    {
+        const __amdgpu_buffer_rsrc_t r0 = rsrc_at(src + row - A, nrec);
+#pragma unroll
+        for (int j = 0; j < LPL; ++j) {
+            behind[j] = kFar;
+            cur[j] = bload(r0, v_nx[j] - (uint32_t)(step * 4));
+        }
+    }
+    int s0 = 0;
+    for (; s0 + STEP < nw; s0 += STEP) {
+        float fm[STEP][LPL], fp[STEP][LPL], cm[STEP][LPL], cp[STEP][LPL], nx[STEP][LPL];
+#pragma unroll
+        for (int k = 0; k < STEP; ++k) {
+            const __amdgpu_buffer_rsrc_t r = rsrc_at(src + (row + k * step) - A, nrec);
+#pragma unroll
+            for (int j = 0; j < LPL; ++j) {
+                fm[k][j] = bload(r, v_fm[j]);
+                fp[k][j] = bload(r, v_fp[j]);
+                cm[k][j] = bload(r, v_cm[j]);
+                cp[k][j] = bload(r, v_cp[j]);
+                nx[k][j] = bload(r, v_nx[j]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < STEP; ++k) {
+            const __amdgpu_buffer_rsrc_t w = rsrc_at(dst + row, nrow);
+#pragma unroll
+            for (int j = 0; j < LPL; ++j) {
+                const float nv = lane_step_nb<SF, SW, SC>(cur[j], fm[k][j], fp[k][j], cm[k][j], cp[k][j], behind[j],
+                                                          nx[k][j], K);
+                chg |= nv < cur[j];
+                bstore(w, v_st[j], nv);
+                behind[j] = nv;
+                cur[j] = nx[k][j];
+            }
+            row += step;
+        }
+    }
+    for (; s0 < nw; ++s0) {
+        const __amdgpu_buffer_rsrc_t r = rsrc_at(src + row - A, nrec);
+        const __amdgpu_buffer_rsrc_t w = rsrc_at(dst + row, nrow);
+        const bool more = s0 + 1 < nw;
+#pragma unroll
+        for (int j = 0; j < LPL; ++j) {
+            const float a = bload(r, v_fm[j]), b = bload(r, v_fp[j]), cmv = bload(r, v_cm[j]),
+                        cpv = bload(r, v_cp[j]);
+            const float next = more ? bload(r, v_nx[j]) : kFar;
+            const float nv = lane_step_nb<SF, SW, SC>(cur[j], a, b, cmv, cpv, behind[j], next, K);
+            chg |= nv < cur[j];
+            bstore(w, v_st[j], nv);
+            behind[j] = nv;
+            cur[j] = next;
+        }
+        row += step;
+    }
+    if (__ballot(chg) != 0 && lane == 0) atomicOr(changed, 1);
+}
+
 template <int SF, int SW, int SC, int DIR, int STEP, int ROWS, bool FMAJ>
 __global__ __launch_bounds__(64 * ROWS) void sweep_lane_kernel(const float* __restrict__ src,
                                                                float* __restrict__ dst, int nf, int nw, int nc,
@@ -416,14 +537,21 @@ __global__ void redblack_kernel(float* __restrict__ g, int dx, int dy, int dz, f
 
 // Launch shape of the lane sweeps (steps per load group, waves per block);
 // set by the tuning entry point, read at every launch.
-static SweepTuning g_sweep_tuning = {8, 4, 0};
+static SweepTuning g_sweep_tuning = {4, 4, 0, 2, 16ll << 20, 0};
 SweepTuning sweep_tuning() { return g_sweep_tuning; }
-bool set_sweep_tuning(int step, int rows, int fmajor) {
+bool set_sweep_tuning(int step, int rows, int fmajor, int lines) {
     if ((step != 2 && step != 4 && step != 8) || (rows != 4 && rows != 8 && rows != 16) ||
-        (fmajor != 0 && fmajor != 1))
+        (fmajor != 0 && fmajor != 1) || lines < 1 || lines > 3)
         return false;
-    g_sweep_tuning = {step, rows, fmajor};
+    g_sweep_tuning.step = step;
+    g_sweep_tuning.rows = rows;
+    g_sweep_tuning.fmajor = fmajor;
+    g_sweep_tuning.lines = lines;
     return true;
+}
+void set_fsm_batching(int64_t max_voxels, int graph) {
+    g_sweep_tuning.batch_rounds_max_voxels = max_voxels;
+    g_sweep_tuning.fsm_graph = graph ? 1 : 0;
 }
 
 namespace {
@@ -445,10 +573,29 @@ hipError_t launch_lane_v(const float* src, float* dst, int nf, int nw, int nc, i
     return hipGetLastError();
 }
 
+template <int SF, int SW, int SC, int DIR, int STEP, int LPL>
+hipError_t launch_lane2(const float* src, float* dst, int nf, int nw, int nc, int64_t ws, int64_t cs, float h,
+                        int* changed, hipStream_t s) {
+    const int nbf = (nf + 64 * LPL - 1) / (64 * LPL);
+    const int64_t blocks = (int64_t)nbf * ((nc + 3) / 4);
+    dim3 blk(64, 4);
+    sweep_lane2_kernel<SF, SW, SC, DIR, STEP, LPL><<<(unsigned)blocks, blk, 0, s>>>(src, dst, nf, nw, nc, ws, cs, nbf,
+                                                                                  (unsigned)blocks, h, changed);
+    return hipGetLastError();
+}
+
 template <int SF, int SW, int SC, int DIR>
 hipError_t launch_lane_d(const float* src, float* dst, int nf, int nw, int nc, int64_t ws, int64_t cs, float h,
                          int* changed, hipStream_t s) {
     const SweepTuning t = sweep_tuning();
+    if (t.lines == 2) {
+        if (t.step == 2) return launch_lane2<SF, SW, SC, DIR, 2, 2>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+        return launch_lane2<SF, SW, SC, DIR, 4, 2>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+    }
+    if (t.lines == 3) {  // one line per lane, branch-free update
+        if (t.step == 8) return launch_lane2<SF, SW, SC, DIR, 8, 1>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+        return launch_lane2<SF, SW, SC, DIR, 4, 1>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+    }
     if (t.step == 8) {
         if (t.rows == 4) return launch_lane_v<SF, SW, SC, DIR, 8, 4>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
         if (t.rows == 16) return launch_lane_v<SF, SW, SC, DIR, 8, 16>(src, dst, nf, nw, nc, ws, cs, h, changed, s);

@@ -20,6 +20,11 @@ inline unsigned stream_blocks(int64_t work_items, int per_block = kBlock) {
 
 // [init] d[i] = v (reference gpu_fill_float_kernel, cuda_rasterizer.cu:194-203),
 // 16-byte stores.
+__global__ void zero_i32_kernel(int* __restrict__ p, int n) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0;
+}
+
 __global__ void fill_kernel(float* __restrict__ d, int64_t n, float v) {
     typedef float vf4 __attribute__((ext_vector_type(4)));
     const int64_t n4 = n >> 2;
@@ -268,6 +273,12 @@ hipError_t scan_i64_inplace_sums(int64_t* data, int64_t n, int64_t* scratch, hip
 }  // namespace
 
 // ---- launchers ---------------------------------------------------------------
+hipError_t launch_zero_i32(int* p, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    zero_i32_kernel<<<(unsigned)((n + 63) / 64), 64, 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(float* d, int64_t n, float v, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     fill_kernel<<<stream_blocks(n / 4 + 1), kBlock, 0, s>>>(d, n, v);

@@ -17,6 +17,8 @@ struct Grid {
 
 // ---- k_grid.hip -----------------------------------------------------------
 hipError_t launch_fill(float* d, int64_t n, float v, hipStream_t s);
+// Zeroes n ints with a kernel (safe inside captured graphs, unlike memset nodes).
+hipError_t launch_zero_i32(int* p, int n, hipStream_t s);
 hipError_t launch_apply(float* d, const uint8_t* sign, int64_t n, hipStream_t s);
 hipError_t launch_clamp_far(float* d, const uint8_t* sign, int64_t n, float bw, int intended,
                             hipStream_t s);
@@ -77,9 +79,14 @@ struct SweepTuning {
     int step;  // walk steps whose loads are issued together (2, 4, 8)
     int rows;  // waves per block (4, 8, 16)
     int fmajor;  // 1: a block's waves are adjacent segments of one row; 0: adjacent rows
+    int lines;   // 1: one line per lane; 2: two lines per lane, branch-free update;
+                 // 3: one line per lane, branch-free update
+    int64_t batch_rounds_max_voxels;  // lattices up to this size enqueue all FSM rounds at once
+    int fsm_graph;                    // 1: those rounds are captured and replayed as one hipGraph
 };
 SweepTuning sweep_tuning();
-bool set_sweep_tuning(int step, int rows, int fmajor);
+bool set_sweep_tuning(int step, int rows, int fmajor, int lines);
+void set_fsm_batching(int64_t max_voxels, int graph);
 // Per-plane transpose between the x-fastest and the y-fastest layouts.
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s);
 // x-sweep on the y-fastest layout (src/dst both y-fastest).

@@ -168,7 +168,8 @@ void* ws_get(Workspace* w, int slot, size_t bytes) {
 
 // counters: ull slots 0..15, int slots at byte offset 128 (32 ints)
 enum CounterId { C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4, C_SEL = 5, C_GROW = 6, C_RISKY = 7 };
-enum FlagId { F_UNKNOWN = 0, F_CHANGED = 1 };
+enum FlagId { F_UNKNOWN = 0, F_CHANGED = 1, F_ROUND0 = 2 };  // F_ROUND0 + r: round r changed a voxel
+constexpr int kMaxBatchedRounds = 24;
 
 unsigned long long* cnt_ptr(Workspace* w, int id) { return w->d_cnt + id; }
 int* flag_ptr(Workspace* w, int id) { return reinterpret_cast<int*>(reinterpret_cast<char*>(w->d_cnt) + 128) + id; }
@@ -369,6 +370,52 @@ int audit(Ctx& c, int64_t* far) {
     return RHZ_OK;
 }
 
+// All `cap` rounds enqueued without a host round trip in between, each round
+// raising its own flag; optionally captured and replayed as one hipGraph.
+// The policy's round count is the first round that changed nothing (+1): the
+// rounds after it see an unchanged field and change nothing either, so the
+// result equals the round-by-round loop.
+int fsm_rounds_batched(Ctx& c, float* b, int cap, int* rounds_out) {
+    int* flags = flag_ptr(c.w, F_ROUND0);
+    const bool graph = sweep_tuning().fsm_graph != 0 && c.s != nullptr;
+    EventAcc* sw = c.sweeps_t;
+    EventAcc* tr = c.trans_t;
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if (graph) {
+        c.sweeps_t = nullptr;  // no event records inside the capture
+        c.trans_t = nullptr;
+        HIP_TRY(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+    }
+    int rc = RHZ_OK;
+    hipError_t e = launch_zero_i32(flags, cap, c.s);
+    if (e != hipSuccess) rc = set_err(RHZ_ERR_HIP, "flag reset failed: %s", hipGetErrorString(e));
+    for (int r = 0; r < cap && rc == RHZ_OK; ++r) rc = sweep_round(c, c.d, b, flags + r);
+    if (graph) {
+        const hipError_t ec = hipStreamEndCapture(c.s, &gr);
+        c.sweeps_t = sw;
+        c.trans_t = tr;
+        if (rc == RHZ_OK && ec != hipSuccess) rc = set_err(RHZ_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+        if (rc == RHZ_OK) {
+            e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+            if (e == hipSuccess) e = hipGraphLaunch(ge, c.s);
+            if (e != hipSuccess) rc = set_err(RHZ_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e));
+        }
+    }
+    if (rc == RHZ_OK) rc = read_counters(c);  // synchronises
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (gr) (void)hipGraphDestroy(gr);
+    if (rc) return rc;
+    int rounds = cap;
+    for (int r = 0; r < cap; ++r)
+        if (hf(c, F_ROUND0 + r) == 0) {
+            rounds = r + 1;
+            break;
+        }
+    *rounds_out = rounds;
+    return RHZ_OK;
+}
+
 // [D] far-field re-distancing with the reference round policy.
 int run_fsm(Ctx& c, const uint8_t* sign) {
     const int64_t V = c.g.n();
@@ -378,6 +425,11 @@ int run_fsm(Ctx& c, const uint8_t* sign) {
     const bool converge = c.p->fsm_policy == RHZ_FSM_CONVERGE;
     const int cap = converge ? c.p->fsm_round_cap : c.p->max_rounds;
     int rounds = 0;
+    const SweepTuning tune = sweep_tuning();
+    if (!converge && cap > 0 && cap <= kMaxBatchedRounds && V <= tune.batch_rounds_max_voxels) {
+        int rc = fsm_rounds_batched(c, b, cap, &rounds);
+        if (rc) return rc;
+    } else
     for (int r = 0; r < cap; ++r) {
         HIP_TRY(hipMemsetAsync(changed, 0, sizeof(int), c.s));
         int rc = sweep_round(c, c.d, b, changed);
@@ -1053,15 +1105,30 @@ int rhz_tuning_set(const char* key, int64_t value) {
     if (!key) return set_err(RHZ_ERR_ARG, "null tuning key");
     const SweepTuning t = sweep_tuning();
     if (!std::strcmp(key, "sweep_step")) {
-        if (!set_sweep_tuning((int)value, t.rows, t.fmajor)) return set_err(RHZ_ERR_ARG, "sweep_step must be 2, 4 or 8");
+        if (!set_sweep_tuning((int)value, t.rows, t.fmajor, t.lines)) return set_err(RHZ_ERR_ARG, "sweep_step must be 2, 4 or 8");
         return RHZ_OK;
     }
     if (!std::strcmp(key, "sweep_rows")) {
-        if (!set_sweep_tuning(t.step, (int)value, t.fmajor)) return set_err(RHZ_ERR_ARG, "sweep_rows must be 4, 8 or 16");
+        if (!set_sweep_tuning(t.step, (int)value, t.fmajor, t.lines)) return set_err(RHZ_ERR_ARG, "sweep_rows must be 4, 8 or 16");
         return RHZ_OK;
     }
     if (!std::strcmp(key, "sweep_fmajor")) {
-        if (!set_sweep_tuning(t.step, t.rows, (int)value)) return set_err(RHZ_ERR_ARG, "sweep_fmajor must be 0 or 1");
+        if (!set_sweep_tuning(t.step, t.rows, (int)value, t.lines)) return set_err(RHZ_ERR_ARG, "sweep_fmajor must be 0 or 1");
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "fsm_batch_rounds_max_voxels")) {
+        if (value < 0) return set_err(RHZ_ERR_ARG, "fsm_batch_rounds_max_voxels must be >= 0");
+        set_fsm_batching(value, t.fsm_graph);
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "fsm_graph")) {
+        if (value != 0 && value != 1) return set_err(RHZ_ERR_ARG, "fsm_graph must be 0 or 1");
+        set_fsm_batching(t.batch_rounds_max_voxels, (int)value);
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "sweep_lines")) {
+        if (!set_sweep_tuning(t.step, t.rows, t.fmajor, (int)value))
+            return set_err(RHZ_ERR_ARG, "sweep_lines must be 1, 2 or 3");
         return RHZ_OK;
     }
     return set_err(RHZ_ERR_ARG, "unknown tuning key '%s'", key);
@@ -1080,6 +1147,18 @@ int rhz_tuning_get(const char* key, int64_t* value) {
     }
     if (!std::strcmp(key, "sweep_fmajor")) {
         *value = t.fmajor;
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "sweep_lines")) {
+        *value = t.lines;
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "fsm_batch_rounds_max_voxels")) {
+        *value = t.batch_rounds_max_voxels;
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "fsm_graph")) {
+        *value = t.fsm_graph;
         return RHZ_OK;
     }
     return set_err(RHZ_ERR_ARG, "unknown tuning key '%s'", key);

@@ -42,3 +42,35 @@ def test_bake_matches_single_builds(rhz, geom, torch_gpu):
 def test_bake_rejects_bad_lane_count(rhz, geom, torch_gpu):
     with pytest.raises(rhz.RhzError):
         rhz.bake([], rhz.default_params(), n_lanes=0)
+
+
+@pytest.mark.parametrize("case", ["small_sphere", "config1"])
+def test_fsm_batched_rounds_and_graph_match_round_loop(rhz, torch_gpu, request, case):
+    """All FSM rounds enqueued at once, and the same rounds captured and
+    replayed as a hipGraph, give the round-by-round loop's field and round
+    count (rounds after a round without change change nothing)."""
+    torch = torch_gpu
+    tris, h, mn, mx = request.getfixturevalue(case)
+    V = int(np.prod([mx[i] - mn[i] + 1 for i in range(3)]))
+    d_t = torch.from_numpy(tris).cuda()
+    stream = torch.cuda.Stream()
+    saved = {k: rhz.tuning_get(k) for k in ("fsm_batch_rounds_max_voxels", "fsm_graph")}
+    outs, rounds = [], []
+    try:
+        for mv, g in ((0, 0), (1 << 40, 0), (1 << 40, 1), (1 << 40, 1)):
+            rhz.tuning_set("fsm_batch_rounds_max_voxels", mv)
+            rhz.tuning_set("fsm_graph", g)
+            out = torch.empty(V, dtype=torch.float32, device="cuda")
+            with torch.cuda.stream(stream):
+                rc, st = rhz.compute_sdf_device(d_t, len(tris), h, mn, mx, out,
+                                                rhz.default_params(method=rhz.METHOD_FAST_LEARNED), stream=stream)
+            stream.synchronize()
+            assert rc == rhz.RHZ_OK
+            outs.append(out)
+            rounds.append(st.eikonal_rounds)
+    finally:
+        for k, v in saved.items():
+            rhz.tuning_set(k, v)
+    for o in outs[1:]:
+        assert torch.equal(o.view(torch.int32), outs[0].view(torch.int32))
+    assert len(set(rounds)) == 1, rounds
