@@ -99,6 +99,11 @@ typedef struct rhz_sdf_params {
     int32_t learned_certify;      /* FAST_LEARNED: 1 = also probe every candidate whose voxel-to-ray
                                      segment may cross a triangle and that the ranking missed
                                      (0 sign errors on closed meshes); 0 = ranked probes only */
+    float far_field_beta;         /* FAST_LEARNED exact probes: triangle clusters farther than
+                                     beta x their radius (beta >= 2) enter through a two-term
+                                     multipole expansion with a rigorous error bound; voxels the
+                                     bound cannot decide go to the brute-force kernel (decisions
+                                     unchanged); 0 = brute force for every probe.  Default 4 */
     const float* mlp_weights;     /* FAST_LEARNED: proposer parameters, rhz_mlp_blob_count()
                                      floats (host memory); NULL = the built-in trained set */
 } rhz_sdf_params;
@@ -148,6 +153,10 @@ typedef struct rhz_sdf_stats {
     int64_t risky_candidates;     /* FAST_LEARNED: candidates the segment certificate flags */
     int64_t fallback_probes;      /* FAST_LEARNED: flagged candidates the ranking missed */
     int64_t fallback_mismatches;  /* FAST_LEARNED: scanline signs the fallback corrected */
+    int64_t far_decided;          /* exact probes decided by the far-field certificate */
+    int64_t far_undecided;        /* exact probes it handed to the brute-force kernel */
+    int64_t n_clusters;           /* triangle clusters of the certificate */
+    double ms_far_field;          /* device time of the certificate (incl. the point sort) */
 } rhz_sdf_stats;
 
 void rhz_sdf_params_default(rhz_sdf_params* p);

@@ -80,6 +80,7 @@ class Params(C.Structure):
         ("candidate_radius", C.c_float),
         ("scorer", C.c_int32),
         ("learned_certify", C.c_int32),
+        ("far_field_beta", C.c_float),
         ("mlp_weights", C.c_void_p),
     ]
 
@@ -128,6 +129,10 @@ class Stats(C.Structure):
         ("risky_candidates", C.c_int64),
         ("fallback_probes", C.c_int64),
         ("fallback_mismatches", C.c_int64),
+        ("far_decided", C.c_int64),
+        ("far_undecided", C.c_int64),
+        ("n_clusters", C.c_int64),
+        ("ms_far_field", C.c_double),
     ]
 
     def as_dict(self):

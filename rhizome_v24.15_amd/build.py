@@ -27,6 +27,7 @@ SOURCES = [
     "k_mesh.hip",
     "k_learned.hip",
     "k_mlp.hip",
+    "k_cluster.hip",
     "sdf_pipeline.cpp",
 ]
 

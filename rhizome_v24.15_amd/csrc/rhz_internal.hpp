@@ -118,6 +118,29 @@ hipError_t launch_winding_points(const float* verts, const uint32_t* idx, int64_
 
 // ---- k_learned.hip --------------------------------------------------------
 // Candidate voxels: band distance d[i] <= thr (appended to list, counter += n).
+// k_cluster: certified far-field winding decisions
+struct ClusterGrid {
+    float ox, oy, oz, inv_s;
+    int nx, ny, nz;
+};
+hipError_t launch_tri_box_area(const float* tris, int64_t nt, int* box, double* area_part, int nblocks,
+                               hipStream_t s);
+hipError_t launch_cluster_build(const float* tris, int64_t nt, const ClusterGrid& g, uint32_t* counts,
+                                uint32_t* cursor, int64_t* start, void* scan_scratch, int32_t* ids, float4* info,
+                                int2* range, unsigned long long* n_clusters, hipStream_t s);
+// Spatial order of list positions (pos: the m positions to sort, or NULL for
+// 0..m-1) by cluster-grid cell, into order[m].
+hipError_t launch_point_sort(const Grid& g, const int64_t* idx, const int64_t* pos, int64_t m, const ClusterGrid& cg,
+                             uint32_t* counts, uint32_t* cursor, int64_t* start, void* scan_scratch, int64_t* order,
+                             hipStream_t s);
+// Certified winding numbers of the n list positions order[0..n) (voxel ids in
+// idx); undecided positions are appended to `undecided`.
+hipError_t launch_winding_far(const float* tris, const float4* info, const int2* range, const int32_t* ids,
+                              int ncl, const Grid& g, const int64_t* idx, const int64_t* order, int64_t n, float beta,
+                              double* w, int64_t* undecided, unsigned long long* n_undecided, hipStream_t s);
+hipError_t launch_gather_i64(const int64_t* src, const int64_t* pos, int64_t m, int64_t* out, hipStream_t s);
+hipError_t launch_scatter_f64(const double* src, const int64_t* pos, int64_t m, double* dst, hipStream_t s);
+
 hipError_t launch_candidates(const float* d, int64_t n, float thr, int64_t* list, int64_t cap,
                              unsigned long long* counter, hipStream_t s);
 // 12 features per candidate (row-major n x 12) from the column candidate lists.

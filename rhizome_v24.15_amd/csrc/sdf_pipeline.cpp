@@ -88,6 +88,18 @@ enum Slot {
     S_HIST,        // score histogram
     S_MLP,         // proposer parameters (device copy)
     S_RISK,        // per-candidate segment-certificate flag
+    S_CL_BOX,      // triangle box (ordered ints) + per-block area partials
+    S_CL_COUNTS,   // triangles per cluster cell
+    S_CL_CURSOR,
+    S_CL_START,
+    S_CL_SCAN,
+    S_CL_IDS,      // triangle ids grouped by cell
+    S_CL_INFO,     // per cluster: centre, radius, area vector, area
+    S_CL_RANGE,    // per cluster: first id, count
+    S_UNDEC,       // positions the far-field certificate left undecided
+    S_UNDVOX,      // their voxel ids
+    S_W2,          // their brute-force winding numbers
+    S_ORDER,       // probe positions in spatial order
     S_COUNT
 };
 
@@ -167,7 +179,10 @@ void* ws_get(Workspace* w, int slot, size_t bytes) {
 }
 
 // counters: ull slots 0..15, int slots at byte offset 128 (32 ints)
-enum CounterId { C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4, C_SEL = 5, C_GROW = 6, C_RISKY = 7 };
+enum CounterId {
+    C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4, C_SEL = 5, C_GROW = 6, C_RISKY = 7, C_NCL = 8,
+    C_UNDEC = 9
+};
 enum FlagId { F_UNKNOWN = 0, F_CHANGED = 1, F_ROUND0 = 2 };  // F_ROUND0 + r: round r changed a voxel
 constexpr int kMaxBatchedRounds = 24;
 
@@ -241,6 +256,13 @@ struct Ctx {
     EventAcc* sweeps_t = nullptr;
     EventAcc* wind_t = nullptr;
     EventAcc* trans_t = nullptr;
+    EventAcc* far_t = nullptr;
+    // far-field clusters of this call's triangles (built on first use)
+    int ncl = -1;
+    ClusterGrid cl_grid{};
+    const float4* cl_info = nullptr;
+    const int2* cl_range = nullptr;
+    const int32_t* cl_ids = nullptr;
 };
 
 // Reads `count` ull counters starting at `first` into h_cnt (synchronises).
@@ -670,11 +692,156 @@ int run_plain_exact(Ctx& c) {
 #define RHZ_HAVE_BUILTIN_PROPOSER 1
 #endif
 
+// Spatial clusters of the call's triangles for the far-field certificate:
+// cells of a grid of side sqrt(24 x mean triangle area) over the triangles'
+// box (at most 4M cells), by centroid.
+int build_clusters(Ctx& c) {
+    if (c.ncl >= 0) return RHZ_OK;
+    const int nblocks = (int)std::min<int64_t>(1024, (c.nt + 255) / 256);
+    char* boxbuf = (char*)ws_get(c.w, S_CL_BOX, 64 + sizeof(double) * (size_t)nblocks);
+    if (!boxbuf) return set_err(RHZ_ERR_NOMEM, "cluster scratch allocation failed");
+    int* box = reinterpret_cast<int*>(boxbuf);
+    double* area_part = reinterpret_cast<double*>(boxbuf + 64);
+    const int init[6] = {0x7fffffff, 0x7fffffff, 0x7fffffff, (int)0x80000000, (int)0x80000000, (int)0x80000000};
+    HIP_TRY(hipMemcpyAsync(box, init, sizeof(init), hipMemcpyHostToDevice, c.s));
+    HIP_TRY(launch_tri_box_area(c.tris, c.nt, box, area_part, nblocks, c.s));
+    std::vector<char> hb(64 + sizeof(double) * (size_t)nblocks);
+    HIP_TRY(hipMemcpyAsync(hb.data(), boxbuf, hb.size(), hipMemcpyDeviceToHost, c.s));
+    HIP_TRY(hipStreamSynchronize(c.s));
+    const int* hbox = reinterpret_cast<const int*>(hb.data());
+    const double* hpart = reinterpret_cast<const double*>(hb.data() + 64);
+    auto unorder = [](int i) {
+        const int b = i >= 0 ? i : i ^ 0x7fffffff;
+        float f;
+        std::memcpy(&f, &b, 4);
+        return f;
+    };
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = unorder(hbox[k]);
+        hi[k] = unorder(hbox[3 + k]);
+    }
+    double area = 0.0;
+    for (int b = 0; b < nblocks; ++b) area += hpart[b];
+    double side = std::sqrt(24.0 * area / (double)std::max<int64_t>(c.nt, 1));
+    double ext = 0.0;
+    for (int k = 0; k < 3; ++k) ext = std::max(ext, (double)hi[k] - (double)lo[k]);
+    if (!(side > 0.0) || !std::isfinite(side)) side = std::max(ext, 1e-3);
+    side = std::max(side, ext / 1024.0);
+    ClusterGrid g;
+    int n3[3];
+    for (;;) {
+        int64_t cells = 1;
+        for (int k = 0; k < 3; ++k) {
+            n3[k] = (int)std::floor(((double)hi[k] - (double)lo[k]) / side) + 1;
+            cells *= n3[k];
+        }
+        if (cells <= (4ll << 20)) break;
+        side *= 1.26;
+    }
+    g.ox = lo[0];
+    g.oy = lo[1];
+    g.oz = lo[2];
+    g.inv_s = (float)(1.0 / side);
+    g.nx = n3[0];
+    g.ny = n3[1];
+    g.nz = n3[2];
+    const int64_t ncells = (int64_t)g.nx * g.ny * g.nz;
+    uint32_t* counts = (uint32_t*)ws_get(c.w, S_CL_COUNTS, sizeof(uint32_t) * (size_t)ncells);
+    uint32_t* cursor = (uint32_t*)ws_get(c.w, S_CL_CURSOR, sizeof(uint32_t) * (size_t)ncells);
+    int64_t* start = (int64_t*)ws_get(c.w, S_CL_START, sizeof(int64_t) * (size_t)(ncells + 1));
+    void* scan = ws_get(c.w, S_CL_SCAN, scan_scratch_bytes(ncells));
+    int32_t* ids = (int32_t*)ws_get(c.w, S_CL_IDS, sizeof(int32_t) * (size_t)c.nt);
+    float4* info = (float4*)ws_get(c.w, S_CL_INFO, sizeof(float4) * 5 * (size_t)ncells);
+    int2* range = (int2*)ws_get(c.w, S_CL_RANGE, sizeof(int2) * (size_t)ncells);
+    if (!counts || !cursor || !start || !scan || !ids || !info || !range)
+        return set_err(RHZ_ERR_NOMEM, "cluster allocation failed");
+    HIP_TRY(hipMemsetAsync(counts, 0, sizeof(uint32_t) * (size_t)ncells, c.s));
+    HIP_TRY(hipMemsetAsync(cursor, 0, sizeof(uint32_t) * (size_t)ncells, c.s));
+    HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_NCL), 0, sizeof(unsigned long long), c.s));
+    HIP_TRY(launch_cluster_build(c.tris, c.nt, g, counts, cursor, start, scan, ids, info, range, cnt_ptr(c.w, C_NCL),
+                                 c.s));
+    int rc = read_counters(c);
+    if (rc) return rc;
+    c.ncl = (int)hc(c, C_NCL);
+    c.cl_grid = g;
+    c.cl_info = info;
+    c.cl_range = range;
+    c.cl_ids = ids;
+    c.st->n_clusters = c.ncl;
+    return RHZ_OK;
+}
+
+// Winding numbers of `m` listed voxels into w: far-field certificate at
+// beta, brute-force exact kernel for the voxels it leaves undecided.  Probe voxels are taken in spatial order (counting
+// sort by cluster cell) so a wave's lanes share their near clusters.
+int winding_certified(Ctx& c, const int64_t* list, int64_t m, double* w) {
+    int rc = build_clusters(c);
+    if (rc) return rc;
+    const float beta = std::max(c.p->far_field_beta, 2.0f);  // the bound needs d > r
+    const ClusterGrid& cg = c.cl_grid;
+    const int64_t ncells = (int64_t)cg.nx * cg.ny * cg.nz;
+    uint32_t* counts = (uint32_t*)ws_get(c.w, S_CL_COUNTS, sizeof(uint32_t) * (size_t)ncells);
+    uint32_t* cursor = (uint32_t*)ws_get(c.w, S_CL_CURSOR, sizeof(uint32_t) * (size_t)ncells);
+    int64_t* start = (int64_t*)ws_get(c.w, S_CL_START, sizeof(int64_t) * (size_t)(ncells + 1));
+    void* scan = ws_get(c.w, S_CL_SCAN, scan_scratch_bytes(ncells));
+    int64_t* order = (int64_t*)ws_get(c.w, S_ORDER, sizeof(int64_t) * (size_t)m);
+    int64_t* und = (int64_t*)ws_get(c.w, S_UNDEC, sizeof(int64_t) * (size_t)m);
+    if (!counts || !cursor || !start || !scan || !order || !und)
+        return set_err(RHZ_ERR_NOMEM, "certificate scratch allocation failed");
+    HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_UNDEC), 0, sizeof(unsigned long long), c.s));
+    if (c.far_t) c.far_t->begin(c.s);
+    HIP_TRY(launch_point_sort(c.g, list, nullptr, m, cg, counts, cursor, start, scan, order, c.s));
+    HIP_TRY(launch_winding_far(c.tris, c.cl_info, c.cl_range, c.cl_ids, c.ncl, c.g, list, order, m, beta, w, und,
+                               cnt_ptr(c.w, C_UNDEC), c.s));
+    if (c.far_t) c.far_t->end(c.s);
+    rc = read_counters(c);
+    if (rc) return rc;
+    const int64_t nu = (int64_t)hc(c, C_UNDEC);
+    const int64_t* left = und;
+    c.st->far_decided += m - nu;
+    c.st->far_undecided += nu;
+    if (nu == 0) return RHZ_OK;
+    int64_t* vox = (int64_t*)ws_get(c.w, S_UNDVOX, sizeof(int64_t) * (size_t)nu);
+    double* w2 = (double*)ws_get(c.w, S_W2, sizeof(double) * (size_t)nu);
+    void* scr = ws_get(c.w, S_WSCR, winding_scratch_bytes(nu, c.nt));
+    int64_t* f64 = (int64_t*)ws_get(c.w, S_FP64, sizeof(int64_t) * (size_t)nu);
+    if (!vox || !w2 || !scr || !f64) return set_err(RHZ_ERR_NOMEM, "winding scratch allocation failed");
+    HIP_TRY(launch_gather_i64(list, left, nu, vox, c.s));
+    const int prec = c.p->winding_precision;
+    HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FP64), 0, sizeof(unsigned long long), c.s));
+    if (c.wind_t) c.wind_t->begin(c.s);
+    HIP_TRY(launch_winding(c.tris, c.nt, c.g, vox, 0, nu, prec, c.p->fp64_band, w2, scr, f64, cnt_ptr(c.w, C_FP64),
+                           c.s));
+    if (c.wind_t) c.wind_t->end(c.s);
+    c.st->winding_pairs += nu * c.nt;
+    if (prec != 0) {
+        rc = read_counters(c);
+        if (rc) return rc;
+        const int64_t nf = (int64_t)hc(c, C_FP64);
+        c.st->fp64_voxels += nf;
+        HIP_TRY(launch_winding_fp64(c.tris, c.nt, c.g, vox, 0, f64, nf, w2, c.s));
+    }
+    HIP_TRY(launch_scatter_f64(w2, left, nu, w, c.s));
+    return RHZ_OK;
+}
+
 // Exact decisions for `n` listed voxels committed into `sign`; voxels whose
 // sign changed are appended to mism (counter C_MISM, not reset here).
 int probe_commit(Ctx& c, const int64_t* list, int64_t n, uint8_t* sign, int64_t* mism, int64_t mism_cap) {
     const int prec = c.p->winding_precision;
     const int64_t chunk = std::min<int64_t>(n, 8ll << 20);
+    if (c.p->far_field_beta > 0.0f) {
+        for (int64_t off = 0; off < n; off += chunk) {
+            const int64_t m = std::min(chunk, n - off);
+            double* w = (double*)ws_get(c.w, S_W, sizeof(double) * (size_t)m);
+            if (!w) return set_err(RHZ_ERR_NOMEM, "winding buffer allocation failed");
+            int rc = winding_certified(c, list + off, m, w);
+            if (rc) return rc;
+            HIP_TRY(launch_commit(w, list + off, m, sign, mism, mism_cap, cnt_ptr(c.w, C_MISM), c.s));
+        }
+        return RHZ_OK;
+    }
     for (int64_t off = 0; off < n; off += chunk) {
         const int64_t m = std::min(chunk, n - off);
         double* w = (double*)ws_get(c.w, S_W, sizeof(double) * (size_t)m);
@@ -980,21 +1147,24 @@ int check_params(const rhz_sdf_params* p) {
 int dispatch_inner(Ctx& c);
 
 int dispatch(Ctx& c) {
-    EventAcc sw, wd, tr;
+    EventAcc sw, wd, tr, ff;
     c.sweeps_t = &sw;
     c.wind_t = &wd;
     c.trans_t = &tr;
+    c.far_t = &ff;
     int rc = dispatch_inner(c);
     if (rc >= 0 && hipStreamSynchronize(c.s) == hipSuccess) {
         c.st->ms_sweeps = sw.total_ms();
         c.st->ms_winding = wd.total_ms();
         c.st->ms_transposes = tr.total_ms();
+        c.st->ms_far_field = ff.total_ms();
         c.st->sweep_calls = (int64_t)(sw.used / 2);
         c.st->winding_calls = (int64_t)(wd.used / 2);
     }
     c.sweeps_t = nullptr;
     c.wind_t = nullptr;
     c.trans_t = nullptr;
+    c.far_t = nullptr;
     return rc;
 }
 
@@ -1087,6 +1257,7 @@ void rhz_sdf_params_default(rhz_sdf_params* p) {
     p->candidate_radius = 0.0f;
     p->scorer = 0;
     p->learned_certify = 1;
+    p->far_field_beta = 4.0f;
     p->mlp_weights = nullptr;
 }
 

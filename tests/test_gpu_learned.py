@@ -280,3 +280,35 @@ def test_network_gain_over_blind_probes(rhz, config1):
         ranked[scorer] = (st.ranked_probes, st.fallback_probes, st.reprobed_voxels)
     print("classifier mode (ranked, fallback, total): network", ranked[0], "margin", ranked[1])
     assert ranked[0][2] <= ranked[1][2]
+
+
+@pytest.mark.parametrize("case", ["config1", "two_cubes", "vessel_coarse"])
+def test_far_field_certificate_keeps_every_decision(rhz, geom, orc, request, case):
+    """The far-field certificate (clusters beyond beta x radius as bounded
+    dipoles, undecided voxels to the brute-force kernel) changes no sign:
+    fields are bit-identical for beta = 0 (brute force), 2, 6 and 20."""
+    if case == "config1":
+        tris, h, mn, mx = request.getfixturevalue("config1")
+    elif case == "two_cubes":
+        v, f = geom.two_cubes()
+        tris = orc.build_triangles(v, f)
+        h = 0.25
+        mn, mx = geom.frame_grid(v, h)
+    else:
+        v, f = geom.vessel()
+        tris = rhz.build_triangles(v, f)
+        h = 0.5
+        mn, mx = geom.frame_grid(v, h)
+    fields, stats = [], []
+    for beta in (0.0, 2.0, 6.0, 20.0):
+        rc, out, st = rhz.compute_sdf(tris, h, mn, mx, rhz.default_params(method=rhz.METHOD_FAST_LEARNED,
+                                                                           far_field_beta=beta))
+        assert rc == rhz.RHZ_OK
+        fields.append(out.reshape(-1))
+        stats.append(st)
+    for fo in fields[1:]:
+        assert np.array_equal(fo.view(np.uint32), fields[0].view(np.uint32))
+    st6 = stats[2]
+    print(case, "clusters", st6.n_clusters, "decided", st6.far_decided, "undecided", st6.far_undecided)
+    assert stats[0].far_decided == 0 and st6.far_decided + st6.far_undecided == st6.reprobed_voxels
+    assert st6.far_decided >= 0.9 * st6.reprobed_voxels
