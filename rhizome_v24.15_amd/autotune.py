@@ -14,7 +14,7 @@ from __future__ import annotations
 import time
 
 LANES = (1, 2, 4, 8)
-SWEEP_SHAPES = ((2, 4), (1, 8), (3, 8))          # (sweep_lines, sweep_step)
+SWEEP_SHAPES = ((2, 2), (2, 4), (4, 2), (1, 8))  # (sweep_lines, sweep_step)
 FSM_MODES = ((16 << 20, 0), (16 << 20, 1), (0, 0))  # (fsm_batch_rounds_max_voxels, fsm_graph)
 
 

@@ -283,125 +283,238 @@ __device__ __forceinline__ float lane_step(float cur, float fm, float fp, float 
     return godunov_min_w(cur, g[0], g[1], g[2], K);
 }
 
-// Branch-free form of the same update (every candidate root computed, then
-// selected): two independent lines per lane interleave without a branch
-// between them.  Bit-identical to godunov_min_w: the roots that the branchy
-// form would not reach are discarded by the selects, and where its early exit
-// returns cur the full formula returns cur too.
-template <int SF, int SW, int SC>
-__device__ __forceinline__ float lane_step_nb(float cur, float fm, float fp, float cm, float cp, float behind,
-                                              float next, const SweepConst& K) {
-    float g[3];
-    g[SF] = fminf(fm, fp);
-    g[SW] = fminf(behind, next);
-    g[SC] = fminf(cm, cp);
-    const float s0 = fminf(g[0], fminf(g[1], g[2]));
-    const float s2 = fmaxf(g[0], fmaxf(g[1], g[2]));
-    const float s1 = g[0] + g[1] + g[2] - s0 - s2;
+// ---- tile sweep (the default shape) --------------------------------------------
+// Every lane walks NF x NC lines at once: NF segments of the fast axis, 64
+// cells apart, times NC adjacent cross lines.  Per step a wave needs only its
+// own rows: the fast-axis neighbours are the adjacent lanes' cells, moved
+// through DPP wave shifts (lane 0 and lane 63 of the outer segments take one
+// extra "edge" load), and the cross-line neighbours inside the tile are the
+// lane's own registers; only the lines just outside the tile (c0 - 1 and
+// c0 + NC) are loaded.  The update is branch-free, so the NF x NC chains
+// interleave.  Waves that touch no lattice face skip the face selects.
+//
+// The square roots use the short correctly rounded form: with
+// 2^-30 <= h <= 2^30 every positive discriminant is >= h^2 2^-25 >= 2^-96.
+// (disc = RN(a - b) for floats a ~ 2h^2 or 3h^2 and b >= 0: either b <= a/2
+// and disc >= a/2, or a - b is exact and a multiple of b's unit in the last
+// place, > b 2^-24 > a 2^-25.)  A zero discriminant gives root 0; a negative
+// one is never selected.
+__device__ __forceinline__ float sqrt_disc(float disc) {
+    const bool pos = disc > 0.0f;
+    const float r = sqrt_rn_normal(pos ? disc : 1.0f);
+    return pos ? r : 0.0f;
+}
+
+// godunov_min with every candidate root computed and then selected: the roots
+// the branchy form would not reach are discarded by the selects, and where its
+// early exit returns cur the full formula returns cur too, so the results are
+// identical bit for bit.
+__device__ __forceinline__ float godunov_select(float cur, float a0, float a1, float a2, const SweepConst& K) {
+    const float s0 = fminf(a0, fminf(a1, a2));
+    const float s2 = fmaxf(a0, fmaxf(a1, a2));
+    const float s1 = a0 + a1 + a2 - s0 - s2;
     const float u1 = s0 + K.h;
     const float d10 = s1 - s0;
     const float disc = K.h2x2 - d10 * d10;
-    const float u2 = (s0 + s1 + sqrtf(fmaxf(disc, 0.0f))) * 0.5f;
+    const float u2 = (s0 + s1 + sqrt_disc(disc)) * 0.5f;
     const float d21 = s2 - s1, d20 = s2 - s0;
     const float disc3 = K.h2x3 - (d10 * d10 + d21 * d21 + d20 * d20) * 0.5f;
-    const float u3 = div3(s0 + s1 + s2 + sqrtf(fmaxf(disc3, 0.0f)));
+    const float u3 = div3(s0 + s1 + s2 + sqrt_disc(disc3));
     const bool two = (u1 > s1) && (disc >= 0.0f);
     const bool three = two && (u2 > s2) && (disc3 >= 0.0f);
     const float u = three ? u3 : (two ? u2 : u1);
     return (u < cur) ? u : cur;
 }
 
-// LPL lines per lane: lane l walks lines f = 64 LPL bf + l + 64 j, j < LPL.
-template <int SF, int SW, int SC, int DIR, int STEP, int LPL>
-__global__ __launch_bounds__(256) void sweep_lane2_kernel(const float* __restrict__ src, float* __restrict__ dst,
-                                                          int nf, int nw, int nc, int64_t ws, int64_t cs, int nbf,
-                                                          unsigned nblocks, float h, int* __restrict__ changed) {
-    const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
-    const int bf = (int)(lb % (unsigned)nbf);
-    const int c = __builtin_amdgcn_readfirstlane((int)(lb / (unsigned)nbf) * 4 + (int)threadIdx.y);
-    if (c >= nc) return;
+// Wave shifts by one lane (DPP): lane i takes lane i - 1 (shr) or i + 1 (shl);
+// the lane with no source keeps `old`.  ror / rol wrap around the wave.
+__device__ __forceinline__ float wave_shr1(float old, float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float wave_shl1(float old, float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float wave_ror1(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13c, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float wave_rol1(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xf, 0xf, false));
+}
+
+template <int SF, int SW, int SC, int DIR, int STEP, int NF, int NC, bool FACE>
+__device__ __forceinline__ bool tile_walk(const float* __restrict__ src, float* __restrict__ dst, int nf, int nw,
+                                          int nc, int64_t ws, int64_t cs, int bf, int c0, const SweepConst& K) {
     const int lane = threadIdx.x;
-    const int64_t ocm = (c > 0) ? -cs : cs;
-    const int64_t ocp = (c < nc - 1) ? cs : -cs;
-    const SweepConst K = sweep_const(h);
-    const int64_t A = cs + ws + 1;
-    uint32_t v_fm[LPL], v_fp[LPL], v_cm[LPL], v_cp[LPL], v_nx[LPL], v_st[LPL];
+    const int64_t A = cs + ws + 1;  // resource base = row - A: every offset below is >= 0
+    const bool need_lo = c0 > 0;
+    const bool need_hi = c0 + NC <= nc - 1;
+    bool lo_face[NC], hi_face[NC];
+    int64_t coff[NC];
 #pragma unroll
-    for (int j = 0; j < LPL; ++j) {
-        const int f = (bf * LPL + j) * 64 + lane;
+    for (int i = 0; i < NC; ++i) {
+        const int c = (c0 + i < nc) ? c0 + i : nc - 1;  // lines past the end repeat the last one, unstored
+        lo_face[i] = c == 0;
+        hi_face[i] = c == nc - 1;
+        coff[i] = (int64_t)(c - c0) * cs;
+    }
+    // byte offsets of the NEXT row's cells (the loads run one step ahead)
+    uint32_t v_nx[NC][NF], v_st[NC][NF], v_lo[NF], v_hi[NF], v_ed[NC];
+    bool hasl[NF], hasr[NF];
+#pragma unroll
+    for (int j = 0; j < NF; ++j) {
+        const int f = (bf * NF + j) * 64 + lane;
         const bool live = f < nf;
         const int fl = live ? f : nf - 1;
-        const int ofm = (fl > 0) ? -1 : 1;
-        const int ofp = (fl < nf - 1) ? 1 : -1;
-        v_fm[j] = (uint32_t)((fl + ofm + A) * 4);
-        v_fp[j] = (uint32_t)((fl + ofp + A) * 4);
-        v_cm[j] = (uint32_t)((fl + ocm + A) * 4);
-        v_cp[j] = (uint32_t)((fl + ocp + A) * 4);
-        v_nx[j] = (uint32_t)((fl + (int64_t)DIR * ws + A) * 4);
-        v_st[j] = live ? (uint32_t)(f * 4) : 0x7ffffff0u;
+        hasl[j] = fl > 0;
+        hasr[j] = fl < nf - 1;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            v_nx[i][j] = (uint32_t)((fl + coff[i] + (int64_t)DIR * ws + A) * 4);
+            v_st[i][j] = (live && c0 + i < nc) ? (uint32_t)((coff[i] + f) * 4) : 0x7ffffff0u;
+        }
+        v_lo[j] = (uint32_t)((fl - cs + (int64_t)DIR * ws + A) * 4);
+        v_hi[j] = (uint32_t)((fl + (int64_t)NC * cs + (int64_t)DIR * ws + A) * 4);
     }
-    const uint32_t nrec = (uint32_t)((2 * A + nf + 64 * LPL) * 4);
-    const uint32_t nrow = (uint32_t)(nf * 4);
+    {
+        // lane 63 reads the cell right of the last segment, every other lane the
+        // cell left of the first (clamped into the row; unused at a face)
+        const int f0 = bf * NF * 64;
+        const int fr = f0 + NF * 64 - 1;
+        const int fe = (lane == 63) ? ((fr < nf - 1) ? fr + 1 : nf - 1) : ((f0 > 0) ? f0 - 1 : 0);
+#pragma unroll
+        for (int i = 0; i < NC; ++i) v_ed[i] = (uint32_t)((fe + coff[i] + (int64_t)DIR * ws + A) * 4);
+    }
+    const uint32_t nrec = (uint32_t)((2 * A + (int64_t)NC * cs + nf + 64 * NF) * 4);
+    const uint32_t nrows = (uint32_t)(((int64_t)(NC - 1) * cs + nf) * 4);
     const int64_t step = (int64_t)DIR * ws;
-    int64_t row = (int64_t)c * cs + (int64_t)((DIR > 0) ? 0 : nw - 1) * ws;
-    float behind[LPL], cur[LPL];
+    int64_t row = (int64_t)c0 * cs + (int64_t)((DIR > 0) ? 0 : nw - 1) * ws;
+    float behind[NC][NF], cur[NC][NF], edge[NC], hlo[NF], hhi[NF];
     bool chg = false;
     {
         const __amdgpu_buffer_rsrc_t r0 = rsrc_at(src + row - A, nrec);
+        const uint32_t back = (uint32_t)(step * 4);
 #pragma unroll
-        for (int j = 0; j < LPL; ++j) {
-            behind[j] = kFar;
-            cur[j] = bload(r0, v_nx[j] - (uint32_t)(step * 4));
+        for (int i = 0; i < NC; ++i) {
+#pragma unroll
+            for (int j = 0; j < NF; ++j) {
+                behind[i][j] = kFar;
+                cur[i][j] = bload(r0, v_nx[i][j] - back);
+            }
+            edge[i] = bload(r0, v_ed[i] - back);
+        }
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            hlo[j] = need_lo ? bload(r0, v_lo[j] - back) : 0.0f;
+            hhi[j] = need_hi ? bload(r0, v_hi[j] - back) : 0.0f;
         }
     }
+    // one step: update every cell of the row from cur / edge / halo and the
+    // next row nx, store, and shift the next row in
+    auto advance = [&](const float (&nx)[NC][NF], const float (&ed)[NC], const float (&lo)[NF],
+                       const float (&hi)[NF]) {
+        const __amdgpu_buffer_rsrc_t w = rsrc_at(dst + row, nrows);
+        float nv[NC][NF];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+#pragma unroll
+            for (int j = 0; j < NF; ++j) {
+                const float left = (j == 0) ? wave_shr1(edge[i], cur[i][0])
+                                            : wave_ror1((lane == 63) ? cur[i][j > 0 ? j - 1 : 0] : cur[i][j]);
+                const float right = (j == NF - 1)
+                                        ? wave_shl1(edge[i], cur[i][NF - 1])
+                                        : wave_rol1((lane == 0) ? cur[i][j < NF - 1 ? j + 1 : 0] : cur[i][j]);
+                float fm = left, fp = right;
+                float cm = (i > 0) ? cur[i > 0 ? i - 1 : 0][j] : hlo[j];
+                float cp = (i < NC - 1) ? cur[i < NC - 1 ? i + 1 : 0][j] : hhi[j];
+                if (FACE) {  // a missing neighbour is replaced by the opposite one
+                    fm = hasl[j] ? left : right;
+                    fp = hasr[j] ? right : left;
+                    if (lo_face[i]) cm = cp;
+                    if (hi_face[i]) cp = cm;
+                }
+                float g[3];
+                g[SF] = fminf(fm, fp);
+                g[SW] = fminf(behind[i][j], nx[i][j]);
+                g[SC] = fminf(cm, cp);
+                nv[i][j] = godunov_select(cur[i][j], g[0], g[1], g[2], K);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+#pragma unroll
+            for (int j = 0; j < NF; ++j) {
+                chg |= nv[i][j] < cur[i][j];
+                bstore(w, v_st[i][j], nv[i][j]);
+                behind[i][j] = nv[i][j];
+                cur[i][j] = nx[i][j];
+            }
+            edge[i] = ed[i];
+        }
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            hlo[j] = lo[j];
+            hhi[j] = hi[j];
+        }
+        row += step;
+    };
     int s0 = 0;
+    // full groups: every step of the group has a next row on the line
     for (; s0 + STEP < nw; s0 += STEP) {
-        float fm[STEP][LPL], fp[STEP][LPL], cm[STEP][LPL], cp[STEP][LPL], nx[STEP][LPL];
+        float nx[STEP][NC][NF], ed[STEP][NC], lo[STEP][NF], hi[STEP][NF];
 #pragma unroll
         for (int k = 0; k < STEP; ++k) {
             const __amdgpu_buffer_rsrc_t r = rsrc_at(src + (row + k * step) - A, nrec);
 #pragma unroll
-            for (int j = 0; j < LPL; ++j) {
-                fm[k][j] = bload(r, v_fm[j]);
-                fp[k][j] = bload(r, v_fp[j]);
-                cm[k][j] = bload(r, v_cm[j]);
-                cp[k][j] = bload(r, v_cp[j]);
-                nx[k][j] = bload(r, v_nx[j]);
+            for (int i = 0; i < NC; ++i) {
+#pragma unroll
+                for (int j = 0; j < NF; ++j) nx[k][i][j] = bload(r, v_nx[i][j]);
+                ed[k][i] = bload(r, v_ed[i]);
+            }
+#pragma unroll
+            for (int j = 0; j < NF; ++j) {
+                lo[k][j] = need_lo ? bload(r, v_lo[j]) : 0.0f;
+                hi[k][j] = need_hi ? bload(r, v_hi[j]) : 0.0f;
             }
         }
 #pragma unroll
-        for (int k = 0; k < STEP; ++k) {
-            const __amdgpu_buffer_rsrc_t w = rsrc_at(dst + row, nrow);
-#pragma unroll
-            for (int j = 0; j < LPL; ++j) {
-                const float nv = lane_step_nb<SF, SW, SC>(cur[j], fm[k][j], fp[k][j], cm[k][j], cp[k][j], behind[j],
-                                                          nx[k][j], K);
-                chg |= nv < cur[j];
-                bstore(w, v_st[j], nv);
-                behind[j] = nv;
-                cur[j] = nx[k][j];
-            }
-            row += step;
-        }
+        for (int k = 0; k < STEP; ++k) advance(nx[k], ed[k], lo[k], hi[k]);
     }
+    // the last steps one at a time; past the end of the line next = 1e30
     for (; s0 < nw; ++s0) {
-        const __amdgpu_buffer_rsrc_t r = rsrc_at(src + row - A, nrec);
-        const __amdgpu_buffer_rsrc_t w = rsrc_at(dst + row, nrow);
         const bool more = s0 + 1 < nw;
+        const __amdgpu_buffer_rsrc_t r = rsrc_at(src + row - A, nrec);
+        float nx[NC][NF], ed[NC], lo[NF], hi[NF];
 #pragma unroll
-        for (int j = 0; j < LPL; ++j) {
-            const float a = bload(r, v_fm[j]), b = bload(r, v_fp[j]), cmv = bload(r, v_cm[j]),
-                        cpv = bload(r, v_cp[j]);
-            const float next = more ? bload(r, v_nx[j]) : kFar;
-            const float nv = lane_step_nb<SF, SW, SC>(cur[j], a, b, cmv, cpv, behind[j], next, K);
-            chg |= nv < cur[j];
-            bstore(w, v_st[j], nv);
-            behind[j] = nv;
-            cur[j] = next;
+        for (int i = 0; i < NC; ++i) {
+#pragma unroll
+            for (int j = 0; j < NF; ++j) nx[i][j] = more ? bload(r, v_nx[i][j]) : kFar;
+            ed[i] = more ? bload(r, v_ed[i]) : 0.0f;
         }
-        row += step;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            lo[j] = (more && need_lo) ? bload(r, v_lo[j]) : 0.0f;
+            hi[j] = (more && need_hi) ? bload(r, v_hi[j]) : 0.0f;
+        }
+        advance(nx, ed, lo, hi);
     }
-    if (__ballot(chg) != 0 && lane == 0) atomicOr(changed, 1);
+    return chg;
+}
+
+// Block = 4 waves; wave (bf, g) takes fast-axis cells [64 NF bf, 64 NF (bf + 1))
+// of cross lines [NC g, NC g + NC).
+template <int SF, int SW, int SC, int DIR, int STEP, int NF, int NC>
+__global__ __launch_bounds__(256) void sweep_tile_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                                         int nf, int nw, int nc, int64_t ws, int64_t cs, int nbf,
+                                                         unsigned nblocks, float h, int* __restrict__ changed) {
+    const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
+    const int bf = (int)(lb % (unsigned)nbf);
+    const int c0 = __builtin_amdgcn_readfirstlane(((int)(lb / (unsigned)nbf) * 4 + (int)threadIdx.y) * NC);
+    if (c0 >= nc) return;
+    const SweepConst K = sweep_const(h);
+    const bool face = c0 == 0 || c0 + NC >= nc || bf == 0 || (bf + 1) * NF * 64 >= nf;
+    const bool chg = face ? tile_walk<SF, SW, SC, DIR, STEP, NF, NC, true>(src, dst, nf, nw, nc, ws, cs, bf, c0, K)
+                          : tile_walk<SF, SW, SC, DIR, STEP, NF, NC, false>(src, dst, nf, nw, nc, ws, cs, bf, c0, K);
+    if (__ballot(chg) != 0 && threadIdx.x == 0) atomicOr(changed, 1);
 }
 
 template <int SF, int SW, int SC, int DIR, int STEP, int ROWS, bool FMAJ>
@@ -537,11 +650,11 @@ __global__ void redblack_kernel(float* __restrict__ g, int dx, int dy, int dz, f
 
 // Launch shape of the lane sweeps (steps per load group, waves per block);
 // set by the tuning entry point, read at every launch.
-static SweepTuning g_sweep_tuning = {4, 4, 0, 2, 16ll << 20, 0};
+static SweepTuning g_sweep_tuning = {2, 4, 0, 2, 16ll << 20, 0};
 SweepTuning sweep_tuning() { return g_sweep_tuning; }
 bool set_sweep_tuning(int step, int rows, int fmajor, int lines) {
     if ((step != 2 && step != 4 && step != 8) || (rows != 4 && rows != 8 && rows != 16) ||
-        (fmajor != 0 && fmajor != 1) || lines < 1 || lines > 3)
+        (fmajor != 0 && fmajor != 1) || (lines != 1 && lines != 2 && lines != 4))
         return false;
     g_sweep_tuning.step = step;
     g_sweep_tuning.rows = rows;
@@ -573,14 +686,14 @@ hipError_t launch_lane_v(const float* src, float* dst, int nf, int nw, int nc, i
     return hipGetLastError();
 }
 
-template <int SF, int SW, int SC, int DIR, int STEP, int LPL>
-hipError_t launch_lane2(const float* src, float* dst, int nf, int nw, int nc, int64_t ws, int64_t cs, float h,
-                        int* changed, hipStream_t s) {
-    const int nbf = (nf + 64 * LPL - 1) / (64 * LPL);
-    const int64_t blocks = (int64_t)nbf * ((nc + 3) / 4);
-    dim3 blk(64, 4);
-    sweep_lane2_kernel<SF, SW, SC, DIR, STEP, LPL><<<(unsigned)blocks, blk, 0, s>>>(src, dst, nf, nw, nc, ws, cs, nbf,
-                                                                                  (unsigned)blocks, h, changed);
+template <int SF, int SW, int SC, int DIR, int STEP, int NF, int NC>
+hipError_t launch_tile(const float* src, float* dst, int nf, int nw, int nc, int64_t ws, int64_t cs, float h,
+                       int* changed, hipStream_t s) {
+    const int nbf = (nf + 64 * NF - 1) / (64 * NF);
+    const int ngroups = (nc + NC - 1) / NC;
+    const int64_t blocks = (int64_t)nbf * ((ngroups + 3) / 4);
+    sweep_tile_kernel<SF, SW, SC, DIR, STEP, NF, NC><<<(unsigned)blocks, dim3(64, 4), 0, s>>>(
+        src, dst, nf, nw, nc, ws, cs, nbf, (unsigned)blocks, h, changed);
     return hipGetLastError();
 }
 
@@ -588,13 +701,15 @@ template <int SF, int SW, int SC, int DIR>
 hipError_t launch_lane_d(const float* src, float* dst, int nf, int nw, int nc, int64_t ws, int64_t cs, float h,
                          int* changed, hipStream_t s) {
     const SweepTuning t = sweep_tuning();
-    if (t.lines == 2) {
-        if (t.step == 2) return launch_lane2<SF, SW, SC, DIR, 2, 2>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
-        return launch_lane2<SF, SW, SC, DIR, 4, 2>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+    if (t.lines == 4) {
+        if (t.step == 2) return launch_tile<SF, SW, SC, DIR, 2, 2, 2>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+        if (t.step == 8) return launch_tile<SF, SW, SC, DIR, 8, 2, 2>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+        return launch_tile<SF, SW, SC, DIR, 4, 2, 2>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
     }
-    if (t.lines == 3) {  // one line per lane, branch-free update
-        if (t.step == 8) return launch_lane2<SF, SW, SC, DIR, 8, 1>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
-        return launch_lane2<SF, SW, SC, DIR, 4, 1>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+    if (t.lines == 2) {
+        if (t.step == 2) return launch_tile<SF, SW, SC, DIR, 2, 2, 1>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+        if (t.step == 8) return launch_tile<SF, SW, SC, DIR, 8, 2, 1>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+        return launch_tile<SF, SW, SC, DIR, 4, 2, 1>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
     }
     if (t.step == 8) {
         if (t.rows == 4) return launch_lane_v<SF, SW, SC, DIR, 8, 4>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
@@ -644,8 +759,12 @@ hipError_t launch_generic_sweep(const float* src, float* dst, const Grid& g, int
 }  // namespace
 
 bool lane_sweeps_ok(const Grid& g) {
+    // 32-bit byte offsets: the largest buffer record, 4 (4 dx dy + 3 max(dx, dy)
+    // + 130) bytes, stays below 2^32 for planes of < 2^27 cells.  The tile
+    // sweep's short square root needs 2^-30 <= h <= 2^30.
     const int64_t dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
-    return dx >= 2 && dy >= 2 && dz >= 2 && dx * dy < (int64_t(1) << 28);
+    return dx >= 2 && dy >= 2 && dz >= 2 && dx * dy < (int64_t(1) << 27) && g.h >= 0x1.0p-30f &&
+           g.h <= 0x1.0p30f;
 }
 
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s) {

@@ -72,15 +72,15 @@ hipError_t launch_parity(const float* tris, const Grid& g, int sup, const CsrBuf
 // ---- k_fsm.hip ------------------------------------------------------------
 hipError_t launch_sweep(const float* src, float* dst, const Grid& g, int axis, int dir,
                         int* changed, hipStream_t s);
-// True when the lane-walker sweeps apply (every axis has >= 2 cells and a
-// plane has < 2^29 cells); otherwise launch_sweep uses generic kernels.
+// True when the lane-walker sweeps apply (every axis has >= 2 cells, a plane
+// has < 2^27 cells, 2^-30 <= h <= 2^30); otherwise launch_sweep uses generic kernels.
 bool lane_sweeps_ok(const Grid& g);
 struct SweepTuning {
     int step;  // walk steps whose loads are issued together (2, 4, 8)
     int rows;  // waves per block (4, 8, 16)
     int fmajor;  // 1: a block's waves are adjacent segments of one row; 0: adjacent rows
-    int lines;   // 1: one line per lane; 2: two lines per lane, branch-free update;
-                 // 3: one line per lane, branch-free update
+    int lines;   // lines per lane: 1 (one-line walker, rows / fmajor apply), 2 (tile of two
+                 // fast-axis segments), 4 (tile of 2 segments x 2 cross lines)
     int64_t batch_rounds_max_voxels;  // lattices up to this size enqueue all FSM rounds at once
     int fsm_graph;                    // 1: those rounds are captured and replayed as one hipGraph
 };

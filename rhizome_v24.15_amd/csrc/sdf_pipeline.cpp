@@ -1299,7 +1299,7 @@ int rhz_tuning_set(const char* key, int64_t value) {
     }
     if (!std::strcmp(key, "sweep_lines")) {
         if (!set_sweep_tuning(t.step, t.rows, t.fmajor, (int)value))
-            return set_err(RHZ_ERR_ARG, "sweep_lines must be 1, 2 or 3");
+            return set_err(RHZ_ERR_ARG, "sweep_lines must be 1, 2 or 4");
         return RHZ_OK;
     }
     return set_err(RHZ_ERR_ARG, "unknown tuning key '%s'", key);

@@ -161,6 +161,58 @@ def test_sweep_odd_shapes(rhz, orc, torch_gpu, shape):
             assert np.array_equal(_host(dd).view(np.uint32), ref.view(np.uint32)), (shape, axis, direction)
 
 
+@pytest.mark.parametrize("lines,step", [(2, 2), (2, 4), (2, 8), (4, 2), (4, 4), (4, 8), (1, 4), (1, 8)])
+def test_sweep_tunings_bit_exact(rhz, orc, torch_gpu, lines, step):
+    """Every sweep launch shape equals the oracle bit for bit, on a lattice
+    with interior tiles (no lattice face) as well as ragged face tiles, and at
+    a voxel size below the short square root's range (generic kernels)."""
+    torch = torch_gpu
+    saved = {k: rhz.tuning_get(k) for k in ("sweep_lines", "sweep_step")}
+    rhz.tuning_set("sweep_lines", lines)
+    rhz.tuning_set("sweep_step", step)
+    try:
+        rng = np.random.default_rng(11)
+        for shape, h in (((300, 71, 41), 0.2), ((130, 9, 7), 1e-10)):
+            dx, dy, dz = shape
+            src = np.full(dx * dy * dz, 1e30, dtype=np.float32)
+            seeds = rng.choice(src.size, size=max(1, src.size // 60), replace=False)
+            src[seeds] = (rng.uniform(0.0, 4.0, size=len(seeds)) * h / 0.2).astype(np.float32)
+            dims = np.array(shape, dtype=np.int32)
+            # one propagated round first, so 2-D and 3-D updates occur
+            tmp = np.empty_like(src)
+            orc.lib().orc_fsm_sweep_db(src, tmp, orc._i32(dims), h, 2, 1)
+            src = tmp
+            for axis in range(3):
+                for direction in (1, -1):
+                    ref = np.empty_like(src)
+                    ch_ref = orc.lib().orc_fsm_sweep_db(src, ref, orc._i32(dims), h, axis, direction)
+                    ds = _dev(torch, src)
+                    dd = torch.empty_like(ds)
+                    ch = torch.zeros(1, dtype=torch.int32, device="cuda")
+                    rhz.Stages.sweep(ds, dd, dims, h, axis, direction, ch)
+                    torch.cuda.synchronize()
+                    got = _host(dd)
+                    assert int(ch.item()) == ch_ref, (shape, axis, direction)
+                    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), \
+                        (shape, axis, direction, int((got != ref).sum()))
+            # a full round: the x-sweeps run on the transposed copy
+            ref = src.copy()
+            ch_ref = 0
+            for axis in range(3):
+                for direction in (1, -1):
+                    ch_ref |= orc.lib().orc_fsm_sweep(ref, orc._i32(dims), h, axis, direction)
+            a = _dev(torch, src)
+            b = torch.empty_like(a)
+            ch = torch.zeros(1, dtype=torch.int32, device="cuda")
+            rhz.Stages.sweep_round(a, b, dims, h, ch)
+            torch.cuda.synchronize()
+            assert int(ch.item()) == ch_ref
+            assert np.array_equal(_host(a).view(np.uint32), ref.view(np.uint32)), (shape, "round")
+    finally:
+        for k, v in saved.items():
+            rhz.tuning_set(k, v)
+
+
 @pytest.mark.parametrize("shape", [(2, 2, 2), (65, 66, 3), (130, 67, 9), (883, 10, 4)])
 def test_sweep_round_bit_exact(rhz, orc, torch_gpu, shape):
     """A full round as the pipeline runs it (x-sweeps on the transposed copy)
