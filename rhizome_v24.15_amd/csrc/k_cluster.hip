@@ -194,29 +194,29 @@ __device__ __forceinline__ void voxel_pos(const Pts& P, int64_t v, float& px, fl
 __global__ __launch_bounds__(kBlock) void winding_far_kernel(const float* __restrict__ tris,
                                                              const float4* __restrict__ info,
                                                              const int2* __restrict__ range,
-                                                             const int32_t* __restrict__ ids, int ncl, Pts P,
-                                                             int64_t n, float beta, double* __restrict__ w,
-                                                             int64_t* __restrict__ undecided,
-                                                             unsigned long long* __restrict__ n_undecided) {
+                                                             const int32_t* __restrict__ ids, int ncl, int per_part,
+                                                             Pts P, int64_t n, float beta,
+                                                             double* __restrict__ part_w,
+                                                             float* __restrict__ part_err) {
+    // blockIdx.y selects the cluster range [per_part y, per_part (y + 1));
+    // the partial sums go to row y of the part arrays, at the sorted position.
     const int64_t k0 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool live = k0 < n;
-    int64_t i = 0;
     float px = 0.f, py = 0.f, pz = 0.f;
-    if (live) {
-        i = P.order[k0];
-        voxel_pos(P, P.idx[i], px, py, pz);
-    }
+    if (live) voxel_pos(P, P.idx[P.order[k0]], px, py, pz);
     float sum = 0.0f, comp = 0.0f;  // near field, compensated
     double far = 0.0;
     float err = 0.0f;
     const float beta2 = beta * beta;
-    for (int k0 = 0; k0 < ncl; k0 += kCl) {
+    const int kb = (int)blockIdx.y * per_part;
+    const int ke = min(ncl, kb + per_part);
+    for (int kc = kb; kc < ke; kc += kCl) {
         bool nearv[kCl];
 #pragma unroll
         for (int q = 0; q < kCl; ++q) {
-            const int k = k0 + q;
+            const int k = kc + q;
             nearv[q] = false;
-            if (k >= ncl) continue;
+            if (k >= ke) continue;
             const float4 a = info[kInfo * k], b = info[kInfo * k + 1], m0 = info[kInfo * k + 2],
                          m1 = info[kInfo * k + 3], m2 = info[kInfo * k + 4];
             const float dx = a.x - px, dy = a.y - py, dz = a.z - pz;
@@ -235,15 +235,17 @@ __global__ __launch_bounds__(kBlock) void winding_far_kernel(const float* __rest
                 const float mz = __builtin_fmaf(m2.x, hz, __builtin_fmaf(m1.w, hy, m1.z * hx));
                 const float xmx = __builtin_fmaf(hz, mz, __builtin_fmaf(hy, my, hx * mx));
                 far += (double)((dip + __builtin_fmaf(-3.0f, xmx, m2.y)) * inv3);
+                // remainder bound 3 r^2 A / (d - r)^4; the hardware reciprocal
+                // (within 1 ulp) is covered by the bound's 1% inflation
                 const float g = d - a.w;
                 const float g2 = g * g;
-                err += 3.0f * a.w * a.w * b.w / (g2 * g2);
+                err += 3.0f * a.w * a.w * b.w * __builtin_amdgcn_rcpf(g2 * g2);
             }
         }
 #pragma unroll
         for (int q = 0; q < kCl; ++q) {
-            const int k = k0 + q;
-            if (k >= ncl) continue;
+            const int k = kc + q;
+            if (k >= ke) continue;
             if (__ballot(live && nearv[q]) != 0) {
                 const int2 rg = range[k];
                 for (int j = 0; j < rg.y; ++j) {
@@ -259,12 +261,36 @@ __global__ __launch_bounds__(kBlock) void winding_far_kernel(const float* __rest
             }
         }
     }
-    bool decided = false;
     if (live) {
+        const int64_t at = (int64_t)blockIdx.y * n + k0;
+        part_w[at] = ((double)sum - (double)comp + far);
+        part_err[at] = err;
+    }
+}
+
+// Sum of the partial rows in a fixed order (deterministic), the winding
+// number, and the decision: undecided positions are appended to `undecided`.
+__global__ __launch_bounds__(kBlock) void winding_far_finalize_kernel(const double* __restrict__ part_w,
+                                                                      const float* __restrict__ part_err, int parts,
+                                                                      const int64_t* __restrict__ order, int64_t n,
+                                                                      double* __restrict__ w,
+                                                                      int64_t* __restrict__ undecided,
+                                                                      unsigned long long* __restrict__ n_undecided) {
+    const int64_t k0 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = k0 < n;
+    bool decided = true;
+    int64_t i = 0;
+    if (live) {
+        double sw = 0.0, se = 0.0;
+        for (int q = 0; q < parts; ++q) {
+            sw += part_w[(int64_t)q * n + k0];
+            se += (double)part_err[(int64_t)q * n + k0];
+        }
         constexpr double kInv4PiD = 0.07957747154594767;
-        const double wv = ((double)sum - (double)comp + far) * kInv4PiD;
-        const double bound = (double)err * kInv4PiD * 1.01 + 2e-3;
+        const double wv = sw * kInv4PiD;
+        const double bound = se * kInv4PiD * 1.01 + 2e-3;
         decided = fabs(fabs(wv) - 0.5) > bound;
+        i = order[k0];
         w[i] = wv;
     }
     const long long slot = wave_append(n_undecided, live && !decided);
@@ -368,13 +394,35 @@ hipError_t launch_point_sort(const Grid& g, const int64_t* idx, const int64_t* p
     return hipGetLastError();
 }
 
+int far_parts(int64_t n, int ncl) {
+    // enough blocks to fill the chip (>= 4096) without splitting below kCl
+    // clusters per part
+    const int64_t bx = (n + kBlock - 1) / kBlock;
+    int64_t parts = (4096 + bx - 1) / std::max<int64_t>(bx, 1);
+    parts = std::min<int64_t>(parts, std::max(1, (ncl + kCl - 1) / kCl));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(parts, 64));
+}
+
+size_t far_scratch_bytes(int64_t n, int ncl) {
+    const size_t parts = (size_t)far_parts(n, ncl);
+    return parts * (size_t)n * (sizeof(double) + sizeof(float)) + 256;
+}
+
 hipError_t launch_winding_far(const float* tris, const float4* info, const int2* range, const int32_t* ids,
                               int ncl, const Grid& g, const int64_t* idx, const int64_t* order, int64_t n, float beta,
-                              double* w, int64_t* undecided, unsigned long long* n_undecided, hipStream_t s) {
+                              void* scratch, double* w, int64_t* undecided, unsigned long long* n_undecided,
+                              hipStream_t s) {
     if (n <= 0) return hipSuccess;
     Pts P{idx, order, g.mn[0], g.mn[1], g.mn[2], g.dims[0], g.dims[1], g.h};
-    winding_far_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(tris, info, range, ids, ncl, P, n,
-                                                                               beta, w, undecided, n_undecided);
+    const int parts = far_parts(n, ncl);
+    int per_part = (ncl + parts - 1) / parts;
+    per_part = (per_part + kCl - 1) / kCl * kCl;
+    double* part_w = (double*)scratch;
+    float* part_err = (float*)(part_w + (size_t)parts * (size_t)n);
+    const unsigned bx = (unsigned)((n + kBlock - 1) / kBlock);
+    winding_far_kernel<<<dim3(bx, (unsigned)parts), kBlock, 0, s>>>(tris, info, range, ids, ncl, per_part, P, n, beta,
+                                                                   part_w, part_err);
+    winding_far_finalize_kernel<<<bx, kBlock, 0, s>>>(part_w, part_err, parts, order, n, w, undecided, n_undecided);
     return hipGetLastError();
 }
 

@@ -134,10 +134,15 @@ hipError_t launch_point_sort(const Grid& g, const int64_t* idx, const int64_t* p
                              uint32_t* counts, uint32_t* cursor, int64_t* start, void* scan_scratch, int64_t* order,
                              hipStream_t s);
 // Certified winding numbers of the n list positions order[0..n) (voxel ids in
-// idx); undecided positions are appended to `undecided`.
+// idx); undecided positions are appended to `undecided`.  The cluster loop is
+// split into far_parts(n, ncl) ranges run as separate blocks; scratch holds
+// their partial sums (far_scratch_bytes(n, ncl) bytes).
+int far_parts(int64_t n, int ncl);
+size_t far_scratch_bytes(int64_t n, int ncl);
 hipError_t launch_winding_far(const float* tris, const float4* info, const int2* range, const int32_t* ids,
                               int ncl, const Grid& g, const int64_t* idx, const int64_t* order, int64_t n, float beta,
-                              double* w, int64_t* undecided, unsigned long long* n_undecided, hipStream_t s);
+                              void* scratch, double* w, int64_t* undecided, unsigned long long* n_undecided,
+                              hipStream_t s);
 hipError_t launch_gather_i64(const int64_t* src, const int64_t* pos, int64_t m, int64_t* out, hipStream_t s);
 hipError_t launch_scatter_f64(const double* src, const int64_t* pos, int64_t m, double* dst, hipStream_t s);
 

@@ -100,6 +100,8 @@ enum Slot {
     S_UNDVOX,      // their voxel ids
     S_W2,          // their brute-force winding numbers
     S_ORDER,       // probe positions in spatial order
+    S_FARPART,     // far-field partial sums per cluster range
+    S_PROBES,      // ranked probes followed by the certificate fallback
     S_COUNT
 };
 
@@ -181,7 +183,7 @@ void* ws_get(Workspace* w, int slot, size_t bytes) {
 // counters: ull slots 0..15, int slots at byte offset 128 (32 ints)
 enum CounterId {
     C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4, C_SEL = 5, C_GROW = 6, C_RISKY = 7, C_NCL = 8,
-    C_UNDEC = 9
+    C_UNDEC = 9, C_MISM2 = 10
 };
 enum FlagId { F_UNKNOWN = 0, F_CHANGED = 1, F_ROUND0 = 2 };  // F_ROUND0 + r: round r changed a voxel
 constexpr int kMaxBatchedRounds = 24;
@@ -787,13 +789,14 @@ int winding_certified(Ctx& c, const int64_t* list, int64_t m, double* w) {
     void* scan = ws_get(c.w, S_CL_SCAN, scan_scratch_bytes(ncells));
     int64_t* order = (int64_t*)ws_get(c.w, S_ORDER, sizeof(int64_t) * (size_t)m);
     int64_t* und = (int64_t*)ws_get(c.w, S_UNDEC, sizeof(int64_t) * (size_t)m);
-    if (!counts || !cursor || !start || !scan || !order || !und)
+    void* part = ws_get(c.w, S_FARPART, far_scratch_bytes(m, c.ncl));
+    if (!counts || !cursor || !start || !scan || !order || !und || !part)
         return set_err(RHZ_ERR_NOMEM, "certificate scratch allocation failed");
     HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_UNDEC), 0, sizeof(unsigned long long), c.s));
     if (c.far_t) c.far_t->begin(c.s);
     HIP_TRY(launch_point_sort(c.g, list, nullptr, m, cg, counts, cursor, start, scan, order, c.s));
-    HIP_TRY(launch_winding_far(c.tris, c.cl_info, c.cl_range, c.cl_ids, c.ncl, c.g, list, order, m, beta, w, und,
-                               cnt_ptr(c.w, C_UNDEC), c.s));
+    HIP_TRY(launch_winding_far(c.tris, c.cl_info, c.cl_range, c.cl_ids, c.ncl, c.g, list, order, m, beta, part, w,
+                               und, cnt_ptr(c.w, C_UNDEC), c.s));
     if (c.far_t) c.far_t->end(c.s);
     rc = read_counters(c);
     if (rc) return rc;
@@ -826,44 +829,57 @@ int winding_certified(Ctx& c, const int64_t* list, int64_t m, double* w) {
     return RHZ_OK;
 }
 
-// Exact decisions for `n` listed voxels committed into `sign`; voxels whose
-// sign changed are appended to mism (counter C_MISM, not reset here).
-int probe_commit(Ctx& c, const int64_t* list, int64_t n, uint8_t* sign, int64_t* mism, int64_t mism_cap) {
+// Exact winding numbers of `m` listed voxels into w[m]: through the far-field
+// certificate (decisions equal to the plain kernel's) or the plain kernel.
+int probe_winding(Ctx& c, const int64_t* list, int64_t m, double* w) {
+    if (c.p->far_field_beta > 0.0f) return winding_certified(c, list, m, w);
     const int prec = c.p->winding_precision;
-    const int64_t chunk = std::min<int64_t>(n, 8ll << 20);
-    if (c.p->far_field_beta > 0.0f) {
-        for (int64_t off = 0; off < n; off += chunk) {
-            const int64_t m = std::min(chunk, n - off);
-            double* w = (double*)ws_get(c.w, S_W, sizeof(double) * (size_t)m);
-            if (!w) return set_err(RHZ_ERR_NOMEM, "winding buffer allocation failed");
-            int rc = winding_certified(c, list + off, m, w);
-            if (rc) return rc;
-            HIP_TRY(launch_commit(w, list + off, m, sign, mism, mism_cap, cnt_ptr(c.w, C_MISM), c.s));
-        }
-        return RHZ_OK;
+    void* scr = ws_get(c.w, S_WSCR, winding_scratch_bytes(m, c.nt));
+    int64_t* f64 = (int64_t*)ws_get(c.w, S_FP64, sizeof(int64_t) * (size_t)m);
+    if (!scr || !f64) return set_err(RHZ_ERR_NOMEM, "winding scratch allocation failed");
+    HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FP64), 0, sizeof(unsigned long long), c.s));
+    if (c.wind_t) c.wind_t->begin(c.s);
+    HIP_TRY(launch_winding(c.tris, c.nt, c.g, list, 0, m, prec, c.p->fp64_band, w, scr, f64, cnt_ptr(c.w, C_FP64),
+                           c.s));
+    if (c.wind_t) c.wind_t->end(c.s);
+    c.st->winding_pairs += m * c.nt;
+    if (prec != 0) {
+        int rc = read_counters(c);
+        if (rc) return rc;
+        const int64_t nf = (int64_t)hc(c, C_FP64);
+        c.st->fp64_voxels += nf;
+        HIP_TRY(launch_winding_fp64(c.tris, c.nt, c.g, list, 0, f64, nf, w, c.s));
     }
+    return RHZ_OK;
+}
+
+// Exact decisions for `n` listed voxels committed into `sign`.  Voxels whose
+// sign changed are appended to mism (counter C_MISM) when their list position
+// is below n_first, else to mism2 (counter C_MISM2); neither counter is reset
+// here.
+int probe_commit_ranges(Ctx& c, const int64_t* list, int64_t n, int64_t n_first, uint8_t* sign, int64_t* mism,
+                        int64_t mism_cap, int64_t* mism2, int64_t mism2_cap) {
+    const int64_t chunk = std::min<int64_t>(n, 8ll << 20);
     for (int64_t off = 0; off < n; off += chunk) {
         const int64_t m = std::min(chunk, n - off);
         double* w = (double*)ws_get(c.w, S_W, sizeof(double) * (size_t)m);
-        void* scr = ws_get(c.w, S_WSCR, winding_scratch_bytes(m, c.nt));
-        int64_t* f64 = (int64_t*)ws_get(c.w, S_FP64, sizeof(int64_t) * (size_t)m);
-        if (!w || !scr || !f64) return set_err(RHZ_ERR_NOMEM, "winding scratch allocation failed");
-        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FP64), 0, sizeof(unsigned long long), c.s));
-        if (c.wind_t) c.wind_t->begin(c.s);
-        HIP_TRY(launch_winding(c.tris, c.nt, c.g, list + off, 0, m, prec, c.p->fp64_band, w, scr, f64,
-                               cnt_ptr(c.w, C_FP64), c.s));
-        if (c.wind_t) c.wind_t->end(c.s);
-        c.st->winding_pairs += m * c.nt;
-        if (prec != 0) {
-            int rc = read_counters(c);
-            if (rc) return rc;
-            const int64_t nf = (int64_t)hc(c, C_FP64);
-            c.st->fp64_voxels += nf;
-            HIP_TRY(launch_winding_fp64(c.tris, c.nt, c.g, list + off, 0, f64, nf, w, c.s));
-        }
-        HIP_TRY(launch_commit(w, list + off, m, sign, mism, mism_cap, cnt_ptr(c.w, C_MISM), c.s));
+        if (!w) return set_err(RHZ_ERR_NOMEM, "winding buffer allocation failed");
+        int rc = probe_winding(c, list + off, m, w);
+        if (rc) return rc;
+        const int64_t split = std::min(std::max(n_first - off, (int64_t)0), m);
+        if (split > 0)
+            HIP_TRY(launch_commit(w, list + off, split, sign, mism, mism_cap, cnt_ptr(c.w, C_MISM), c.s));
+        if (split < m)
+            HIP_TRY(launch_commit(w + split, list + off + split, m - split, sign, mism2, mism2_cap,
+                                  cnt_ptr(c.w, C_MISM2), c.s));
     }
     return RHZ_OK;
+}
+
+// Exact decisions for `n` listed voxels committed into `sign`; voxels whose
+// sign changed are appended to mism (counter C_MISM, not reset here).
+int probe_commit(Ctx& c, const int64_t* list, int64_t n, uint8_t* sign, int64_t* mism, int64_t mism_cap) {
+    return probe_commit_ranges(c, list, n, n, sign, mism, mism_cap, nullptr, 0);
 }
 
 // Band distance (voxels) beyond which a scanline sign cannot be wrong: the
@@ -1058,6 +1074,37 @@ int run_learned(Ctx& c) {
     int64_t n_cur = n_sel;
     int gens = 0;
     st->ranked_probes = n_sel;
+    const bool merged = p->learned_certify && n_risky > 0 && p->learned_max_patch_gens == 0;
+    if (merged) {
+        // Without growth the fallback (every flagged candidate the ranking left
+        // out) does not depend on the ranked probes' outcome: both lists go
+        // through one spatially sorted winding pass, committed as two ranges
+        // so that their mismatches are counted apart.
+        int64_t* all = (int64_t*)ws_get(c.w, S_PROBES, sizeof(int64_t) * (size_t)(n_sel + n_risky));
+        if (!all) return set_err(RHZ_ERR_NOMEM, "probe list allocation failed");
+        if (n_sel > 0)
+            HIP_TRY(hipMemcpyAsync(all, sel, sizeof(int64_t) * (size_t)n_sel, hipMemcpyDeviceToDevice, c.s));
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_GROW), 0, sizeof(unsigned long long), c.s));
+        HIP_TRY(launch_unprobed_risky(cand, risk, n_cand, bits, all + n_sel, n_risky, cnt_ptr(c.w, C_GROW), c.s));
+        rc = read_counters(c);
+        if (rc) return rc;
+        const int64_t n_fb = std::min<int64_t>((int64_t)hc(c, C_GROW), n_risky);
+        const int64_t n_all = n_sel + n_fb;
+        mism = (int64_t*)ws_get(c.w, S_MISM, sizeof(int64_t) * (size_t)std::max<int64_t>(n_all, 1));
+        if (!mism) return set_err(RHZ_ERR_NOMEM, "mismatch list allocation failed");
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_MISM), 0, sizeof(unsigned long long), c.s));
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_MISM2), 0, sizeof(unsigned long long), c.s));
+        rc = probe_commit_ranges(c, all, n_all, n_sel, sign, mism, n_sel, mism + n_sel, n_fb);
+        if (rc) return rc;
+        rc = read_counters(c);
+        if (rc) return rc;
+        st->ranked_mismatches = (int64_t)hc(c, C_MISM);
+        st->fallback_probes = n_fb;
+        st->fallback_mismatches = (int64_t)hc(c, C_MISM2);
+        st->mismatches_found = st->ranked_mismatches + st->fallback_mismatches;
+        st->reprobed_voxels = n_all;
+        n_cur = 0;
+    }
     while (n_cur > 0) {
         HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_MISM), 0, sizeof(unsigned long long), c.s));
         rc = probe_commit(c, cur, n_cur, sign, mism, mism_cap);
@@ -1088,7 +1135,7 @@ int run_learned(Ctx& c) {
         ++gens;
     }
     st->patch_rounds = gens;
-    if (p->learned_certify && n_risky > 0) {
+    if (p->learned_certify && n_risky > 0 && !merged) {
         // certificate fallback: every flagged candidate not probed yet
         int64_t* fb = (int64_t*)ws_get(c.w, S_SEL, sizeof(int64_t) * (size_t)n_risky);
         if (!fb) return set_err(RHZ_ERR_NOMEM, "fallback list allocation failed");
