@@ -285,7 +285,7 @@ def main():
         # one directional sweep reads and writes every voxel once: 8 B/voxel
         per_ms = ms_sweeps / sweep_launches
         achieved = FSM_BYTES_PER_VOXEL * V / (per_ms * 1e-3) / 1e9
-        rl_fsm = {"bound": "hbm", "kernel": "sweep_lane_kernel (one directional sweep)", "achieved": achieved,
+        rl_fsm = {"bound": "hbm", "kernel": "sweep_tile_kernel (one directional sweep)", "achieved": achieved,
                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                   "avg_launch_ms": per_ms, "launches": sweep_launches,
                   "algorithmic_bytes_per_launch": FSM_BYTES_PER_VOXEL * V}
