@@ -1,5 +1,5 @@
 // C++ drop-in for the mesh-to-SDF entry points of rhizome::CudaRasterizer
-// (native/src/engine/sdf/cuda_rasterizer.hpp:15-35, :280-291), implemented
+// (native/src/engine/sdf/cuda_rasterizer.hpp:15-35, :255-261, :280-291), implemented
 // over the C-ABI of rhizome_sdf.h (link with -lrhizome_sdf).  Header-only:
 // nothing of the C++ standard library crosses the shared-library boundary.
 //
@@ -142,6 +142,29 @@ public:
         out_time_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (rc != RHZ_OK) {
             std::fprintf(stderr, "[CudaRasterizer] compute_winding_numbers_gpu failed (%d): %s\n", rc,
+                         rhz_last_error());
+            return false;
+        }
+        return true;
+    }
+
+    // cuda_mesh_ops.cu:707-771: out_distances is resized (new elements 0); an
+    // empty query list or mesh returns true with time 0 and nothing written.
+    static bool query_distance_batch_gpu(const TriangleMesh& mesh, const std::vector<vec3>& query_points,
+                                         std::vector<float>& out_distances, float& out_time_ms) {
+        const auto t0 = std::chrono::steady_clock::now();
+        out_distances.resize(query_points.size(), 0.0f);
+        if (query_points.empty() || mesh.indices.empty()) {
+            out_time_ms = 0.0f;
+            return true;
+        }
+        const int rc = rhz_query_distances(mesh.vertices.data(), (int64_t)mesh.vertex_count(), mesh.indices.data(),
+                                           (int64_t)mesh.triangle_count(),
+                                           reinterpret_cast<const float*>(query_points.data()),
+                                           (int64_t)query_points.size(), out_distances.data());
+        out_time_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (rc != RHZ_OK) {
+            std::fprintf(stderr, "[CudaRasterizer] query_distance_batch_gpu failed (%d): %s\n", rc,
                          rhz_last_error());
             return false;
         }

@@ -241,6 +241,14 @@ int rhz_frame_grid(const float* verts, int64_t n_verts, float h,
 int rhz_winding_numbers(const float* verts, int64_t n_verts, const uint32_t* idx,
                         int64_t n_tris, const float* pts, int64_t n_pts, float* out_w);
 
+/* Unsigned distance from each query point to an indexed mesh: the square root
+ * of the smallest squared point-triangle distance over every triangle
+ * (degenerate ones included).  No triangles: every distance is 0.  Replaces
+ * CudaRasterizer::query_distance_batch_gpu (cuda_rasterizer.hpp:255-261,
+ * engine/mesh/cuda_mesh_ops.cu:668-771). Host buffers. */
+int rhz_query_distances(const float* verts, int64_t n_verts, const uint32_t* idx,
+                        int64_t n_tris, const float* pts, int64_t n_pts, float* out_d);
+
 /* Red-black Gauss-Seidel eikonal on a signed grid, `iters` iterations (the
  * reference uses 8).  Replaces CudaRasterizer::fast_sweeping_eikonal_gpu
  * (cuda_rasterizer.hpp:288-291, cuda_rasterizer.cu:1536-1629).  Host grid. */

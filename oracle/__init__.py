@@ -135,6 +135,7 @@ def lib():
         L.orc_plain_exact_sdf.restype = C.c_int
         L.orc_legacy_sdf.argtypes = [_f32p, C.c_int64, C.c_float, _i32p, _i32p, _f32p]
         L.orc_winding_points.argtypes = [_f32p, _u32p, C.c_int64, _f32p, C.c_int64, _f32p]
+        L.orc_query_distances.argtypes = [_f32p, _u32p, C.c_int64, _f32p, C.c_int64, _f32p]
         L.orc_eikonal_redblack.argtypes = [_f32p, _i32p, C.c_float, C.c_int]
         L.orc_hash_01.argtypes = [C.c_uint32]
         L.orc_hash_01.restype = C.c_float
@@ -258,6 +259,15 @@ def winding_points(verts, idx, pts):
     p = _f32(pts).reshape(-1)
     out = np.zeros(len(p) // 3, dtype=np.float32)
     lib().orc_winding_points(v, i, len(i) // 3, p, len(out), out)
+    return out
+
+
+def query_distances(verts, idx, pts):
+    v = _f32(verts).reshape(-1)
+    i = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    p = _f32(pts).reshape(-1)
+    out = np.zeros(len(p) // 3, dtype=np.float32)
+    lib().orc_query_distances(v, i, len(i) // 3, p, len(out), out)
     return out
 
 

@@ -1233,6 +1233,32 @@ void orc_winding_points(const float* verts, const uint32_t* idx, int64_t nt,
     orc_parallel_for(np, 64, wpts_range, &c);
 }
 
+/* gpu_query_point_distances_kernel; cuda_mesh_ops.cu:668-703: sqrt of the
+ * smallest squared point-triangle distance over every triangle, starting
+ * from 1e18 (a NaN distance never replaces the minimum). */
+static void qdist_range(void* vctx, int64_t b, int64_t e, int tid) {
+    (void)tid;
+    wpts_ctx* c = (wpts_ctx*)vctx;
+    for (int64_t i = b; i < e; ++i) {
+        const float p[3] = {c->pts[3 * i], c->pts[3 * i + 1], c->pts[3 * i + 2]};
+        float best = 1e18f;
+        for (int64_t t = 0; t < c->nt; ++t) {
+            const float* A = c->v + 3 * (size_t)c->idx[3 * t];
+            const float* B = c->v + 3 * (size_t)c->idx[3 * t + 1];
+            const float* C = c->v + 3 * (size_t)c->idx[3 * t + 2];
+            const float d2 = orc_point_triangle_distance_sq(p, A, B, C);
+            if (d2 < best) best = d2;
+        }
+        c->w[i] = sqrtf(best);
+    }
+}
+
+void orc_query_distances(const float* verts, const uint32_t* idx, int64_t nt,
+                         const float* pts, int64_t np, float* out_d) {
+    wpts_ctx c = {verts, idx, nt, pts, out_d};
+    orc_parallel_for(np, 64, qdist_range, &c);
+}
+
 /* gpu_eikonal_update_kernel red-black iterations; cuda_rasterizer.cu:1536-1629.
  * Same-colour voxels never neighbour each other, so each half-step is
  * order-independent. */

@@ -39,6 +39,9 @@
  *   orc_winding_points               reference Test 5 fixtures (open box,
  *                                    inverted sphere, two cubes) with the
  *                                    reference hash point generator
+ *   orc_query_distances              FP64 brute-force minimum over the mesh,
+ *                                    closed forms (height over a face,
+ *                                    distance to an edge / vertex)
  *   orc_verify_indices               closed form (double arithmetic)
  *   orc_patch_box_*                  closed form
  *   orc_splat_band                   equals the brute-force min over all
@@ -197,6 +200,9 @@ void orc_legacy_sdf(const orc_tri* t, int64_t nt, float h,
 /* Standalone winding numbers over an indexed mesh (cuda_mesh_ops.cu:774-823). */
 void orc_winding_points(const float* verts, const uint32_t* idx, int64_t nt,
                         const float* pts, int64_t np, float* out_w);
+/* Standalone point-to-mesh distances over an indexed mesh (cuda_mesh_ops.cu:668-703). */
+void orc_query_distances(const float* verts, const uint32_t* idx, int64_t nt,
+                         const float* pts, int64_t np, float* out_d);
 /* Standalone red-black eikonal (cuda_rasterizer.cu:1536-1629), 8 iterations. */
 void orc_eikonal_redblack(float* grid, const int32_t dims[3], float h, int iters);
 /* Reference hash (math_utils.cuh:219-226) and Test 5 point generator

@@ -186,6 +186,7 @@ def lib():
     L.rhz_mesh_bbox.argtypes = [vp, i64, vp, vp]
     L.rhz_frame_grid.argtypes = [vp, i64, f32, vp, vp]
     L.rhz_winding_numbers.argtypes = [vp, i64, vp, i64, vp, i64, vp]
+    L.rhz_query_distances.argtypes = [vp, i64, vp, i64, vp, i64, vp]
     L.rhz_eikonal_redblack.argtypes = [vp, vp, f32, i32]
     L.rhz_stage_fill.argtypes = [vp, i64, f32, vp]
     L.rhz_stage_splat.argtypes = [vp, i64, f32, f32, vp, vp, vp, vp]
@@ -366,6 +367,17 @@ def winding_numbers(verts, idx, pts):
     out = np.empty(len(p) // 3, dtype=np.float32)
     check(lib().rhz_winding_numbers(v.ctypes.data, len(v) // 3, i.ctypes.data, len(i) // 3, p.ctypes.data,
                                     len(out), out.ctypes.data), "rhz_winding_numbers")
+    return out
+
+
+def query_distances(verts, idx, pts):
+    """Unsigned point-to-mesh distances (rhz_query_distances)."""
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
+    i = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    p = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1)
+    out = np.empty(len(p) // 3, dtype=np.float32)
+    check(lib().rhz_query_distances(v.ctypes.data, len(v) // 3, i.ctypes.data, len(i) // 3, p.ctypes.data,
+                                    len(out), out.ctypes.data), "rhz_query_distances")
     return out
 
 

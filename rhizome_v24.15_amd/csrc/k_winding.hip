@@ -225,6 +225,25 @@ __global__ __launch_bounds__(kWBlock) void legacy_kernel(const float* __restrict
     }
 }
 
+// Stages triangles tb .. tb + cnt - 1 of an indexed mesh into the LDS tile s
+// (component-major: s[c * kWTile + k], c = a.x a.y a.z b.x ... c.z).
+__device__ __forceinline__ void stage_indexed_tris(float* s, const float* __restrict__ verts,
+                                                   const uint32_t* __restrict__ idx, int64_t tb, int cnt) {
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+        const int64_t t = tb + k;
+        const uint32_t i0 = idx[3 * t], i1 = idx[3 * t + 1], i2 = idx[3 * t + 2];
+        s[0 * kWTile + k] = verts[3 * (int64_t)i0];
+        s[1 * kWTile + k] = verts[3 * (int64_t)i0 + 1];
+        s[2 * kWTile + k] = verts[3 * (int64_t)i0 + 2];
+        s[3 * kWTile + k] = verts[3 * (int64_t)i1];
+        s[4 * kWTile + k] = verts[3 * (int64_t)i1 + 1];
+        s[5 * kWTile + k] = verts[3 * (int64_t)i1 + 2];
+        s[6 * kWTile + k] = verts[3 * (int64_t)i2];
+        s[7 * kWTile + k] = verts[3 * (int64_t)i2 + 1];
+        s[8 * kWTile + k] = verts[3 * (int64_t)i2 + 2];
+    }
+}
+
 // ---- standalone winding numbers over an indexed mesh ------------------------
 // Reference gpu_compute_winding_numbers_kernel (engine/mesh/cuda_mesh_ops.cu:
 // 774-823): guard < 1e-7 skips the triangle, plain FP32 sum in index order.
@@ -245,19 +264,7 @@ __global__ __launch_bounds__(kWBlock) void winding_points_kernel(const float* __
     for (int64_t tb = 0; tb < nt; tb += kWTile) {
         const int cnt = (int)min((int64_t)kWTile, nt - tb);
         __syncthreads();
-        for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-            const int64_t t = tb + k;
-            const uint32_t i0 = idx[3 * t], i1 = idx[3 * t + 1], i2 = idx[3 * t + 2];
-            s[0 * kWTile + k] = verts[3 * (int64_t)i0];
-            s[1 * kWTile + k] = verts[3 * (int64_t)i0 + 1];
-            s[2 * kWTile + k] = verts[3 * (int64_t)i0 + 2];
-            s[3 * kWTile + k] = verts[3 * (int64_t)i1];
-            s[4 * kWTile + k] = verts[3 * (int64_t)i1 + 1];
-            s[5 * kWTile + k] = verts[3 * (int64_t)i1 + 2];
-            s[6 * kWTile + k] = verts[3 * (int64_t)i2];
-            s[7 * kWTile + k] = verts[3 * (int64_t)i2 + 1];
-            s[8 * kWTile + k] = verts[3 * (int64_t)i2 + 2];
-        }
+        stage_indexed_tris(s, verts, idx, tb, cnt);
         __syncthreads();
         if (live) {
             for (int k = 0; k < cnt; ++k) {
@@ -279,6 +286,40 @@ __global__ __launch_bounds__(kWBlock) void winding_points_kernel(const float* __
         }
     }
     if (live) out[i] = total * kInv4Pi;
+}
+
+// ---- standalone point-to-mesh distances -------------------------------------
+// Reference gpu_query_point_distances_kernel (engine/mesh/cuda_mesh_ops.cu:
+// 668-703): the square root of the smallest squared point-triangle distance
+// over every triangle of the indexed mesh (degenerate ones included; a NaN
+// distance never replaces the running minimum), starting from 1e18.
+__global__ __launch_bounds__(kWBlock) void point_distance_kernel(const float* __restrict__ verts,
+                                                                 const uint32_t* __restrict__ idx, int64_t nt,
+                                                                 const float* __restrict__ pts, int64_t np,
+                                                                 float* __restrict__ out) {
+    __shared__ float s[9 * kWTile];
+    const int64_t i = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
+    const bool live = i < np;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (live) {
+        px = pts[3 * i];
+        py = pts[3 * i + 1];
+        pz = pts[3 * i + 2];
+    }
+    float best = 1e18f;
+    for (int64_t tb = 0; tb < nt; tb += kWTile) {
+        const int cnt = (int)min((int64_t)kWTile, nt - tb);
+        __syncthreads();
+        stage_indexed_tris(s, verts, idx, tb, cnt);
+        __syncthreads();
+        if (live) {
+            for (int k = 0; k < cnt; ++k) {
+                const float d2 = point_tri_dist2(px, py, pz, tile_tri(s, k));
+                if (d2 < best) best = d2;
+            }
+        }
+    }
+    if (live) out[i] = sqrtf(best);
 }
 
 inline int split_count(int64_t n, int64_t nt) {
@@ -360,6 +401,13 @@ hipError_t launch_winding_points(const float* verts, const uint32_t* idx, int64_
                                  float* out, hipStream_t s) {
     if (np <= 0) return hipSuccess;
     winding_points_kernel<<<(unsigned)((np + kWBlock - 1) / kWBlock), kWBlock, 0, s>>>(verts, idx, nt, pts, np, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_point_distances(const float* verts, const uint32_t* idx, int64_t nt, const float* pts, int64_t np,
+                                  float* out, hipStream_t s) {
+    if (np <= 0) return hipSuccess;
+    point_distance_kernel<<<(unsigned)((np + kWBlock - 1) / kWBlock), kWBlock, 0, s>>>(verts, idx, nt, pts, np, out);
     return hipGetLastError();
 }
 

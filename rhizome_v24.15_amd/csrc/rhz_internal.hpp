@@ -115,6 +115,9 @@ size_t legacy_scratch_bytes(int64_t nt);
 // Standalone winding numbers over an indexed mesh.
 hipError_t launch_winding_points(const float* verts, const uint32_t* idx, int64_t nt,
                                  const float* pts, int64_t np, float* out, hipStream_t s);
+// Unsigned distance of each point to an indexed mesh (brute force).
+hipError_t launch_point_distances(const float* verts, const uint32_t* idx, int64_t nt, const float* pts, int64_t np,
+                                  float* out, hipStream_t s);
 
 // ---- k_learned.hip --------------------------------------------------------
 // Candidate voxels: band distance d[i] <= thr (appended to list, counter += n).

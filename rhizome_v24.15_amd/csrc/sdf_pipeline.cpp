@@ -1629,6 +1629,36 @@ int rhz_winding_numbers(const float* verts, int64_t n_verts, const uint32_t* idx
     return RHZ_OK;
 }
 
+int rhz_query_distances(const float* verts, int64_t n_verts, const uint32_t* idx, int64_t n_tris, const float* pts,
+                        int64_t n_pts, float* out_d) {
+    int rc = need_device();
+    if (rc) return rc;
+    if (n_pts < 0 || n_tris < 0 || n_verts < 0 || (n_pts > 0 && (!pts || !out_d)) ||
+        (n_tris > 0 && (!verts || !idx)))
+        return set_err(RHZ_ERR_ARG, "null buffer");
+    if (n_pts == 0) return RHZ_OK;
+    if (n_tris == 0) {
+        for (int64_t i = 0; i < n_pts; ++i) out_d[i] = 0.0f;  // cuda_mesh_ops.cu:714-718
+        return RHZ_OK;
+    }
+    for (int64_t t = 0; t < 3 * n_tris; ++t)
+        if ((int64_t)idx[t] >= n_verts) return set_err(RHZ_ERR_ARG, "index out of range");
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    float* dv = (float*)ws_get(w, S_MISC, sizeof(float) * 3 * (size_t)n_verts);
+    uint32_t* di = (uint32_t*)ws_get(w, S_MISC2, sizeof(uint32_t) * 3 * (size_t)n_tris);
+    float* dp = (float*)ws_get(w, S_TRIS, sizeof(float) * 3 * (size_t)n_pts);
+    float* dd = (float*)ws_get(w, S_OUT, sizeof(float) * (size_t)n_pts);
+    if (!dv || !di || !dp || !dd) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+    HIP_TRY(hipMemcpy(dv, verts, sizeof(float) * 3 * (size_t)n_verts, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(di, idx, sizeof(uint32_t) * 3 * (size_t)n_tris, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dp, pts, sizeof(float) * 3 * (size_t)n_pts, hipMemcpyHostToDevice));
+    HIP_TRY(launch_point_distances(dv, di, n_tris, dp, n_pts, dd, nullptr));
+    HIP_TRY(hipMemcpy(out_d, dd, sizeof(float) * (size_t)n_pts, hipMemcpyDeviceToHost));
+    return RHZ_OK;
+}
+
 int rhz_eikonal_redblack(float* grid, const int32_t dims[3], float h, int32_t iters) {
     int rc = need_device();
     if (rc) return rc;

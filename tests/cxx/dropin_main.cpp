@@ -1,6 +1,6 @@
 // Drives the C++ drop-in (include/rhizome/cuda_rasterizer.hpp) the way the
 // reference's callers do (build_from_mesh, compute_mesh_sdf,
-// compute_winding_numbers_gpu), for the parity tests.
+// compute_winding_numbers_gpu, query_distance_batch_gpu), for the parity tests.
 //
 //   dropin_main info
 //   dropin_main sdf <mesh.bin> <h> <out.bin>
@@ -57,7 +57,7 @@ int main(int argc, char** argv) {
                     (unsigned long long)g->version);
         return 0;
     }
-    if (argc == 5 && !std::strcmp(argv[1], "wind")) {
+    if (argc == 5 && (!std::strcmp(argv[1], "wind") || !std::strcmp(argv[1], "dist"))) {
         TriangleMesh m;
         if (!read_mesh(argv[2], m)) return 1;
         FILE* f = std::fopen(argv[3], "rb");
@@ -73,13 +73,16 @@ int main(int argc, char** argv) {
         if (!ok) return 1;
         std::vector<float> w;
         float ms = 0.f;
-        if (!CudaRasterizer::compute_winding_numbers_gpu(m, pts, w, ms)) return 2;
+        const bool dist = !std::strcmp(argv[1], "dist");
+        if (dist ? !CudaRasterizer::query_distance_batch_gpu(m, pts, w, ms)
+                 : !CudaRasterizer::compute_winding_numbers_gpu(m, pts, w, ms))
+            return 2;
         FILE* o = std::fopen(argv[4], "wb");
         if (!o) return 1;
         std::fwrite(w.data(), 4, w.size(), o);
         std::fclose(o);
         return 0;
     }
-    std::fprintf(stderr, "usage: dropin_main info | sdf mesh.bin h out.bin | wind mesh.bin pts.bin out.bin\n");
+    std::fprintf(stderr, "usage: dropin_main info | sdf mesh.bin h out.bin | wind|dist mesh.bin pts.bin out.bin\n");
     return 1;
 }

@@ -71,3 +71,31 @@ def test_dropin_winding_matches_binding(rhz, geom, tmp_path):
     assert np.array_equal(w.view(np.uint32), np.asarray(ref, np.float32).view(np.uint32))
     r = np.linalg.norm(p, axis=1)
     assert np.all(np.abs(w[r < 15]) > 0.99) and np.all(np.abs(w[r > 25]) < 0.01)
+
+
+def test_dropin_query_distances_matches_binding(rhz, orc, geom, tmp_path):
+    """query_distance_batch_gpu through the C++ drop-in equals the binding and
+    the oracle bit for bit."""
+    v, f = geom.perturbed_icosphere(seed=2, subdiv=3)
+    mesh = tmp_path / "mesh.bin"
+    pts = tmp_path / "pts.bin"
+    out = tmp_path / "d.bin"
+    _write_mesh(mesh, v, f)
+    p = np.random.default_rng(6).uniform(-30, 30, size=(3000, 3)).astype(np.float32)
+    with open(pts, "wb") as fh:
+        fh.write(np.array([len(p)], np.int64).tobytes())
+        fh.write(p.tobytes())
+    _run("dist", str(mesh), str(pts), str(out))
+    d = np.fromfile(out, dtype=np.float32)
+    ref = rhz.query_distances(v, f, p)
+    assert np.array_equal(d.view(np.uint32), np.asarray(ref, np.float32).view(np.uint32))
+    assert np.array_equal(d.view(np.uint32), orc.query_distances(v, f, p).view(np.uint32))
+
+
+def test_query_distances_empty_inputs(rhz, geom):
+    """No query points: nothing to do; no triangles: every distance is 0
+    (cuda_mesh_ops.cu:714-718)."""
+    v, f = geom.perturbed_icosphere(seed=2, subdiv=1)
+    assert rhz.query_distances(v, f, np.zeros((0, 3), np.float32)).shape == (0,)
+    d = rhz.query_distances(v, f[:0], np.ones((5, 3), np.float32))
+    assert np.array_equal(d, np.zeros(5, np.float32))

@@ -130,3 +130,49 @@ def test_godunov_reference_quirks(orc):
     got = L.orc_godunov(1.0, 1.0, 1e30, h)
     assert got == pytest.approx((1.0 + 0.0 + math.sqrt(2 - 1.0)) / 2, rel=1e-6)
     assert got < 1.0 + h / math.sqrt(2)
+
+
+def _cube_mesh():
+    v = np.array([[x, y, z] for z in (0, 1) for y in (0, 1) for x in (0, 1)], np.float32)
+    # 12 triangles, two per face (orientation does not matter for distances)
+    f = np.array([[0, 1, 3], [0, 3, 2], [4, 6, 7], [4, 7, 5], [0, 4, 5], [0, 5, 1],
+                  [2, 3, 7], [2, 7, 6], [0, 2, 6], [0, 6, 4], [1, 5, 7], [1, 7, 3]], np.uint32)
+    return v, f
+
+
+def test_query_distances_closed_forms(orc):
+    """Unit cube surface: face, edge and vertex regions, inside and outside,
+    and the 1e18 start for an empty mesh."""
+    v, f = _cube_mesh()
+    p = np.array([[2.0, 0.5, 0.5],      # face region, outside: 1
+                  [0.5, 0.5, 0.2],      # inside, nearest face z = 0: 0.2
+                  [2.0, 2.0, 0.5],      # edge x = y = 1: sqrt(2)
+                  [2.0, 2.0, 2.0],      # vertex (1, 1, 1): sqrt(3)
+                  [0.25, 0.5, 0.5],     # inside, nearest face x = 0: 0.25
+                  [1.0, 1.0, 1.0]],     # on a vertex: 0
+                 np.float32)
+    d = orc.query_distances(v, f, p)
+    want = [1.0, 0.2, math.sqrt(2), math.sqrt(3), 0.25, 0.0]
+    assert np.allclose(d, want, rtol=1e-6, atol=1e-7)
+    assert orc.query_distances(v, f[:0], p[:1])[0] == np.float32(1e9)  # sqrt(1e18)
+
+
+def test_query_distances_vs_fp64_min(orc, geom):
+    """Every distance equals the FP64 minimum over all triangles to FP32
+    accuracy, and degenerate triangles (a point, a segment) take part."""
+    L = orc.lib()
+    v, f = geom.perturbed_icosphere(seed=3, subdiv=2)
+    v = np.concatenate([v, np.array([[0.0, 0.0, 40.0], [5.0, 5.0, 40.0], [10.0, 10.0, 40.0]], np.float32)])
+    n = len(v)
+    f = np.concatenate([f, np.array([[n - 3, n - 3, n - 3], [n - 3, n - 2, n - 1]], np.uint32)])
+    rng = np.random.default_rng(4)
+    p = rng.uniform(-35, 45, size=(300, 3)).astype(np.float32)
+    d = orc.query_distances(v, f, p)
+    v64 = v.astype(np.float64)
+    for k in range(len(p)):
+        pk = np.ascontiguousarray(p[k], np.float64)
+        best = min(L.orc_point_triangle_distance_sq_f64(pk, *(np.ascontiguousarray(v64[i]) for i in tri))
+                   for tri in f)
+        assert d[k] == pytest.approx(math.sqrt(best), rel=2e-5, abs=1e-5)
+    # the point-triangle at (0, 0, 40): a query right above it is 1 away
+    assert orc.query_distances(v, f, np.array([[0.0, 0.0, 41.0]], np.float32))[0] == pytest.approx(1.0, abs=1e-6)
