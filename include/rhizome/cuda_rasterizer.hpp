@@ -1,5 +1,6 @@
 // C++ drop-in for the mesh-to-SDF entry points of rhizome::CudaRasterizer
-// (native/src/engine/sdf/cuda_rasterizer.hpp:15-35, :255-261, :280-291), implemented
+// (native/src/engine/sdf/cuda_rasterizer.hpp:15-35, :73-86, :255-261, :280-291,
+// :661-665), implemented
 // over the C-ABI of rhizome_sdf.h (link with -lrhizome_sdf).  Header-only:
 // nothing of the C++ standard library crosses the shared-library boundary.
 //
@@ -168,6 +169,67 @@ public:
                          rhz_last_error());
             return false;
         }
+        return true;
+    }
+
+    // cuda_marching_cubes.cu:829-962: out_mesh is cleared; a lattice with an
+    // axis below 2 cells (or an empty grid) gives an empty mesh and true.
+    // Unlike the reference, a grid smaller than dims or a device failure
+    // returns false (the reference returns true with an empty mesh).
+    static bool extract_iso_surface(const std::vector<float>& distance_grid, ivec3 grid_dims, float voxel_size,
+                                    vec3 origin, float iso_value, TriangleMesh& out_mesh, float& out_time_ms) {
+        const auto t0 = std::chrono::steady_clock::now();
+        out_mesh.vertices.clear();
+        out_mesh.indices.clear();
+        out_time_ms = 0.0f;
+        if (grid_dims.x < 2 || grid_dims.y < 2 || grid_dims.z < 2 || distance_grid.empty()) return true;
+        if (distance_grid.size() < (size_t)grid_dims.x * (size_t)grid_dims.y * (size_t)grid_dims.z) return false;
+        const int32_t dims[3] = {grid_dims.x, grid_dims.y, grid_dims.z};
+        const float org[3] = {origin.x, origin.y, origin.z};
+        int64_t nv = 0, nt = 0;
+        int rc = rhz_iso_surface(distance_grid.data(), dims, voxel_size, org, iso_value, nullptr, 0, nullptr, 0, &nv,
+                                 &nt);
+        if (rc == RHZ_OK && nv > 0 && nt > 0) {
+            out_mesh.vertices.resize((size_t)nv * 3);
+            out_mesh.indices.resize((size_t)nt * 3);
+            rc = rhz_iso_surface(distance_grid.data(), dims, voxel_size, org, iso_value, out_mesh.vertices.data(), nv,
+                                 out_mesh.indices.data(), nt, &nv, &nt);
+        }
+        out_time_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (rc != RHZ_OK) {
+            out_mesh.vertices.clear();
+            out_mesh.indices.clear();
+            std::fprintf(stderr, "[CudaRasterizer] extract_iso_surface failed (%d): %s\n", rc, rhz_last_error());
+            return false;
+        }
+        return true;
+    }
+
+    // cuda_marching_cubes.cu:952-962: origin = min_coord * voxel_size.
+    static TriangleMesh extract_iso_surface(const CudaVolumeGrid& volume, float iso_value = 0.0f) {
+        TriangleMesh mesh;
+        float ms = 0.0f;
+        const vec3 origin{(float)volume.min_coord.x * volume.voxel_size, (float)volume.min_coord.y * volume.voxel_size,
+                          (float)volume.min_coord.z * volume.voxel_size};
+        extract_iso_surface(volume.data, volume.dims, volume.voxel_size, origin, iso_value, mesh, ms);
+        return mesh;
+    }
+
+    // ui/viewport/cuda_viewport_gl.cu:702-740: voxels with value <= 0, times h^3.
+    static bool compute_sdf_interior_volume_gpu(const CudaVolumeGrid& grid, float& out_volume_mm3,
+                                                float& out_time_ms) {
+        const auto t0 = std::chrono::steady_clock::now();
+        out_volume_mm3 = 0.0f;
+        out_time_ms = 0.0f;
+        if (!grid.is_valid()) return false;
+        int64_t count = 0;
+        if (rhz_interior_count(grid.data.data(), (int64_t)grid.data.size(), &count) != RHZ_OK) {
+            std::fprintf(stderr, "[CudaRasterizer] compute_sdf_interior_volume_gpu failed: %s\n", rhz_last_error());
+            return false;
+        }
+        const float vox = grid.voxel_size * grid.voxel_size * grid.voxel_size;
+        out_volume_mm3 = (float)count * vox;
+        out_time_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return true;
     }
 

@@ -249,6 +249,32 @@ int rhz_winding_numbers(const float* verts, int64_t n_verts, const uint32_t* idx
 int rhz_query_distances(const float* verts, int64_t n_verts, const uint32_t* idx,
                         int64_t n_tris, const float* pts, int64_t n_pts, float* out_d);
 
+/* Marching cubes of an x-fastest signed field at `iso`: border voxels count as
+ * max(value, iso + h) (the reference's seal), a corner is inside when
+ * value - iso <= 0, one vertex per cut lattice edge at origin + (i + t) h,
+ * triangles from the standard table.  Writes *n_verts / *n_tris always, and
+ * the vertices (3 floats each) and triangles (3 vertex ids each) when both
+ * buffers are given and hold them; call with NULL buffers to size them.  The
+ * order is deterministic: vertices voxel by voxel (x-, y-, z-edge), triangles
+ * cube by cube.  A lattice with an axis below 2 cells has no surface.
+ * Replaces CudaRasterizer::extract_iso_surface (cuda_rasterizer.hpp:73-86,
+ * engine/mesh/cuda_marching_cubes.cu:829-962). Host buffers. */
+int rhz_iso_surface(const float* grid, const int32_t dims[3], float voxel_size, const float origin[3],
+                    float iso, float* verts, int64_t verts_cap, uint32_t* tris, int64_t tris_cap,
+                    int64_t* n_verts, int64_t* n_tris);
+/* The same on device buffers, on the caller's stream (hipStream_t). */
+int rhz_iso_surface_device(const float* d_grid, const int32_t dims[3], float voxel_size,
+                           const float origin[3], float iso, float* d_verts, int64_t verts_cap,
+                           uint32_t* d_tris, int64_t tris_cap, int64_t* n_verts, int64_t* n_tris,
+                           void* stream);
+
+/* Number of values <= 0 (NaN is not inside).  The volume of the reference's
+ * CudaRasterizer::compute_sdf_interior_volume_gpu (cuda_rasterizer.hpp:661-665,
+ * ui/viewport/cuda_viewport_gl.cu:84-110, 702-740) is count * h^3.  Host /
+ * device buffer. */
+int rhz_interior_count(const float* grid, int64_t n, int64_t* count);
+int rhz_interior_count_device(const float* d_grid, int64_t n, int64_t* count, void* stream);
+
 /* Red-black Gauss-Seidel eikonal on a signed grid, `iters` iterations (the
  * reference uses 8).  Replaces CudaRasterizer::fast_sweeping_eikonal_gpu
  * (cuda_rasterizer.hpp:288-291, cuda_rasterizer.cu:1536-1629).  Host grid. */

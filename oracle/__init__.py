@@ -136,6 +136,12 @@ def lib():
         L.orc_legacy_sdf.argtypes = [_f32p, C.c_int64, C.c_float, _i32p, _i32p, _f32p]
         L.orc_winding_points.argtypes = [_f32p, _u32p, C.c_int64, _f32p, C.c_int64, _f32p]
         L.orc_query_distances.argtypes = [_f32p, _u32p, C.c_int64, _f32p, C.c_int64, _f32p]
+        L.orc_mc_table.argtypes = [C.c_int, C.POINTER(C.c_int)]
+        L.orc_mc_table.restype = C.c_int
+        L.orc_mc_count.argtypes = [_f32p, _i32p, C.c_float, C.c_float, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.orc_mc_extract.argtypes = [_f32p, _i32p, C.c_float, _f32p, C.c_float, _f32p, _u32p]
+        L.orc_interior_count.argtypes = [_f32p, C.c_int64]
+        L.orc_interior_count.restype = C.c_int64
         L.orc_eikonal_redblack.argtypes = [_f32p, _i32p, C.c_float, C.c_int]
         L.orc_hash_01.argtypes = [C.c_uint32]
         L.orc_hash_01.restype = C.c_float
@@ -269,6 +275,32 @@ def query_distances(verts, idx, pts):
     out = np.zeros(len(p) // 3, dtype=np.float32)
     lib().orc_query_distances(v, i, len(i) // 3, p, len(out), out)
     return out
+
+
+def mc_table(c):
+    e = (C.c_int * 16)()
+    n = lib().orc_mc_table(int(c), e)
+    return [e[k] for k in range(n)]
+
+
+def mc_extract(grid, dims, h, origin, iso):
+    """Marching cubes of an x-fastest grid: (verts (n, 3) float32, tris (m, 3) uint32)."""
+    g = _f32(grid).reshape(-1)
+    d = _i32(dims)
+    if min(int(x) for x in dims) < 2 or len(g) == 0:  # cuda_marching_cubes.cu:842-845
+        return np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32)
+    nv, nt = C.c_int64(0), C.c_int64(0)
+    lib().orc_mc_count(g, d, float(h), float(iso), C.byref(nv), C.byref(nt))
+    verts = np.zeros((nv.value, 3), np.float32)
+    tris = np.zeros((nt.value, 3), np.uint32)
+    if nv.value and nt.value:
+        lib().orc_mc_extract(g, d, float(h), _f32(origin), float(iso), verts.reshape(-1), tris.reshape(-1))
+    return verts, tris
+
+
+def interior_count(grid):
+    g = _f32(grid).reshape(-1)
+    return int(lib().orc_interior_count(g, len(g)))
 
 
 def gwn_test_points(mode, seed):

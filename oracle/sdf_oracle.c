@@ -1339,3 +1339,122 @@ void orc_gwn_test_points(int mode, uint32_t seed, float* out) {
         }
     }
 }
+
+/* ---- marching cubes and interior volume ---------------------------------- */
+/* CudaRasterizer::extract_iso_surface (cuda_marching_cubes.cu:312-556,
+ * 829-962) with a deterministic output order: vertices voxel by voxel (index
+ * order), x-, y-, then z-edge; triangles cube by cube, in table order, each
+ * emitted as (e0, e2, e1).  The reference assigns both orders with atomic
+ * counters, so its output is this one up to a permutation of vertices and
+ * triangles.  Border voxels are raised to iso + h first (the seal pass, here
+ * applied on read), and a corner is inside when value - iso <= 0. */
+#include "../rhizome_v24.15_amd/csrc/mc_table.inc"
+
+static const int kMcCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                    {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+/* cube edge -> (owner corner, axis) of the lattice edge it lies on */
+static const int kMcEdgeOwner[12][2] = {{0, 0}, {1, 1}, {3, 0}, {0, 1}, {4, 0}, {5, 1},
+                                        {7, 0}, {4, 1}, {0, 2}, {1, 2}, {2, 2}, {3, 2}};
+
+int orc_mc_table(int c, int* edges) {
+    int n = kMcTriStart[c + 1] - kMcTriStart[c];
+    for (int k = 0; k < n; ++k) edges[k] = kMcTriEdges[kMcTriStart[c] + k];
+    return n;
+}
+
+static float mc_val(const float* g, const int32_t d[3], int x, int y, int z, float seal) {
+    float v = g[((size_t)z * (size_t)d[1] + (size_t)y) * (size_t)d[0] + (size_t)x];
+    if (x == 0 || x == d[0] - 1 || y == 0 || y == d[1] - 1 || z == 0 || z == d[2] - 1)
+        if (v < seal) v = seal;
+    return v;
+}
+
+static int mc_case(const float* g, const int32_t d[3], int x, int y, int z, float iso, float seal) {
+    int c = 0;
+    for (int k = 0; k < 8; ++k)
+        if (mc_val(g, d, x + kMcCorner[k][0], y + kMcCorner[k][1], z + kMcCorner[k][2], seal) - iso <= 0.0f)
+            c |= 1 << k;
+    return c;
+}
+
+void orc_mc_count(const float* g, const int32_t d[3], float h, float iso, int64_t* nv, int64_t* nt) {
+    const float seal = iso + h;
+    int64_t v = 0, t = 0;
+    for (int z = 0; z < d[2]; ++z)
+        for (int y = 0; y < d[1]; ++y)
+            for (int x = 0; x < d[0]; ++x) {
+                const int s0 = mc_val(g, d, x, y, z, seal) - iso <= 0.0f;
+                if (x + 1 < d[0] && s0 != (mc_val(g, d, x + 1, y, z, seal) - iso <= 0.0f)) ++v;
+                if (y + 1 < d[1] && s0 != (mc_val(g, d, x, y + 1, z, seal) - iso <= 0.0f)) ++v;
+                if (z + 1 < d[2] && s0 != (mc_val(g, d, x, y, z + 1, seal) - iso <= 0.0f)) ++v;
+                if (x + 1 < d[0] && y + 1 < d[1] && z + 1 < d[2]) {
+                    const int c = mc_case(g, d, x, y, z, iso, seal);
+                    t += (kMcTriStart[c + 1] - kMcTriStart[c]) / 3;
+                }
+            }
+    *nv = v;
+    *nt = t;
+}
+
+/* verts: 3 * nv floats, idx: 3 * nt entries (sizes from orc_mc_count). */
+void orc_mc_extract(const float* g, const int32_t d[3], float h, const float origin[3], float iso,
+                    float* verts, uint32_t* idx) {
+    const float seal = iso + h;
+    const size_t n = (size_t)d[0] * (size_t)d[1] * (size_t)d[2];
+    int32_t* ev = (int32_t*)malloc(3 * n * sizeof(int32_t));  /* vertex id per (voxel, axis) */
+    if (!ev) abort();
+    for (size_t i = 0; i < 3 * n; ++i) ev[i] = -1;
+    int64_t nv = 0, nt = 0;
+    for (int z = 0; z < d[2]; ++z)
+        for (int y = 0; y < d[1]; ++y)
+            for (int x = 0; x < d[0]; ++x) {
+                const size_t i = ((size_t)z * (size_t)d[1] + (size_t)y) * (size_t)d[0] + (size_t)x;
+                const float v0 = mc_val(g, d, x, y, z, seal) - iso;
+                const int s0 = v0 <= 0.0f;
+                const float p0[3] = {origin[0] + (float)x * h, origin[1] + (float)y * h, origin[2] + (float)z * h};
+                for (int a = 0; a < 3; ++a) {
+                    const int q[3] = {x + (a == 0), y + (a == 1), z + (a == 2)};
+                    if (q[a] >= d[a]) continue;
+                    const float v1 = mc_val(g, d, q[0], q[1], q[2], seal) - iso;
+                    if (s0 == (v1 <= 0.0f)) continue;
+                    const float den = v0 - v1;
+                    const float t = (fabsf(den) > 1e-7f) ? fminf(fmaxf(v0 / den, 0.001f), 0.999f) : 0.5f;
+                    float* o = verts + 3 * nv;
+                    o[0] = p0[0];
+                    o[1] = p0[1];
+                    o[2] = p0[2];
+                    o[a] = p0[a] + t * h;
+                    ev[3 * i + (size_t)a] = (int32_t)nv++;
+                }
+            }
+    for (int z = 0; z + 1 < d[2]; ++z)
+        for (int y = 0; y + 1 < d[1]; ++y)
+            for (int x = 0; x + 1 < d[0]; ++x) {
+                const int c = mc_case(g, d, x, y, z, iso, seal);
+                for (int k = kMcTriStart[c]; k < kMcTriStart[c + 1]; k += 3) {
+                    int32_t id[3];
+                    for (int j = 0; j < 3; ++j) {
+                        const int e = kMcTriEdges[k + j];
+                        const int* oc = kMcCorner[kMcEdgeOwner[e][0]];
+                        const size_t o = ((size_t)(z + oc[2]) * (size_t)d[1] + (size_t)(y + oc[1])) * (size_t)d[0] +
+                                         (size_t)(x + oc[0]);
+                        id[j] = ev[3 * o + (size_t)kMcEdgeOwner[e][1]];
+                    }
+                    if (id[0] < 0 || id[1] < 0 || id[2] < 0 || id[0] == id[1] || id[1] == id[2] || id[2] == id[0])
+                        continue;
+                    idx[3 * nt] = (uint32_t)id[0];
+                    idx[3 * nt + 1] = (uint32_t)id[2];
+                    idx[3 * nt + 2] = (uint32_t)id[1];
+                    ++nt;
+                }
+            }
+    free(ev);
+}
+
+/* gpu_count_interior_voxels_kernel (ui/viewport/cuda_viewport_gl.cu:84-110):
+ * voxels with value <= 0 (NaN is not inside). */
+int64_t orc_interior_count(const float* g, int64_t n) {
+    int64_t c = 0;
+    for (int64_t i = 0; i < n; ++i) c += g[i] <= 0.0f;
+    return c;
+}

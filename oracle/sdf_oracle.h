@@ -42,6 +42,12 @@
  *   orc_query_distances              FP64 brute-force minimum over the mesh,
  *                                    closed forms (height over a face,
  *                                    distance to an edge / vertex)
+ *   orc_mc_*                         table: cut edges, closed and consistently
+ *                                    oriented in every case; an independent
+ *                                    per-cube extraction (same triangles);
+ *                                    sphere: closed 2-manifold, genus 0,
+ *                                    enclosed volume; plane: exact positions
+ *   orc_interior_count               direct count
  *   orc_verify_indices               closed form (double arithmetic)
  *   orc_patch_box_*                  closed form
  *   orc_splat_band                   equals the brute-force min over all
@@ -203,6 +209,15 @@ void orc_winding_points(const float* verts, const uint32_t* idx, int64_t nt,
 /* Standalone point-to-mesh distances over an indexed mesh (cuda_mesh_ops.cu:668-703). */
 void orc_query_distances(const float* verts, const uint32_t* idx, int64_t nt,
                          const float* pts, int64_t np, float* out_d);
+/* Marching cubes (cuda_marching_cubes.cu:312-556, 829-962) in a deterministic
+ * order: vertices voxel by voxel, x-, y-, z-edge; triangles cube by cube in
+ * table order, emitted (e0, e2, e1).  orc_mc_count gives the sizes. */
+int orc_mc_table(int c, int* edges);
+void orc_mc_count(const float* g, const int32_t dims[3], float h, float iso, int64_t* nv, int64_t* nt);
+void orc_mc_extract(const float* g, const int32_t dims[3], float h, const float origin[3], float iso,
+                    float* verts, uint32_t* idx);
+/* Voxels with value <= 0 (ui/viewport/cuda_viewport_gl.cu:84-110). */
+int64_t orc_interior_count(const float* g, int64_t n);
 /* Standalone red-black eikonal (cuda_rasterizer.cu:1536-1629), 8 iterations. */
 void orc_eikonal_redblack(float* grid, const int32_t dims[3], float h, int iters);
 /* Reference hash (math_utils.cuh:219-226) and Test 5 point generator

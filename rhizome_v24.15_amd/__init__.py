@@ -187,6 +187,11 @@ def lib():
     L.rhz_frame_grid.argtypes = [vp, i64, f32, vp, vp]
     L.rhz_winding_numbers.argtypes = [vp, i64, vp, i64, vp, i64, vp]
     L.rhz_query_distances.argtypes = [vp, i64, vp, i64, vp, i64, vp]
+    L.rhz_iso_surface.argtypes = [vp, vp, f32, vp, f32, vp, i64, vp, i64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.rhz_iso_surface_device.argtypes = [vp, vp, f32, vp, f32, vp, i64, vp, i64, C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_int64), vp]
+    L.rhz_interior_count.argtypes = [vp, i64, C.POINTER(C.c_int64)]
+    L.rhz_interior_count_device.argtypes = [vp, i64, C.POINTER(C.c_int64), vp]
     L.rhz_eikonal_redblack.argtypes = [vp, vp, f32, i32]
     L.rhz_stage_fill.argtypes = [vp, i64, f32, vp]
     L.rhz_stage_splat.argtypes = [vp, i64, f32, f32, vp, vp, vp, vp]
@@ -379,6 +384,49 @@ def query_distances(verts, idx, pts):
     check(lib().rhz_query_distances(v.ctypes.data, len(v) // 3, i.ctypes.data, len(i) // 3, p.ctypes.data,
                                     len(out), out.ctypes.data), "rhz_query_distances")
     return out
+
+
+def iso_surface(grid, dims, h, origin, iso=0.0):
+    """Marching cubes of an x-fastest field (rhz_iso_surface): (verts (n, 3)
+    float32, tris (m, 3) uint32)."""
+    g = np.ascontiguousarray(grid, dtype=np.float32).reshape(-1)
+    d = (C.c_int32 * 3)(*[int(x) for x in dims])
+    o = (C.c_float * 3)(*[float(x) for x in origin])
+    nv, nt = C.c_int64(0), C.c_int64(0)
+    check(lib().rhz_iso_surface(g.ctypes.data, d, float(h), o, float(iso), None, 0, None, 0, C.byref(nv),
+                                C.byref(nt)), "rhz_iso_surface")
+    verts = np.zeros((nv.value, 3), np.float32)
+    tris = np.zeros((nt.value, 3), np.uint32)
+    if nv.value and nt.value:
+        check(lib().rhz_iso_surface(g.ctypes.data, d, float(h), o, float(iso), verts.ctypes.data, nv.value,
+                                    tris.ctypes.data, nt.value, C.byref(nv), C.byref(nt)), "rhz_iso_surface")
+    return verts, tris
+
+
+def iso_surface_device(d_grid, dims, h, origin, iso, d_verts, verts_cap, d_tris, tris_cap, stream=None):
+    """Device variant: returns (n_verts, n_tris); writes the buffers when they hold the surface."""
+    d = (C.c_int32 * 3)(*[int(x) for x in dims])
+    o = (C.c_float * 3)(*[float(x) for x in origin])
+    nv, nt = C.c_int64(0), C.c_int64(0)
+    pv = _ptr(d_verts) if d_verts is not None else None
+    pt = _ptr(d_tris) if d_tris is not None else None
+    check(lib().rhz_iso_surface_device(_ptr(d_grid), d, float(h), o, float(iso), pv, int(verts_cap), pt,
+                                       int(tris_cap), C.byref(nv), C.byref(nt), stream),
+          "rhz_iso_surface_device")
+    return nv.value, nt.value
+
+
+def interior_count(grid):
+    g = np.ascontiguousarray(grid, dtype=np.float32).reshape(-1)
+    c = C.c_int64(0)
+    check(lib().rhz_interior_count(g.ctypes.data, len(g), C.byref(c)), "rhz_interior_count")
+    return c.value
+
+
+def interior_count_device(d_grid, n, stream=None):
+    c = C.c_int64(0)
+    check(lib().rhz_interior_count_device(_ptr(d_grid), int(n), C.byref(c), stream), "rhz_interior_count_device")
+    return c.value
 
 
 def eikonal_redblack(grid, h, iters=8):

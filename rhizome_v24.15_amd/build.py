@@ -28,6 +28,7 @@ SOURCES = [
     "k_learned.hip",
     "k_mlp.hip",
     "k_cluster.hip",
+    "k_mc.hip",
     "sdf_pipeline.cpp",
 ]
 

@@ -102,6 +102,13 @@ enum Slot {
     S_ORDER,       // probe positions in spatial order
     S_FARPART,     // far-field partial sums per cluster range
     S_PROBES,      // ranked probes followed by the certificate fallback
+    S_MC_INFO,     // marching cubes: cut mask + triangle count per voxel
+    S_MC_BLK,      // per-block vertex and triangle totals
+    S_MC_START,    // their prefix sums
+    S_MC_SCAN,
+    S_MC_VOFF,     // first vertex id per voxel
+    S_MC_VERTS,    // vertices / triangles staged by the host API
+    S_MC_TRIS,
     S_COUNT
 };
 
@@ -183,7 +190,7 @@ void* ws_get(Workspace* w, int slot, size_t bytes) {
 // counters: ull slots 0..15, int slots at byte offset 128 (32 ints)
 enum CounterId {
     C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4, C_SEL = 5, C_GROW = 6, C_RISKY = 7, C_NCL = 8,
-    C_UNDEC = 9, C_MISM2 = 10
+    C_UNDEC = 9, C_MISM2 = 10, C_INSIDE = 11
 };
 enum FlagId { F_UNKNOWN = 0, F_CHANGED = 1, F_ROUND0 = 2 };  // F_ROUND0 + r: round r changed a voxel
 constexpr int kMaxBatchedRounds = 24;
@@ -1657,6 +1664,115 @@ int rhz_query_distances(const float* verts, int64_t n_verts, const uint32_t* idx
     HIP_TRY(launch_point_distances(dv, di, n_tris, dp, n_pts, dd, nullptr));
     HIP_TRY(hipMemcpy(out_d, dd, sizeof(float) * (size_t)n_pts, hipMemcpyDeviceToHost));
     return RHZ_OK;
+}
+
+// Marching cubes of a device field (extract_iso_surface, cuda_marching_cubes.cu:
+// 829-962): the counts always, the vertices and triangles when the buffers
+// hold them and both counts are positive.
+int iso_surface_run(Workspace* w, const float* d_grid, const int32_t dims[3], float h, const float origin[3], float iso,
+                    float* d_verts, int64_t verts_cap, uint32_t* d_tris, int64_t tris_cap, int64_t* n_verts,
+                    int64_t* n_tris, hipStream_t s) {
+    *n_verts = *n_tris = 0;
+    if (dims[0] < 2 || dims[1] < 2 || dims[2] < 2) return RHZ_OK;  // :842-845
+    const int64_t V = (int64_t)dims[0] * dims[1] * dims[2];
+    const int64_t nb = mc_blocks(dims);
+    if (nb > (int64_t)INT32_MAX) return set_err(RHZ_ERR_ARG, "lattice too large");
+    uint8_t* info = (uint8_t*)ws_get(w, S_MC_INFO, (size_t)V);
+    uint32_t* blk = (uint32_t*)ws_get(w, S_MC_BLK, sizeof(uint32_t) * 2 * (size_t)nb);
+    int64_t* start = (int64_t*)ws_get(w, S_MC_START, sizeof(int64_t) * 2 * (size_t)(nb + 1));
+    void* scan = ws_get(w, S_MC_SCAN, scan_scratch_bytes(nb));
+    if (!info || !blk || !start || !scan) return set_err(RHZ_ERR_NOMEM, "marching-cubes scratch allocation failed");
+    HIP_TRY(launch_mc_classify(d_grid, dims, h, iso, info, blk, blk + nb, s));
+    HIP_TRY(launch_scan_u32(blk, nb, start, scan, s));
+    HIP_TRY(launch_scan_u32(blk + nb, nb, start + nb + 1, scan, s));
+    int64_t tot[2];
+    HIP_TRY(hipMemcpyAsync(&tot[0], start + nb, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&tot[1], start + 2 * nb + 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *n_verts = tot[0];
+    *n_tris = tot[1];
+    if (tot[0] > (int64_t)UINT32_MAX) return set_err(RHZ_ERR_ARG, "more than 2^32 marching-cubes vertices");
+    if (tot[0] == 0 || tot[1] == 0 || !d_verts || !d_tris || verts_cap < tot[0] || tris_cap < tot[1]) return RHZ_OK;
+    uint32_t* voff = (uint32_t*)ws_get(w, S_MC_VOFF, sizeof(uint32_t) * (size_t)V);
+    if (!voff) return set_err(RHZ_ERR_NOMEM, "marching-cubes scratch allocation failed");
+    HIP_TRY(launch_mc_emit(d_grid, dims, h, origin, iso, info, start, start + nb + 1, voff, d_verts, d_tris, s));
+    return RHZ_OK;
+}
+
+int rhz_iso_surface_device(const float* d_grid, const int32_t dims[3], float h, const float origin[3], float iso,
+                           float* d_verts, int64_t verts_cap, uint32_t* d_tris, int64_t tris_cap, int64_t* n_verts,
+                           int64_t* n_tris, void* stream) {
+    int rc = need_device();
+    if (rc) return rc;
+    if (!d_grid || !dims || !origin || !n_verts || !n_tris) return set_err(RHZ_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    return iso_surface_run(w, d_grid, dims, h, origin, iso, d_verts, verts_cap, d_tris, tris_cap, n_verts, n_tris,
+                           (hipStream_t)stream);
+}
+
+int rhz_iso_surface(const float* grid, const int32_t dims[3], float h, const float origin[3], float iso, float* verts,
+                    int64_t verts_cap, uint32_t* tris, int64_t tris_cap, int64_t* n_verts, int64_t* n_tris) {
+    int rc = need_device();
+    if (rc) return rc;
+    if (!grid || !dims || !origin || !n_verts || !n_tris) return set_err(RHZ_ERR_ARG, "null argument");
+    *n_verts = *n_tris = 0;
+    if (dims[0] < 2 || dims[1] < 2 || dims[2] < 2) return RHZ_OK;
+    const int64_t V = (int64_t)dims[0] * dims[1] * dims[2];
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    float* dg = (float*)ws_get(w, S_OUT, sizeof(float) * (size_t)V);
+    if (!dg) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+    HIP_TRY(hipMemcpy(dg, grid, sizeof(float) * (size_t)V, hipMemcpyHostToDevice));
+    rc = iso_surface_run(w, dg, dims, h, origin, iso, nullptr, 0, nullptr, 0, n_verts, n_tris, nullptr);
+    if (rc) return rc;
+    const int64_t nv = *n_verts, nt = *n_tris;
+    if (nv == 0 || nt == 0 || !verts || !tris || verts_cap < nv || tris_cap < nt) return RHZ_OK;
+    float* dv = (float*)ws_get(w, S_MC_VERTS, sizeof(float) * 3 * (size_t)nv);
+    uint32_t* dt = (uint32_t*)ws_get(w, S_MC_TRIS, sizeof(uint32_t) * 3 * (size_t)nt);
+    if (!dv || !dt) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+    rc = iso_surface_run(w, dg, dims, h, origin, iso, dv, nv, dt, nt, n_verts, n_tris, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(verts, dv, sizeof(float) * 3 * (size_t)nv, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(tris, dt, sizeof(uint32_t) * 3 * (size_t)nt, hipMemcpyDeviceToHost));
+    return RHZ_OK;
+}
+
+int interior_count_run(Workspace* w, const float* d_grid, int64_t n, int64_t* count, hipStream_t s) {
+    HIP_TRY(hipMemsetAsync(cnt_ptr(w, C_INSIDE), 0, sizeof(unsigned long long), s));
+    HIP_TRY(launch_interior_count(d_grid, n, cnt_ptr(w, C_INSIDE), s));
+    unsigned long long c = 0;
+    HIP_TRY(hipMemcpyAsync(&c, cnt_ptr(w, C_INSIDE), sizeof(c), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *count = (int64_t)c;
+    return RHZ_OK;
+}
+
+int rhz_interior_count_device(const float* d_grid, int64_t n, int64_t* count, void* stream) {
+    int rc = need_device();
+    if (rc) return rc;
+    if (!count || n < 0 || (n > 0 && !d_grid)) return set_err(RHZ_ERR_ARG, "invalid argument");
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    return interior_count_run(w, d_grid, n, count, (hipStream_t)stream);
+}
+
+int rhz_interior_count(const float* grid, int64_t n, int64_t* count) {
+    int rc = need_device();
+    if (rc) return rc;
+    if (!count || n < 0 || (n > 0 && !grid)) return set_err(RHZ_ERR_ARG, "invalid argument");
+    *count = 0;
+    if (n == 0) return RHZ_OK;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    float* dg = (float*)ws_get(w, S_OUT, sizeof(float) * (size_t)n);
+    if (!dg) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+    HIP_TRY(hipMemcpy(dg, grid, sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
+    return interior_count_run(w, dg, n, count, nullptr);
 }
 
 int rhz_eikonal_redblack(float* grid, const int32_t dims[3], float h, int32_t iters) {

@@ -1,6 +1,7 @@
 // Drives the C++ drop-in (include/rhizome/cuda_rasterizer.hpp) the way the
 // reference's callers do (build_from_mesh, compute_mesh_sdf,
-// compute_winding_numbers_gpu, query_distance_batch_gpu), for the parity tests.
+// compute_winding_numbers_gpu, query_distance_batch_gpu, extract_iso_surface,
+// compute_sdf_interior_volume_gpu), for the parity tests.
 //
 //   dropin_main info
 //   dropin_main sdf <mesh.bin> <h> <out.bin>
@@ -83,6 +84,43 @@ int main(int argc, char** argv) {
         std::fclose(o);
         return 0;
     }
-    std::fprintf(stderr, "usage: dropin_main info | sdf mesh.bin h out.bin | wind|dist mesh.bin pts.bin out.bin\n");
+    if (argc == 4 && (!std::strcmp(argv[1], "mc") || !std::strcmp(argv[1], "vol"))) {
+        // grid file: int32 dims[3], float origin[3], float h, float iso, dims product floats
+        FILE* f = std::fopen(argv[2], "rb");
+        if (!f) return 1;
+        int32_t d[3];
+        float org[3], h = 0.f, iso = 0.f;
+        bool ok = std::fread(d, 4, 3, f) == 3 && std::fread(org, 4, 3, f) == 3 && std::fread(&h, 4, 1, f) == 1 &&
+                  std::fread(&iso, 4, 1, f) == 1;
+        CudaVolumeGrid g;
+        if (ok) {
+            g.data.resize((size_t)d[0] * (size_t)d[1] * (size_t)d[2]);
+            ok = std::fread(g.data.data(), 4, g.data.size(), f) == g.data.size();
+        }
+        std::fclose(f);
+        if (!ok) return 1;
+        g.dims = {d[0], d[1], d[2]};
+        g.voxel_size = h;
+        FILE* o = std::fopen(argv[3], "wb");
+        if (!o) return 1;
+        float ms = 0.f;
+        if (!std::strcmp(argv[1], "vol")) {
+            float vol = 0.f;
+            if (!CudaRasterizer::compute_sdf_interior_volume_gpu(g, vol, ms)) return 2;
+            std::fwrite(&vol, 4, 1, o);
+        } else {
+            TriangleMesh m;
+            if (!CudaRasterizer::extract_iso_surface(g.data, g.dims, h, vec3{org[0], org[1], org[2]}, iso, m, ms))
+                return 2;
+            const int64_t nv = (int64_t)m.vertex_count(), nt = (int64_t)m.triangle_count();
+            std::fwrite(&nv, 8, 1, o);
+            std::fwrite(&nt, 8, 1, o);
+            std::fwrite(m.vertices.data(), 4, m.vertices.size(), o);
+            std::fwrite(m.indices.data(), 4, m.indices.size(), o);
+        }
+        std::fclose(o);
+        return 0;
+    }
+    std::fprintf(stderr, "usage: dropin_main info | sdf mesh.bin h out.bin | wind|dist mesh.bin pts.bin out.bin | mc|vol grid.bin out.bin\n");
     return 1;
 }
