@@ -157,6 +157,8 @@ typedef struct rhz_sdf_stats {
     int64_t far_undecided;        /* exact probes it handed to the brute-force kernel */
     int64_t n_clusters;           /* triangle clusters of the certificate */
     double ms_far_field;          /* device time of the certificate (incl. the point sort) */
+    int64_t n_parent_clusters;    /* their 2x2x2 parents (the certificate's first level) */
+    int64_t far_second_pass;      /* probes the parent level left open, re-examined per cluster */
 } rhz_sdf_stats;
 
 void rhz_sdf_params_default(rhz_sdf_params* p);

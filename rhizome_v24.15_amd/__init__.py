@@ -133,6 +133,8 @@ class Stats(C.Structure):
         ("far_undecided", C.c_int64),
         ("n_clusters", C.c_int64),
         ("ms_far_field", C.c_double),
+        ("n_parent_clusters", C.c_int64),
+        ("far_second_pass", C.c_int64),
     ]
 
     def as_dict(self):

@@ -135,30 +135,47 @@ hipError_t launch_point_distances(const float* verts, const uint32_t* idx, int64
 // ---- k_learned.hip --------------------------------------------------------
 // Candidate voxels: band distance d[i] <= thr (appended to list, counter += n).
 // k_cluster: certified far-field winding decisions
+// Cells of side 1 / inv_s from (ox, oy, oz); nx x ny x nz cells, indexed
+// parent-major: the 2 x 2 x 2 cells of parent (X, Y, Z) are the eight
+// consecutive ids 8 ((Z NY + Y) NX + X) + (x & 1) + 2 (y & 1) + 4 (z & 1),
+// NX = ceil(nx / 2) and so on; cells() = 8 NX NY NZ.
 struct ClusterGrid {
     float ox, oy, oz, inv_s;
     int nx, ny, nz;
+    __host__ __device__ int px() const { return (nx + 1) / 2; }
+    __host__ __device__ int py() const { return (ny + 1) / 2; }
+    __host__ __device__ int pz() const { return (nz + 1) / 2; }
+    __host__ __device__ int64_t parents() const { return (int64_t)px() * py() * pz(); }
+    __host__ __device__ int64_t cells() const { return 8 * parents(); }
 };
 hipError_t launch_tri_box_area(const float* tris, int64_t nt, int* box, double* area_part, int nblocks,
                                hipStream_t s);
+// Clusters of the non-empty cells in id order (info: 5 float4 each; range:
+// first triangle id slot and count) and of the non-empty parents (pinfo,
+// prange: first child cluster and count); slot: cells() + 1 int64, pslot:
+// parents() + 1 int64.  The cluster counts are slot[cells()] and
+// pslot[parents()].
 hipError_t launch_cluster_build(const float* tris, int64_t nt, const ClusterGrid& g, uint32_t* counts,
                                 uint32_t* cursor, int64_t* start, void* scan_scratch, int32_t* ids, float4* info,
-                                int2* range, unsigned long long* n_clusters, hipStream_t s);
+                                int2* range, int64_t* slot, uint32_t* pflags, int64_t* pslot, float4* pinfo,
+                                int2* prange, hipStream_t s);
 // Spatial order of list positions (pos: the m positions to sort, or NULL for
 // 0..m-1) by cluster-grid cell, into order[m].
 hipError_t launch_point_sort(const Grid& g, const int64_t* idx, const int64_t* pos, int64_t m, const ClusterGrid& cg,
                              uint32_t* counts, uint32_t* cursor, int64_t* start, void* scan_scratch, int64_t* order,
                              hipStream_t s);
 // Certified winding numbers of the n list positions order[0..n) (voxel ids in
-// idx); undecided positions are appended to `undecided`.  The cluster loop is
-// split into far_parts(n, ncl) ranges run as separate blocks; scratch holds
-// their partial sums (far_scratch_bytes(n, ncl) bytes).
+// idx); undecided positions are appended to `undecided`.  use_parents = false
+// evaluates every cluster at the child level (tighter bounds).  The loop over
+// the npar parent clusters is split into far_parts(n, npar) ranges run as
+// separate blocks; scratch holds their partial sums (far_scratch_bytes(n,
+// npar) bytes).
 int far_parts(int64_t n, int ncl);
 size_t far_scratch_bytes(int64_t n, int ncl);
 hipError_t launch_winding_far(const float* tris, const float4* info, const int2* range, const int32_t* ids,
-                              int ncl, const Grid& g, const int64_t* idx, const int64_t* order, int64_t n, float beta,
-                              void* scratch, double* w, int64_t* undecided, unsigned long long* n_undecided,
-                              hipStream_t s);
+                              const float4* pinfo, const int2* prange, int npar, bool use_parents, const Grid& g,
+                              const int64_t* idx, const int64_t* order, int64_t n, float beta, void* scratch,
+                              double* w, int64_t* undecided, unsigned long long* n_undecided, hipStream_t s);
 hipError_t launch_gather_i64(const int64_t* src, const int64_t* pos, int64_t m, int64_t* out, hipStream_t s);
 hipError_t launch_scatter_f64(const double* src, const int64_t* pos, int64_t m, double* dst, hipStream_t s);
 

@@ -96,7 +96,13 @@ enum Slot {
     S_CL_IDS,      // triangle ids grouped by cell
     S_CL_INFO,     // per cluster: centre, radius, area vector, area
     S_CL_RANGE,    // per cluster: first id, count
+    S_CL_SLOT,     // cluster slot of every cell (prefix sums of non-empty flags)
+    S_CL_PFLAG,    // non-empty parents
+    S_CL_PSLOT,    // parent slots
+    S_CL_PINFO,    // per parent cluster: centre, radius, area vector, area, moment
+    S_CL_PRANGE,   // per parent cluster: first child cluster, count
     S_UNDEC,       // positions the far-field certificate left undecided
+    S_UNDEC2,      // ... and still undecided at the child level
     S_UNDVOX,      // their voxel ids
     S_W2,          // their brute-force winding numbers
     S_ORDER,       // probe positions in spatial order
@@ -190,7 +196,7 @@ void* ws_get(Workspace* w, int slot, size_t bytes) {
 // counters: ull slots 0..15, int slots at byte offset 128 (32 ints)
 enum CounterId {
     C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4, C_SEL = 5, C_GROW = 6, C_RISKY = 7, C_NCL = 8,
-    C_UNDEC = 9, C_MISM2 = 10, C_INSIDE = 11
+    C_UNDEC = 9, C_MISM2 = 10, C_INSIDE = 11, C_UNDEC2 = 12
 };
 enum FlagId { F_UNKNOWN = 0, F_CHANGED = 1, F_ROUND0 = 2 };  // F_ROUND0 + r: round r changed a voxel
 constexpr int kMaxBatchedRounds = 24;
@@ -271,6 +277,9 @@ struct Ctx {
     ClusterGrid cl_grid{};
     const float4* cl_info = nullptr;
     const int2* cl_range = nullptr;
+    const float4* cl_pinfo = nullptr;
+    const int2* cl_prange = nullptr;
+    int npar = 0;
     const int32_t* cl_ids = nullptr;
 };
 
@@ -755,7 +764,7 @@ int build_clusters(Ctx& c) {
     g.nx = n3[0];
     g.ny = n3[1];
     g.nz = n3[2];
-    const int64_t ncells = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t ncells = g.cells(), npar = g.parents();
     uint32_t* counts = (uint32_t*)ws_get(c.w, S_CL_COUNTS, sizeof(uint32_t) * (size_t)ncells);
     uint32_t* cursor = (uint32_t*)ws_get(c.w, S_CL_CURSOR, sizeof(uint32_t) * (size_t)ncells);
     int64_t* start = (int64_t*)ws_get(c.w, S_CL_START, sizeof(int64_t) * (size_t)(ncells + 1));
@@ -763,19 +772,30 @@ int build_clusters(Ctx& c) {
     int32_t* ids = (int32_t*)ws_get(c.w, S_CL_IDS, sizeof(int32_t) * (size_t)c.nt);
     float4* info = (float4*)ws_get(c.w, S_CL_INFO, sizeof(float4) * 5 * (size_t)ncells);
     int2* range = (int2*)ws_get(c.w, S_CL_RANGE, sizeof(int2) * (size_t)ncells);
-    if (!counts || !cursor || !start || !scan || !ids || !info || !range)
+    int64_t* slot = (int64_t*)ws_get(c.w, S_CL_SLOT, sizeof(int64_t) * (size_t)(ncells + 1));
+    uint32_t* pflag = (uint32_t*)ws_get(c.w, S_CL_PFLAG, sizeof(uint32_t) * (size_t)npar);
+    int64_t* pslot = (int64_t*)ws_get(c.w, S_CL_PSLOT, sizeof(int64_t) * (size_t)(npar + 1));
+    float4* pinfo = (float4*)ws_get(c.w, S_CL_PINFO, sizeof(float4) * 5 * (size_t)npar);
+    int2* prange = (int2*)ws_get(c.w, S_CL_PRANGE, sizeof(int2) * (size_t)npar);
+    if (!counts || !cursor || !start || !scan || !ids || !info || !range || !slot || !pflag || !pslot || !pinfo ||
+        !prange)
         return set_err(RHZ_ERR_NOMEM, "cluster allocation failed");
     HIP_TRY(hipMemsetAsync(counts, 0, sizeof(uint32_t) * (size_t)ncells, c.s));
     HIP_TRY(hipMemsetAsync(cursor, 0, sizeof(uint32_t) * (size_t)ncells, c.s));
-    HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_NCL), 0, sizeof(unsigned long long), c.s));
-    HIP_TRY(launch_cluster_build(c.tris, c.nt, g, counts, cursor, start, scan, ids, info, range, cnt_ptr(c.w, C_NCL),
-                                 c.s));
-    int rc = read_counters(c);
-    if (rc) return rc;
-    c.ncl = (int)hc(c, C_NCL);
+    HIP_TRY(launch_cluster_build(c.tris, c.nt, g, counts, cursor, start, scan, ids, info, range, slot, pflag, pslot,
+                                 pinfo, prange, c.s));
+    int64_t nums[2];
+    HIP_TRY(hipMemcpyAsync(&nums[0], slot + ncells, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+    HIP_TRY(hipMemcpyAsync(&nums[1], pslot + npar, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+    HIP_TRY(hipStreamSynchronize(c.s));
+    c.ncl = (int)nums[0];
+    c.npar = (int)nums[1];
+    c.st->n_parent_clusters = c.npar;
     c.cl_grid = g;
     c.cl_info = info;
     c.cl_range = range;
+    c.cl_pinfo = pinfo;
+    c.cl_prange = prange;
     c.cl_ids = ids;
     c.st->n_clusters = c.ncl;
     return RHZ_OK;
@@ -789,26 +809,40 @@ int winding_certified(Ctx& c, const int64_t* list, int64_t m, double* w) {
     if (rc) return rc;
     const float beta = std::max(c.p->far_field_beta, 2.0f);  // the bound needs d > r
     const ClusterGrid& cg = c.cl_grid;
-    const int64_t ncells = (int64_t)cg.nx * cg.ny * cg.nz;
+    const int64_t ncells = cg.cells();
     uint32_t* counts = (uint32_t*)ws_get(c.w, S_CL_COUNTS, sizeof(uint32_t) * (size_t)ncells);
     uint32_t* cursor = (uint32_t*)ws_get(c.w, S_CL_CURSOR, sizeof(uint32_t) * (size_t)ncells);
     int64_t* start = (int64_t*)ws_get(c.w, S_CL_START, sizeof(int64_t) * (size_t)(ncells + 1));
     void* scan = ws_get(c.w, S_CL_SCAN, scan_scratch_bytes(ncells));
     int64_t* order = (int64_t*)ws_get(c.w, S_ORDER, sizeof(int64_t) * (size_t)m);
     int64_t* und = (int64_t*)ws_get(c.w, S_UNDEC, sizeof(int64_t) * (size_t)m);
-    void* part = ws_get(c.w, S_FARPART, far_scratch_bytes(m, c.ncl));
+    void* part = ws_get(c.w, S_FARPART, far_scratch_bytes(m, c.npar));
     if (!counts || !cursor || !start || !scan || !order || !und || !part)
         return set_err(RHZ_ERR_NOMEM, "certificate scratch allocation failed");
     HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_UNDEC), 0, sizeof(unsigned long long), c.s));
     if (c.far_t) c.far_t->begin(c.s);
     HIP_TRY(launch_point_sort(c.g, list, nullptr, m, cg, counts, cursor, start, scan, order, c.s));
-    HIP_TRY(launch_winding_far(c.tris, c.cl_info, c.cl_range, c.cl_ids, c.ncl, c.g, list, order, m, beta, part, w,
-                               und, cnt_ptr(c.w, C_UNDEC), c.s));
-    if (c.far_t) c.far_t->end(c.s);
+    HIP_TRY(launch_winding_far(c.tris, c.cl_info, c.cl_range, c.cl_ids, c.cl_pinfo, c.cl_prange, c.npar, true, c.g,
+                               list, order, m, beta, part, w, und, cnt_ptr(c.w, C_UNDEC), c.s));
     rc = read_counters(c);
     if (rc) return rc;
-    const int64_t nu = (int64_t)hc(c, C_UNDEC);
+    int64_t nu = (int64_t)hc(c, C_UNDEC);
     const int64_t* left = und;
+    if (nu > 0) {
+        // the probes the parents' looser bounds left open, at the child level
+        int64_t* und2 = (int64_t*)ws_get(c.w, S_UNDEC2, sizeof(int64_t) * (size_t)nu);
+        void* part2 = ws_get(c.w, S_FARPART, far_scratch_bytes(nu, c.npar));
+        if (!und2 || !part2) return set_err(RHZ_ERR_NOMEM, "certificate scratch allocation failed");
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_UNDEC2), 0, sizeof(unsigned long long), c.s));
+        HIP_TRY(launch_winding_far(c.tris, c.cl_info, c.cl_range, c.cl_ids, c.cl_pinfo, c.cl_prange, c.npar, false,
+                                   c.g, list, und, nu, beta, part2, w, und2, cnt_ptr(c.w, C_UNDEC2), c.s));
+        rc = read_counters(c);
+        if (rc) return rc;
+        c.st->far_second_pass += nu;
+        nu = (int64_t)hc(c, C_UNDEC2);
+        left = und2;
+    }
+    if (c.far_t) c.far_t->end(c.s);
     c.st->far_decided += m - nu;
     c.st->far_undecided += nu;
     if (nu == 0) return RHZ_OK;

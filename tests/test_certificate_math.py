@@ -93,3 +93,56 @@ def test_second_term_improves_and_a_planted_sign_error_does_not():
         e_bad.append(abs(exact - (one - q)))
     assert np.median(e_two) < 0.3 * np.median(e_one)
     assert np.median(e_bad) > np.median(e_one)
+
+
+def _props(tris, c):
+    N = np.zeros(3)
+    M = np.zeros((3, 3))
+    A = 0.0
+    for a, b, cc in tris:
+        s = 0.5 * np.cross(b - a, cc - a)
+        N += s
+        A += np.linalg.norm(s)
+        M += np.outer((a + b + cc) / 3 - c, s)
+    return N, M, A
+
+
+def test_parent_cluster_shift_and_bound():
+    """A parent made of child clusters (the k_cluster.hip parent build):
+    moving each child's moment to the parent centre, M' = sum M_k +
+    (c_k - C) N_k^T, equals the moment computed from the triangles about C,
+    and the radius max |c_k - C| + r_k keeps the two-term bound valid.  A
+    planted transposition of the shift term breaks the identity."""
+    rng = np.random.default_rng(3)
+    worst = 0.0
+    for _ in range(100):
+        children = []
+        for _ in range(int(rng.integers(2, 9))):
+            tris, c, r, N, M = _cluster(rng, n=int(rng.integers(1, 6)))
+            off = rng.normal(size=3) * 2
+            tris = [(a + off, b + off, cc + off) for a, b, cc in tris]
+            children.append((tris, c + off, r, N, M))
+        lo = np.min([c - r for _, c, r, _, _ in children], axis=0)
+        hi = np.max([c + r for _, c, r, _, _ in children], axis=0)
+        C = 0.5 * (lo + hi)
+        R = max(np.linalg.norm(c - C) + r for _, c, r, _, _ in children)
+        Np = sum(N for _, _, _, N, _ in children)
+        Mp = sum(M + np.outer(c - C, N) for _, c, _, N, M in children)
+        bad = sum(M + np.outer(N, c - C) for _, c, _, N, M in children)
+        all_tris = [t for ch in children for t in ch[0]]
+        N2, M2, A = _props(all_tris, C)
+        assert np.allclose(Np, N2, atol=1e-9) and np.allclose(Mp, M2, atol=1e-9)
+        assert not np.allclose(bad, M2, atol=1e-6)
+        V = np.array([v for t in all_tris for v in t])
+        assert np.sqrt(((V - C) ** 2).sum(1).max()) <= R + 1e-12
+        for beta in (2.0, 4.0):
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            d = R * beta * (1 + rng.random())
+            p = C + u * d
+            x = C - p
+            xh = x / d
+            exact = sum(_omega(p, a, b, cc) for a, b, cc in all_tris)
+            two = (Np @ x + np.trace(Mp) - 3 * xh @ Mp @ xh) / d ** 3
+            worst = max(worst, abs(exact - two) / (3 * R * R * A / (d - R) ** 4))
+    assert worst <= 1.0

@@ -309,6 +309,7 @@ def test_far_field_certificate_keeps_every_decision(rhz, geom, orc, request, cas
     for fo in fields[1:]:
         assert np.array_equal(fo.view(np.uint32), fields[0].view(np.uint32))
     st6 = stats[2]
-    print(case, "clusters", st6.n_clusters, "decided", st6.far_decided, "undecided", st6.far_undecided)
+    print(case, "clusters", st6.n_clusters, "parents", st6.n_parent_clusters, "decided", st6.far_decided,
+          "second pass", st6.far_second_pass, "undecided", st6.far_undecided)
     assert stats[0].far_decided == 0 and st6.far_decided + st6.far_undecided == st6.reprobed_voxels
     assert st6.far_decided >= 0.9 * st6.reprobed_voxels
