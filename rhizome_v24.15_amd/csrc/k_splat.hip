@@ -1,7 +1,8 @@
 // k_splat.hip -- [A] exact +-band distance splat.
 //
 // One 256-thread block per triangle walks the triangle's band-dilated box
-// (x fastest, so neighbouring lanes hit neighbouring voxels) and folds the
+// (layer by layer in z, x fastest inside a layer, so neighbouring lanes hit
+// neighbouring voxels) and folds the
 // exact point-triangle distance into the grid with an unsigned atomic min on
 // the float bit pattern (valid because every value is a non-negative float;
 // the grid starts at the 1e30 sentinel).  The min over exact values does not
@@ -59,26 +60,28 @@ __global__ __launch_bounds__(kBlock) void splat_kernel(const float* __restrict__
     while (z1 >= 0 && lattice(mnz, z1, h) > zhi) --z1;
     if (z1 < z0) return;
 
-    const unsigned nx = (unsigned)(x1 - x0 + 1), ny = (unsigned)(y1 - y0 + 1);
-    const unsigned nxy = nx * ny;
-    // grids are capped at 2e9 voxels, so one box always fits 32-bit indexing
-    const unsigned total = nxy * (unsigned)(z1 - z0 + 1);
+    // layer by layer in z; inside a layer the x-y box is indexed linearly
+    // (k -> (k / nx, k % nx) by a float reciprocal and an exact correction)
+    const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, nxy = nx * ny;
+    const float inv_nx = 1.0f / (float)nx;
     const int64_t zs = (int64_t)dx * dy;
-    for (unsigned k = threadIdx.x; k < total; k += kBlock) {
-        unsigned kz = k / nxy;
-        unsigned r = k - kz * nxy;
-        unsigned ky = r / nx;
-        unsigned kx = r - ky * nx;
-        int x = x0 + (int)kx, y = y0 + (int)ky, z = z0 + (int)kz;
-        float px = lattice(mnx, x, h), py = lattice(mny, y, h), pz = lattice(mnz, z, h);
-        float bdx = (px < tx0) ? (tx0 - px) : ((px > tx1) ? (px - tx1) : 0.0f);
-        float bdy = (py < ty0) ? (ty0 - py) : ((py > ty1) ? (py - ty1) : 0.0f);
-        float bdz = (pz < tz0) ? (tz0 - pz) : ((pz > tz1) ? (pz - tz1) : 0.0f);
-        if (bdx * bdx + bdy * bdy + bdz * bdz > band_sq) continue;
-        float d2 = point_tri_dist2(px, py, pz, tr);
-        if (d2 <= band_sq) {
-            float dist = sqrtf(d2);
-            atomicMin(grid + (int64_t)z * zs + (int64_t)y * dx + x, __float_as_uint(dist));
+    for (int z = z0; z <= z1; ++z) {
+        const float pz = lattice(mnz, z, h);
+        const float bdz = (pz < tz0) ? (tz0 - pz) : ((pz > tz1) ? (pz - tz1) : 0.0f);
+        for (int k = (int)threadIdx.x; k < nxy; k += kBlock) {
+            int ky = (int)((float)k * inv_nx);
+            if (ky * nx > k) --ky;
+            if ((ky + 1) * nx <= k) ++ky;
+            const int x = x0 + (k - ky * nx), y = y0 + ky;
+            const float px = lattice(mnx, x, h), py = lattice(mny, y, h);
+            const float bdx = (px < tx0) ? (tx0 - px) : ((px > tx1) ? (px - tx1) : 0.0f);
+            const float bdy = (py < ty0) ? (ty0 - py) : ((py > ty1) ? (py - ty1) : 0.0f);
+            if (bdx * bdx + bdy * bdy + bdz * bdz > band_sq) continue;
+            const float d2 = point_tri_dist2(px, py, pz, tr);
+            if (d2 <= band_sq) {
+                const float dist = sqrtf(d2);
+                atomicMin(grid + (int64_t)z * zs + (int64_t)y * dx + x, __float_as_uint(dist));
+            }
         }
     }
 }
