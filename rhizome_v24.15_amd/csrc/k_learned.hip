@@ -126,8 +126,22 @@ __device__ __forceinline__ bool segment_may_cross(float px, float py, float pz, 
 //  8 log1p(#triangles within 1 h)  9 normal spread within 1.5 h
 //  10 the certificate flag itself  11 distance from the voxel centre to the
 //  nearest crossing of its jittered column ray (voxels)
+// z extent of every triangle (the features kernel's cheap reject).
+__global__ void tri_zrange_kernel(const float* __restrict__ tris, int64_t nt, float2* __restrict__ zr) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (int64_t)gridDim.x * blockDim.x) {
+        const Tri9 v = load_tri(tris, t);
+        zr[t] = make_float2(fminf(v.az, fminf(v.bz, v.cz)), fmaxf(v.az, fmaxf(v.bz, v.cz)));
+    }
+}
+
+// Every feature depends only on triangles within 4 h of the voxel (the
+// second-nearest distance is capped at 4 h, ray crossings at 2 h, the rest
+// are nearer), so a triangle whose z extent is farther than 4.004 h from the
+// voxel is skipped before its distance is computed; the features are the same
+// bits as with every triangle of the column list.
 __global__ void features_kernel(const float* __restrict__ tris, const float4* __restrict__ xyab,
-                                const int64_t* __restrict__ start, const int32_t* __restrict__ ids,
+                                const float2* __restrict__ zr, const int64_t* __restrict__ start,
+                                const int32_t* __restrict__ ids,
                                 const uint8_t* __restrict__ sign, const int64_t* __restrict__ list, int64_t n,
                                 int mnx, int mny, int mnz, int dx, int dy, float h, int sup, int sdx, float jx,
                                 float jy, float* __restrict__ feats, uint8_t* __restrict__ risk,
@@ -153,8 +167,11 @@ __global__ void features_kernel(const float* __restrict__ tris, const float4* __
     const float eps = 2e-3f * h;
     const float half_len = 0.5f * sqrtf(jx * jx + jy * jy) * h;
     const int64_t s = (int64_t)(y / sup) * sdx + (x / sup);
+    const float reach = 4.004f * h;
     for (int64_t e = start[s]; e < start[s + 1]; ++e) {
         const int ti = ids[e];
+        const float2 q = zr[ti];
+        if (pz < q.x - reach || pz > q.y + reach) continue;
         const Tri9 t = load_tri(tris, ti);
         const float d2 = point_tri_dist2(px, py, pz, t);
         if (!risky && d2 <= (2.0f * half_len + eps) * (2.0f * half_len + eps)) {
@@ -354,12 +371,13 @@ hipError_t launch_candidates(const float* d, int64_t n, float thr, int64_t* list
     return hipGetLastError();
 }
 
-hipError_t launch_features(const float* tris, const CsrBuffers& b, const uint8_t* sign, const int64_t* list,
-                           int64_t n, const Grid& g, int sup, float jx, float jy, float* feats, uint8_t* risk,
-                           unsigned long long* n_risky, hipStream_t s) {
+hipError_t launch_features(const float* tris, int64_t nt, const CsrBuffers& b, float2* zr, const uint8_t* sign,
+                           const int64_t* list, int64_t n, const Grid& g, int sup, float jx, float jy, float* feats,
+                           uint8_t* risk, unsigned long long* n_risky, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int sdx = (g.dims[0] + sup - 1) / sup;
-    features_kernel<<<(unsigned)((n + 127) / 128), 128, 0, s>>>(tris, b.xyab, b.start, b.ids, sign, list, n, g.mn[0],
+    tri_zrange_kernel<<<(unsigned)std::min<int64_t>((nt + 255) / 256, 2048), 256, 0, s>>>(tris, nt, zr);
+    features_kernel<<<(unsigned)((n + 127) / 128), 128, 0, s>>>(tris, b.xyab, zr, b.start, b.ids, sign, list, n, g.mn[0],
                                                               g.mn[1], g.mn[2], g.dims[0], g.dims[1], g.h, sup, sdx,
                                                               jx, jy, feats, risk, n_risky);
     return hipGetLastError();

@@ -182,9 +182,9 @@ hipError_t launch_scatter_f64(const double* src, const int64_t* pos, int64_t m, 
 hipError_t launch_candidates(const float* d, int64_t n, float thr, int64_t* list, int64_t cap,
                              unsigned long long* counter, hipStream_t s);
 // 12 features per candidate (row-major n x 12) from the column candidate lists.
-hipError_t launch_features(const float* tris, const CsrBuffers& b, const uint8_t* sign, const int64_t* list,
-                           int64_t n, const Grid& g, int sup, float jx, float jy, float* feats, uint8_t* risk,
-                           unsigned long long* n_risky, hipStream_t s);
+hipError_t launch_features(const float* tris, int64_t nt, const CsrBuffers& b, float2* zr, const uint8_t* sign,
+                           const int64_t* list, int64_t n, const Grid& g, int sup, float jx, float jy, float* feats,
+                           uint8_t* risk, unsigned long long* n_risky, hipStream_t s);
 hipError_t launch_unprobed_risky(const int64_t* list, const uint8_t* risk, int64_t n, const unsigned* bits,
                                  int64_t* out, int64_t cap, unsigned long long* counter, hipStream_t s);
 hipError_t launch_margin_score(const float* feats, int64_t n, float* score, hipStream_t s);

@@ -108,6 +108,7 @@ enum Slot {
     S_ORDER,       // probe positions in spatial order
     S_FARPART,     // far-field partial sums per cluster range
     S_PROBES,      // ranked probes followed by the certificate fallback
+    S_ZRANGE,      // z extent of every triangle (candidate features)
     S_MC_INFO,     // marching cubes: cut mask + triangle count per voxel
     S_MC_BLK,      // per-block vertex and triangle totals
     S_MC_START,    // their prefix sums
@@ -952,10 +953,11 @@ int learned_candidates(Ctx& c, uint8_t* sign, const CsrBuffers& b, int64_t** can
         if (n <= cap) {
             float* f = (float*)ws_get(c.w, S_FEAT, sizeof(float) * 12 * (size_t)std::max<int64_t>(n, 1));
             uint8_t* r = (uint8_t*)ws_get(c.w, S_RISK, (size_t)std::max<int64_t>(n, 1));
-            if (!f || !r) return set_err(RHZ_ERR_NOMEM, "feature buffer allocation failed");
+            float2* zr = (float2*)ws_get(c.w, S_ZRANGE, sizeof(float2) * (size_t)std::max<int64_t>(c.nt, 1));
+            if (!f || !r || !zr) return set_err(RHZ_ERR_NOMEM, "feature buffer allocation failed");
             HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_RISKY), 0, sizeof(unsigned long long), c.s));
-            HIP_TRY(launch_features(c.tris, b, sign, list, n, c.g, c.p->sup, c.p->jitter_x, c.p->jitter_y, f, r,
-                                    cnt_ptr(c.w, C_RISKY), c.s));
+            HIP_TRY(launch_features(c.tris, c.nt, b, zr, sign, list, n, c.g, c.p->sup, c.p->jitter_x, c.p->jitter_y, f,
+                                    r, cnt_ptr(c.w, C_RISKY), c.s));
             rc = read_counters(c);
             if (rc) return rc;
             *cand = list;

@@ -313,3 +313,46 @@ def test_far_field_certificate_keeps_every_decision(rhz, geom, orc, request, cas
           "second pass", st6.far_second_pass, "undecided", st6.far_undecided)
     assert stats[0].far_decided == 0 and st6.far_decided + st6.far_undecided == st6.reprobed_voxels
     assert st6.far_decided >= 0.9 * st6.reprobed_voxels
+
+
+def test_candidate_features_against_fp64(rhz, orc, torch_gpu, config1):
+    """Features 0 (nearest distance / h), 2-4 (nearest normal) and 8
+    (log1p of the triangles within h) of a sample of candidates equal a
+    binary64 evaluation over EVERY triangle of the mesh: the kernel's column
+    lists and its z-extent reject drop no triangle these features need."""
+    torch = torch_gpu
+    tris, h, mn, mx = config1
+    d_tris = torch.from_numpy(tris).to("cuda")
+    cap = 1 << 20
+    d_vox = torch.empty(cap, dtype=torch.int64, device="cuda")
+    d_feat = torch.empty(cap * 12, dtype=torch.float32, device="cuda")
+    n = rhz.Proposer.candidates(d_tris, len(tris), h, mn, mx, d_vox, d_feat, cap)
+    assert n > 1000
+    vox = d_vox[:n].cpu().numpy()
+    feat = d_feat[:12 * n].cpu().numpy().reshape(n, 12)
+    dims = [mx[i] - mn[i] + 1 for i in range(3)]
+    L = orc.lib()
+    rng = np.random.default_rng(12)
+    V = tris.reshape(-1, 4, 3).astype(np.float64)
+    cen = V[:, :3].mean(axis=1)
+    reach = 2 * h + np.max(np.linalg.norm(V[:, :3] - cen[:, None], axis=2))  # every triangle within 2 h
+    checked = 0
+    for k in rng.choice(n, 150, replace=False):
+        v = int(vox[k])
+        x, y, z = v % dims[0], (v // dims[0]) % dims[1], v // (dims[0] * dims[1])
+        p = np.array([(mn[0] + x) * h, (mn[1] + y) * h, (mn[2] + z) * h], np.float64)
+        cand = np.nonzero(np.linalg.norm(cen - p, axis=1) <= reach)[0]
+        d2 = np.full(len(V), np.inf)
+        d2[cand] = [L.orc_point_triangle_distance_sq_f64(p, *(np.ascontiguousarray(V[t, j]) for j in range(3)))
+                    for t in cand]
+        if np.any(np.abs(np.sqrt(d2) - h) < 1e-4 * h):
+            continue  # a triangle at the 1 h threshold: its count depends on rounding
+        best = int(np.argmin(d2))
+        assert feat[k, 0] == pytest.approx(np.sqrt(d2[best]) / h, rel=1e-4, abs=1e-5)
+        # the nearest point may be a shared vertex or edge: any triangle tied at the minimum
+        tied = np.nonzero(d2 <= d2[best] * (1 + 1e-5) + 1e-12)[0]
+        normals = tris.reshape(-1, 4, 3)[tied, 3]
+        assert np.any(np.all(np.abs(normals - feat[k, 2:5]) <= 1e-6, axis=1))
+        assert feat[k, 8] == pytest.approx(np.log1p(np.sum(d2 <= h * h)), rel=1e-6)
+        checked += 1
+    assert checked > 100
