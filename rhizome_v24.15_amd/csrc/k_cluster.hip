@@ -119,10 +119,10 @@ __global__ void cluster_props_kernel(const float* __restrict__ tris, int64_t nce
                                      const int64_t* __restrict__ start, const int32_t* __restrict__ ids,
                                      const int64_t* __restrict__ slots, float4* __restrict__ info,
                                      int2* __restrict__ range) {
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t iters = (ncells + stride - 1) / stride;
     for (int64_t it = 0; it < iters; ++it) {
-        const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x + it * stride;
+        const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + it * stride;
         int64_t b = 0, e = 0;
         if (c < ncells) {
             b = start[c];
@@ -497,7 +497,10 @@ hipError_t launch_cluster_build(const float* tris, int64_t nt, const ClusterGrid
     e = hipGetLastError();
     if (e == hipSuccess) e = launch_scan_u32(counts, ncells, slot, scan_scratch, s);
     if (e != hipSuccess) return e;
-    cluster_props_kernel<<<cb, kBlock, 0, s>>>(tris, ncells, start, ids, slot, info, range);
+    // one thread per cell with a serial loop over its triangles: small blocks
+    // spread the cells over every CU
+    const unsigned pcb = (unsigned)std::min<int64_t>((ncells + 63) / 64, 65536);
+    cluster_props_kernel<<<pcb, 64, 0, s>>>(tris, ncells, start, ids, slot, info, range);
     const unsigned pb = (unsigned)std::min<int64_t>((npar + kBlock - 1) / kBlock, 8192);
     parent_flags_kernel<<<pb, kBlock, 0, s>>>(slot, npar, pflags);
     e = hipGetLastError();

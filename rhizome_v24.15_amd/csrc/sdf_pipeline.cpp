@@ -1392,6 +1392,10 @@ int rhz_tuning_set(const char* key, int64_t value) {
             return set_err(RHZ_ERR_ARG, "sweep_lines must be 1, 2 or 4");
         return RHZ_OK;
     }
+    if (!std::strcmp(key, "sweep_lines_z")) {
+        if (!set_sweep_lines_z((int)value)) return set_err(RHZ_ERR_ARG, "sweep_lines_z must be 1, 2 or 4");
+        return RHZ_OK;
+    }
     return set_err(RHZ_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -1412,6 +1416,10 @@ int rhz_tuning_get(const char* key, int64_t* value) {
     }
     if (!std::strcmp(key, "sweep_lines")) {
         *value = t.lines;
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "sweep_lines_z")) {
+        *value = t.lines_z;
         return RHZ_OK;
     }
     if (!std::strcmp(key, "fsm_batch_rounds_max_voxels")) {

@@ -167,8 +167,9 @@ def test_sweep_tunings_bit_exact(rhz, orc, torch_gpu, lines, step):
     with interior tiles (no lattice face) as well as ragged face tiles, and at
     a voxel size below the short square root's range (generic kernels)."""
     torch = torch_gpu
-    saved = {k: rhz.tuning_get(k) for k in ("sweep_lines", "sweep_step")}
+    saved = {k: rhz.tuning_get(k) for k in ("sweep_lines", "sweep_lines_z", "sweep_step")}
     rhz.tuning_set("sweep_lines", lines)
+    rhz.tuning_set("sweep_lines_z", lines)
     rhz.tuning_set("sweep_step", step)
     try:
         rng = np.random.default_rng(11)
