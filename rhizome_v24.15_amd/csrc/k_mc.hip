@@ -12,8 +12,9 @@
 //
 // Output order is deterministic (the reference hands out vertex and triangle
 // slots with atomic counters): vertices voxel by voxel in index order, x-, y-,
-// then z-edge; triangles cube by cube in table order.  Three passes over the
-// field: classify (cut-edge mask and triangle count per voxel, block totals),
+// then z-edge; triangles cube by cube in table order.  Passes over the field:
+// inside bytes (sealed value - iso <= 0, one byte per voxel), classify
+// (cut-edge mask and triangle count per voxel, block totals),
 // block prefix sums, then emission, where a block recomputes its in-block
 // prefix in LDS and a vertex id is its owner voxel's offset plus its rank in
 // the owner's mask.
@@ -59,18 +60,26 @@ __device__ __forceinline__ void mc_coords(const McField& F, int64_t v, int& x, i
     z = (int)(r / F.dy);
 }
 
-__device__ __forceinline__ int mc_case(const McField& F, int x, int y, int z) {
-    int c = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-        if (mc_inside(F, mc_val(F, x + c_corner[k][0], y + c_corner[k][1], z + c_corner[k][2]))) c |= 1 << k;
-    return c;
-}
-
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
     return v;
+}
+
+// Pass 0: one byte per voxel, 1 when the sealed value is inside.
+__global__ __launch_bounds__(kMcBlock) void mc_inside_kernel(McField F, uint8_t* __restrict__ in) {
+    for (int64_t v = (int64_t)blockIdx.x * kMcBlock + threadIdx.x; v < F.n; v += (int64_t)gridDim.x * kMcBlock) {
+        int x, y, z;
+        mc_coords(F, v, x, y, z);
+        in[v] = mc_inside(F, mc_val(F, x, y, z)) ? 1 : 0;
+    }
+}
+
+// Cube case of the cube at v from the inside bytes.
+__device__ __forceinline__ int mc_case_bytes(const uint8_t* __restrict__ in, int64_t v, int64_t ys, int64_t zs) {
+    return (int)in[v] | ((int)in[v + 1] << 1) | ((int)in[v + ys + 1] << 2) | ((int)in[v + ys] << 3) |
+           ((int)in[v + zs] << 4) | ((int)in[v + zs + 1] << 5) | ((int)in[v + zs + ys + 1] << 6) |
+           ((int)in[v + zs + ys] << 7);
 }
 
 // Exclusive prefix sum of `val` over the block (kMcBlock threads); *total
@@ -93,7 +102,8 @@ __device__ __forceinline__ uint32_t block_exclusive(uint32_t val, uint32_t* sh, 
 
 // Pass 1: info[v] = cut mask (bit a: the edge from v along axis a) | triangle
 // count of the cube at v << 3; per-block totals of vertices and triangles.
-__global__ __launch_bounds__(kMcBlock) void mc_classify_kernel(McField F, uint8_t* __restrict__ info,
+__global__ __launch_bounds__(kMcBlock) void mc_classify_kernel(McField F, const uint8_t* __restrict__ in,
+                                                               uint8_t* __restrict__ info,
                                                                uint32_t* __restrict__ blk_v,
                                                                uint32_t* __restrict__ blk_t) {
     __shared__ uint32_t sv, st;
@@ -104,13 +114,14 @@ __global__ __launch_bounds__(kMcBlock) void mc_classify_kernel(McField F, uint8_
     if (v < F.n) {
         int x, y, z;
         mc_coords(F, v, x, y, z);
-        const bool s0 = mc_inside(F, mc_val(F, x, y, z));
+        const int64_t ys = F.dx, zs = (int64_t)F.dx * F.dy;
+        const uint8_t s0 = in[v];
         int mask = 0;
-        if (x + 1 < F.dx && s0 != mc_inside(F, mc_val(F, x + 1, y, z))) mask |= 1;
-        if (y + 1 < F.dy && s0 != mc_inside(F, mc_val(F, x, y + 1, z))) mask |= 2;
-        if (z + 1 < F.dz && s0 != mc_inside(F, mc_val(F, x, y, z + 1))) mask |= 4;
+        if (x + 1 < F.dx && s0 != in[v + 1]) mask |= 1;
+        if (y + 1 < F.dy && s0 != in[v + ys]) mask |= 2;
+        if (z + 1 < F.dz && s0 != in[v + zs]) mask |= 4;
         if (x + 1 < F.dx && y + 1 < F.dy && z + 1 < F.dz) {
-            const int c = mc_case(F, x, y, z);
+            const int c = mc_case_bytes(in, v, ys, zs);
             ntri = (uint32_t)(kMcTriStart[c + 1] - kMcTriStart[c]) / 3u;
         }
         nvert = (uint32_t)__popc(mask);
@@ -161,7 +172,8 @@ __global__ __launch_bounds__(kMcBlock) void mc_vertex_kernel(McField F, float h,
 }
 
 // Pass 3b: triangles of every cube, (e0, e2, e1) per table triple.
-__global__ __launch_bounds__(kMcBlock) void mc_triangle_kernel(McField F, const uint8_t* __restrict__ info,
+__global__ __launch_bounds__(kMcBlock) void mc_triangle_kernel(McField F, const uint8_t* __restrict__ in,
+                                                               const uint8_t* __restrict__ info,
                                                                const int64_t* __restrict__ tstart,
                                                                const uint32_t* __restrict__ voff,
                                                                uint32_t* __restrict__ tris) {
@@ -171,10 +183,8 @@ __global__ __launch_bounds__(kMcBlock) void mc_triangle_kernel(McField F, const 
     uint32_t total;
     const uint32_t excl = block_exclusive(ntri, sh, &total);
     if (ntri == 0) return;
-    int x, y, z;
-    mc_coords(F, v, x, y, z);
-    const int c = mc_case(F, x, y, z);
     const int64_t ys = F.dx, zs = (int64_t)F.dx * F.dy;
+    const int c = mc_case_bytes(in, v, ys, zs);
     int64_t out = tstart[blockIdx.x] + excl;
     for (int k = kMcTriStart[c]; k < kMcTriStart[c + 1]; k += 3, ++out) {
         uint32_t id[3];
@@ -208,20 +218,22 @@ int64_t mc_blocks(const int32_t dims[3]) {
     return (n + kMcBlock - 1) / kMcBlock;
 }
 
-hipError_t launch_mc_classify(const float* g, const int32_t dims[3], float h, float iso, uint8_t* info,
+hipError_t launch_mc_classify(const float* g, const int32_t dims[3], float h, float iso, uint8_t* in, uint8_t* info,
                               uint32_t* blk_v, uint32_t* blk_t, hipStream_t s) {
     const McField F{g, dims[0], dims[1], dims[2], iso, iso + h, (int64_t)dims[0] * dims[1] * dims[2]};
-    mc_classify_kernel<<<(unsigned)mc_blocks(dims), kMcBlock, 0, s>>>(F, info, blk_v, blk_t);
+    const unsigned ib = (unsigned)std::min<int64_t>(mc_blocks(dims), 65536);
+    mc_inside_kernel<<<ib, kMcBlock, 0, s>>>(F, in);
+    mc_classify_kernel<<<(unsigned)mc_blocks(dims), kMcBlock, 0, s>>>(F, in, info, blk_v, blk_t);
     return hipGetLastError();
 }
 
 hipError_t launch_mc_emit(const float* g, const int32_t dims[3], float h, const float origin[3], float iso,
-                          const uint8_t* info, const int64_t* vstart, const int64_t* tstart, uint32_t* voff,
-                          float* verts, uint32_t* tris, hipStream_t s) {
+                          const uint8_t* in, const uint8_t* info, const int64_t* vstart, const int64_t* tstart,
+                          uint32_t* voff, float* verts, uint32_t* tris, hipStream_t s) {
     const McField F{g, dims[0], dims[1], dims[2], iso, iso + h, (int64_t)dims[0] * dims[1] * dims[2]};
     const unsigned nb = (unsigned)mc_blocks(dims);
     mc_vertex_kernel<<<nb, kMcBlock, 0, s>>>(F, h, origin[0], origin[1], origin[2], info, vstart, voff, verts);
-    mc_triangle_kernel<<<nb, kMcBlock, 0, s>>>(F, info, tstart, voff, tris);
+    mc_triangle_kernel<<<nb, kMcBlock, 0, s>>>(F, in, info, tstart, voff, tris);
     return hipGetLastError();
 }
 

@@ -94,16 +94,16 @@ hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool 
 // x-sweep on the y-fastest layout (src/dst both y-fastest).
 hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir, int* changed, hipStream_t s);
 // ---- k_mc.hip -------------------------------------------------------------
-// Marching cubes in three passes: classify (info: u8 per voxel; blk_v / blk_t:
+// Marching cubes in three passes: classify (in: inside byte per voxel; info: u8 per voxel; blk_v / blk_t:
 // u32 per block of mc_blocks(dims)), prefix sums of the block totals into
 // vstart / tstart (int64, blocks + 1 entries), then emission (voff: u32 per
 // voxel scratch; verts: 3 floats per vertex; tris: 3 ids per triangle).
 int64_t mc_blocks(const int32_t dims[3]);
-hipError_t launch_mc_classify(const float* g, const int32_t dims[3], float h, float iso, uint8_t* info,
+hipError_t launch_mc_classify(const float* g, const int32_t dims[3], float h, float iso, uint8_t* in, uint8_t* info,
                               uint32_t* blk_v, uint32_t* blk_t, hipStream_t s);
 hipError_t launch_mc_emit(const float* g, const int32_t dims[3], float h, const float origin[3], float iso,
-                          const uint8_t* info, const int64_t* vstart, const int64_t* tstart, uint32_t* voff,
-                          float* verts, uint32_t* tris, hipStream_t s);
+                          const uint8_t* in, const uint8_t* info, const int64_t* vstart, const int64_t* tstart,
+                          uint32_t* voff, float* verts, uint32_t* tris, hipStream_t s);
 // count += number of values <= 0 among g[0..n).
 hipError_t launch_interior_count(const float* g, int64_t n, unsigned long long* count, hipStream_t s);
 hipError_t launch_redblack(float* g, const int32_t dims[3], float h, int parity, hipStream_t s);

@@ -110,6 +110,7 @@ enum Slot {
     S_PROBES,      // ranked probes followed by the certificate fallback
     S_ZRANGE,      // z extent of every triangle (candidate features)
     S_MC_INFO,     // marching cubes: cut mask + triangle count per voxel
+    S_MC_IN,       // marching cubes: inside byte per voxel
     S_MC_BLK,      // per-block vertex and triangle totals
     S_MC_START,    // their prefix sums
     S_MC_SCAN,
@@ -1722,11 +1723,13 @@ int iso_surface_run(Workspace* w, const float* d_grid, const int32_t dims[3], fl
     const int64_t nb = mc_blocks(dims);
     if (nb > (int64_t)INT32_MAX) return set_err(RHZ_ERR_ARG, "lattice too large");
     uint8_t* info = (uint8_t*)ws_get(w, S_MC_INFO, (size_t)V);
+    uint8_t* in = (uint8_t*)ws_get(w, S_MC_IN, (size_t)V);
     uint32_t* blk = (uint32_t*)ws_get(w, S_MC_BLK, sizeof(uint32_t) * 2 * (size_t)nb);
     int64_t* start = (int64_t*)ws_get(w, S_MC_START, sizeof(int64_t) * 2 * (size_t)(nb + 1));
     void* scan = ws_get(w, S_MC_SCAN, scan_scratch_bytes(nb));
-    if (!info || !blk || !start || !scan) return set_err(RHZ_ERR_NOMEM, "marching-cubes scratch allocation failed");
-    HIP_TRY(launch_mc_classify(d_grid, dims, h, iso, info, blk, blk + nb, s));
+    if (!info || !in || !blk || !start || !scan)
+        return set_err(RHZ_ERR_NOMEM, "marching-cubes scratch allocation failed");
+    HIP_TRY(launch_mc_classify(d_grid, dims, h, iso, in, info, blk, blk + nb, s));
     HIP_TRY(launch_scan_u32(blk, nb, start, scan, s));
     HIP_TRY(launch_scan_u32(blk + nb, nb, start + nb + 1, scan, s));
     int64_t tot[2];
@@ -1739,7 +1742,7 @@ int iso_surface_run(Workspace* w, const float* d_grid, const int32_t dims[3], fl
     if (tot[0] == 0 || tot[1] == 0 || !d_verts || !d_tris || verts_cap < tot[0] || tris_cap < tot[1]) return RHZ_OK;
     uint32_t* voff = (uint32_t*)ws_get(w, S_MC_VOFF, sizeof(uint32_t) * (size_t)V);
     if (!voff) return set_err(RHZ_ERR_NOMEM, "marching-cubes scratch allocation failed");
-    HIP_TRY(launch_mc_emit(d_grid, dims, h, origin, iso, info, start, start + nb + 1, voff, d_verts, d_tris, s));
+    HIP_TRY(launch_mc_emit(d_grid, dims, h, origin, iso, in, info, start, start + nb + 1, voff, d_verts, d_tris, s));
     return RHZ_OK;
 }
 
