@@ -353,7 +353,8 @@ def main():
     batch = None
     if not args.no_batch:
         try:
-            saved = {k: rhz.tuning_get(k) for k in ("sweep_lines", "sweep_lines_z", "sweep_step", "sweep_rows",
+            saved = {k: rhz.tuning_get(k) for k in ("sweep_shape_x", "sweep_shape_y", "sweep_shape_z", "sweep_step",
+                                                    "sweep_rows",
                                                     "fsm_batch_rounds_max_voxels", "fsm_graph")}
             batch = batched_bake(rhz, torch, params, log=log)
             for k, v in saved.items():

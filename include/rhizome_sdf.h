@@ -169,16 +169,19 @@ int rhz_device_count(void);
  * CudaRasterizer::query_gpu_memory (cuda_rasterizer.hpp:18). */
 int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
 /* Launch-shape tuning of the kernels (no counterpart in the reference; used by
- * the autotuner): "sweep_lines" (1, 2, 4: lines each lane of an x- or
- * y-sweep walks: 2 = two fast-axis segments, the default; 4 = two segments x
- * two cross lines), "sweep_lines_z" (the same for the z-sweeps, default 4),
- * "sweep_step" (2, 4, 8: steps whose loads a sweep wave issues together,
- * default 2), "sweep_rows"
- * (4, 8, 16: waves per block) and "sweep_fmajor" (1: a block's waves take
- * adjacent segments of one row, 0: adjacent rows; both for sweep_lines 1),
- * "fsm_batch_rounds_max_voxels" (lattices up to this size enqueue all FSM
- * rounds without host round trips) and "fsm_graph" (1: those rounds replay as
- * one captured graph).  Process-wide; results are identical for every setting. */
+ * the autotuner), process-wide; results are identical for every setting:
+ *   "sweep_shape_x" / "_y" / "_z"  tile each lane of an x-, y- or z-sweep walks:
+ *                                  1 = one line (rows / fmajor apply), or
+ *                                  10 NF + NC = NF fast-axis segments x NC cross
+ *                                  lines: 21, 22, 12, 14 (defaults 14, 21, 22);
+ *   "sweep_step"                   2, 4, 8: steps whose loads a sweep wave issues
+ *                                  together (default 2);
+ *   "sweep_rows", "sweep_fmajor"   waves per block (4, 8, 16) and 1: a block's
+ *                                  waves take adjacent segments of one row, for
+ *                                  the one-line walker;
+ *   "fsm_batch_rounds_max_voxels"  lattices up to this size enqueue all FSM
+ *                                  rounds without host round trips;
+ *   "fsm_graph"                    1: those rounds replay as one captured graph. */
 int rhz_tuning_set(const char* key, int64_t value);
 int rhz_tuning_get(const char* key, int64_t* value);
 /* Frees the cached device workspace of the current device. */

@@ -3,7 +3,7 @@ sample of the batch and keeps the fastest.  Every configuration computes
 identical results; only the time differs.
 
 Searched, coordinate-wise: the number of concurrent lanes; the sweep kernel
-shape (lines per lane for x/y- and for z-sweeps, steps whose loads are issued
+shape (the tile per sweep orientation, steps whose loads are issued
 together); whether the
 FSM rounds of small lattices are enqueued in one go and replayed as a
 hipGraph.
@@ -15,7 +15,8 @@ from __future__ import annotations
 import time
 
 LANES = (1, 2, 4, 8)
-SWEEP_SHAPES = ((2, 4, 2), (2, 2, 2), (2, 4, 4), (1, 1, 8), (1, 4, 8))  # (sweep_lines, sweep_lines_z, sweep_step)
+# (sweep_shape_x, sweep_shape_y, sweep_shape_z, sweep_step)
+SWEEP_SHAPES = ((14, 21, 22, 2), (21, 21, 21, 2), (12, 12, 14, 2), (1, 1, 1, 8), (14, 1, 22, 8))
 FSM_MODES = ((16 << 20, 0), (16 << 20, 1), (0, 0))  # (fsm_batch_rounds_max_voxels, fsm_graph)
 
 
@@ -30,9 +31,8 @@ def _time_bake(rhz, torch, items, params, n_lanes):
 
 
 def _apply(rhz, cfg):
-    rhz.tuning_set("sweep_lines", cfg["sweep_lines"])
-    rhz.tuning_set("sweep_lines_z", cfg["sweep_lines_z"])
-    rhz.tuning_set("sweep_step", cfg["sweep_step"])
+    for k in ("sweep_shape_x", "sweep_shape_y", "sweep_shape_z", "sweep_step"):
+        rhz.tuning_set(k, cfg[k])
     rhz.tuning_set("fsm_batch_rounds_max_voxels", cfg["fsm_batch_rounds_max_voxels"])
     rhz.tuning_set("fsm_graph", cfg["fsm_graph"])
 
@@ -41,8 +41,8 @@ def autotune_bake(rhz, torch, items, params, sample=16, log=None):
     """Returns (best config with its time, [all timings]) and leaves the
     library tuned to the winner."""
     sub = list(items[:sample])
-    cfg = {"n_lanes": 4, "sweep_lines": rhz.tuning_get("sweep_lines"),
-           "sweep_lines_z": rhz.tuning_get("sweep_lines_z"), "sweep_step": rhz.tuning_get("sweep_step"),
+    cfg = {"n_lanes": 4, **{k: rhz.tuning_get(k) for k in ("sweep_shape_x", "sweep_shape_y", "sweep_shape_z",
+                                                           "sweep_step")},
            "fsm_batch_rounds_max_voxels": rhz.tuning_get("fsm_batch_rounds_max_voxels"),
            "fsm_graph": rhz.tuning_get("fsm_graph")}
     table = []
@@ -64,8 +64,8 @@ def autotune_bake(rhz, torch, items, params, sample=16, log=None):
         ms = trial(c)
         if best_ms is None or ms < best_ms:
             best_ms, cfg = ms, c
-    for lines, lines_z, step in SWEEP_SHAPES:
-        c = dict(cfg, sweep_lines=lines, sweep_lines_z=lines_z, sweep_step=step)
+    for sx, sy, sz, step in SWEEP_SHAPES:
+        c = dict(cfg, sweep_shape_x=sx, sweep_shape_y=sy, sweep_shape_z=sz, sweep_step=step)
         ms = trial(c)
         if ms < best_ms:
             best_ms, cfg = ms, c

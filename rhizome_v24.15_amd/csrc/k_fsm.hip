@@ -650,21 +650,20 @@ __global__ void redblack_kernel(float* __restrict__ g, int dx, int dy, int dz, f
 
 // Launch shape of the lane sweeps (steps per load group, waves per block);
 // set by the tuning entry point, read at every launch.
-static SweepTuning g_sweep_tuning = {2, 4, 0, 2, 4, 16ll << 20, 0};
+static SweepTuning g_sweep_tuning = {2, 4, 0, {14, 21, 22}, 16ll << 20, 0};
 SweepTuning sweep_tuning() { return g_sweep_tuning; }
-bool set_sweep_tuning(int step, int rows, int fmajor, int lines) {
+bool set_sweep_tuning(int step, int rows, int fmajor) {
     if ((step != 2 && step != 4 && step != 8) || (rows != 4 && rows != 8 && rows != 16) ||
-        (fmajor != 0 && fmajor != 1) || (lines != 1 && lines != 2 && lines != 4))
+        (fmajor != 0 && fmajor != 1))
         return false;
     g_sweep_tuning.step = step;
     g_sweep_tuning.rows = rows;
     g_sweep_tuning.fmajor = fmajor;
-    g_sweep_tuning.lines = lines;
     return true;
 }
-bool set_sweep_lines_z(int lines) {
-    if (lines != 1 && lines != 2 && lines != 4) return false;
-    g_sweep_tuning.lines_z = lines;
+bool set_sweep_shape(int axis, int shape) {
+    if (axis < 0 || axis > 2 || (shape != 1 && shape != 21 && shape != 22 && shape != 12 && shape != 14)) return false;
+    g_sweep_tuning.shape[axis] = shape;
     return true;
 }
 void set_fsm_batching(int64_t max_voxels, int graph) {
@@ -706,17 +705,19 @@ template <int SF, int SW, int SC, int DIR>
 hipError_t launch_lane_d(const float* src, float* dst, int nf, int nw, int nc, int64_t ws, int64_t cs, float h,
                          int* changed, hipStream_t s) {
     const SweepTuning t = sweep_tuning();
-    const int lines = (SW == 2) ? t.lines_z : t.lines;  // z-sweeps have a shape of their own
-    if (lines == 4) {
-        if (t.step == 2) return launch_tile<SF, SW, SC, DIR, 2, 2, 2>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
-        if (t.step == 8) return launch_tile<SF, SW, SC, DIR, 8, 2, 2>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
-        return launch_tile<SF, SW, SC, DIR, 4, 2, 2>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
+    // the walk axis names the orientation (the x-sweep runs on the y-fastest copy)
+    const int shape = t.shape[SW];
+#define RHZ_TILE(NF, NC)                                                                                    \
+    if (shape == 10 * NF + NC) {                                                                             \
+        if (t.step == 2) return launch_tile<SF, SW, SC, DIR, 2, NF, NC>(src, dst, nf, nw, nc, ws, cs, h, changed, s); \
+        if (t.step == 8) return launch_tile<SF, SW, SC, DIR, 8, NF, NC>(src, dst, nf, nw, nc, ws, cs, h, changed, s); \
+        return launch_tile<SF, SW, SC, DIR, 4, NF, NC>(src, dst, nf, nw, nc, ws, cs, h, changed, s);         \
     }
-    if (lines == 2) {
-        if (t.step == 2) return launch_tile<SF, SW, SC, DIR, 2, 2, 1>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
-        if (t.step == 8) return launch_tile<SF, SW, SC, DIR, 8, 2, 1>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
-        return launch_tile<SF, SW, SC, DIR, 4, 2, 1>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
-    }
+    RHZ_TILE(2, 1)
+    RHZ_TILE(2, 2)
+    RHZ_TILE(1, 2)
+    RHZ_TILE(1, 4)
+#undef RHZ_TILE
     if (t.step == 8) {
         if (t.rows == 4) return launch_lane_v<SF, SW, SC, DIR, 8, 4>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
         if (t.rows == 16) return launch_lane_v<SF, SW, SC, DIR, 8, 16>(src, dst, nf, nw, nc, ws, cs, h, changed, s);

@@ -79,15 +79,16 @@ struct SweepTuning {
     int step;  // walk steps whose loads are issued together (2, 4, 8)
     int rows;  // waves per block (4, 8, 16)
     int fmajor;  // 1: a block's waves are adjacent segments of one row; 0: adjacent rows
-    int lines;   // lines per lane: 1 (one-line walker, rows / fmajor apply), 2 (tile of two
-                 // fast-axis segments), 4 (tile of 2 segments x 2 cross lines)
-    int lines_z; // the same for the z-sweeps (their cross lines are one row apart)
+    // Tile shape per sweep orientation (x: on the y-fastest copy; y; z): 1 =
+    // one-line walker (rows / fmajor apply); 10 NF + NC = tile of NF fast-axis
+    // segments x NC cross lines per lane, one of 21, 22, 12, 14.
+    int shape[3];
     int64_t batch_rounds_max_voxels;  // lattices up to this size enqueue all FSM rounds at once
     int fsm_graph;                    // 1: those rounds are captured and replayed as one hipGraph
 };
 SweepTuning sweep_tuning();
-bool set_sweep_tuning(int step, int rows, int fmajor, int lines);
-bool set_sweep_lines_z(int lines);
+bool set_sweep_tuning(int step, int rows, int fmajor);
+bool set_sweep_shape(int axis, int shape);
 void set_fsm_batching(int64_t max_voxels, int graph);
 // Per-plane transpose between the x-fastest and the y-fastest layouts.
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s);

@@ -1367,15 +1367,15 @@ int rhz_tuning_set(const char* key, int64_t value) {
     if (!key) return set_err(RHZ_ERR_ARG, "null tuning key");
     const SweepTuning t = sweep_tuning();
     if (!std::strcmp(key, "sweep_step")) {
-        if (!set_sweep_tuning((int)value, t.rows, t.fmajor, t.lines)) return set_err(RHZ_ERR_ARG, "sweep_step must be 2, 4 or 8");
+        if (!set_sweep_tuning((int)value, t.rows, t.fmajor)) return set_err(RHZ_ERR_ARG, "sweep_step must be 2, 4 or 8");
         return RHZ_OK;
     }
     if (!std::strcmp(key, "sweep_rows")) {
-        if (!set_sweep_tuning(t.step, (int)value, t.fmajor, t.lines)) return set_err(RHZ_ERR_ARG, "sweep_rows must be 4, 8 or 16");
+        if (!set_sweep_tuning(t.step, (int)value, t.fmajor)) return set_err(RHZ_ERR_ARG, "sweep_rows must be 4, 8 or 16");
         return RHZ_OK;
     }
     if (!std::strcmp(key, "sweep_fmajor")) {
-        if (!set_sweep_tuning(t.step, t.rows, (int)value, t.lines)) return set_err(RHZ_ERR_ARG, "sweep_fmajor must be 0 or 1");
+        if (!set_sweep_tuning(t.step, t.rows, (int)value)) return set_err(RHZ_ERR_ARG, "sweep_fmajor must be 0 or 1");
         return RHZ_OK;
     }
     if (!std::strcmp(key, "fsm_batch_rounds_max_voxels")) {
@@ -1388,15 +1388,13 @@ int rhz_tuning_set(const char* key, int64_t value) {
         set_fsm_batching(t.batch_rounds_max_voxels, (int)value);
         return RHZ_OK;
     }
-    if (!std::strcmp(key, "sweep_lines")) {
-        if (!set_sweep_tuning(t.step, t.rows, t.fmajor, (int)value))
-            return set_err(RHZ_ERR_ARG, "sweep_lines must be 1, 2 or 4");
-        return RHZ_OK;
-    }
-    if (!std::strcmp(key, "sweep_lines_z")) {
-        if (!set_sweep_lines_z((int)value)) return set_err(RHZ_ERR_ARG, "sweep_lines_z must be 1, 2 or 4");
-        return RHZ_OK;
-    }
+    static const char* kShapeKeys[3] = {"sweep_shape_x", "sweep_shape_y", "sweep_shape_z"};
+    for (int a = 0; a < 3; ++a)
+        if (!std::strcmp(key, kShapeKeys[a])) {
+            if (!set_sweep_shape(a, (int)value))
+                return set_err(RHZ_ERR_ARG, "%s must be 1, 21, 22, 12 or 14", kShapeKeys[a]);
+            return RHZ_OK;
+        }
     return set_err(RHZ_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -1415,14 +1413,12 @@ int rhz_tuning_get(const char* key, int64_t* value) {
         *value = t.fmajor;
         return RHZ_OK;
     }
-    if (!std::strcmp(key, "sweep_lines")) {
-        *value = t.lines;
-        return RHZ_OK;
-    }
-    if (!std::strcmp(key, "sweep_lines_z")) {
-        *value = t.lines_z;
-        return RHZ_OK;
-    }
+    static const char* kShapeKeys[3] = {"sweep_shape_x", "sweep_shape_y", "sweep_shape_z"};
+    for (int a = 0; a < 3; ++a)
+        if (!std::strcmp(key, kShapeKeys[a])) {
+            *value = t.shape[a];
+            return RHZ_OK;
+        }
     if (!std::strcmp(key, "fsm_batch_rounds_max_voxels")) {
         *value = t.batch_rounds_max_voxels;
         return RHZ_OK;

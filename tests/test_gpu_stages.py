@@ -161,15 +161,17 @@ def test_sweep_odd_shapes(rhz, orc, torch_gpu, shape):
             assert np.array_equal(_host(dd).view(np.uint32), ref.view(np.uint32)), (shape, axis, direction)
 
 
-@pytest.mark.parametrize("lines,step", [(2, 2), (2, 4), (2, 8), (4, 2), (4, 4), (4, 8), (1, 4), (1, 8)])
-def test_sweep_tunings_bit_exact(rhz, orc, torch_gpu, lines, step):
+@pytest.mark.parametrize("shape,step", [(21, 2), (21, 4), (21, 8), (22, 2), (22, 8), (12, 2), (12, 4), (14, 2),
+                                        (14, 8), (1, 4), (1, 8)])
+def test_sweep_tunings_bit_exact(rhz, orc, torch_gpu, shape, step):
     """Every sweep launch shape equals the oracle bit for bit, on a lattice
     with interior tiles (no lattice face) as well as ragged face tiles, and at
     a voxel size below the short square root's range (generic kernels)."""
     torch = torch_gpu
-    saved = {k: rhz.tuning_get(k) for k in ("sweep_lines", "sweep_lines_z", "sweep_step")}
-    rhz.tuning_set("sweep_lines", lines)
-    rhz.tuning_set("sweep_lines_z", lines)
+    keys = ("sweep_shape_x", "sweep_shape_y", "sweep_shape_z", "sweep_step")
+    saved = {k: rhz.tuning_get(k) for k in keys}
+    for k in keys[:3]:
+        rhz.tuning_set(k, shape)
     rhz.tuning_set("sweep_step", step)
     try:
         rng = np.random.default_rng(11)
