@@ -601,28 +601,47 @@ __global__ __launch_bounds__(64 * ROWS) void sweep_lane_kernel(const float* __re
 // ---- plane transpose (x-fastest <-> y-fastest) ----------------------------------
 // Each z-plane is an (rows x cols) matrix; out[col][row] = in[row][col].
 // 64x64 tiles through LDS (one padding column: conflict-free column reads).
+// A block transposes kTrTiles tiles along a tile row: the next tile's loads
+// are issued before the current one is stored (double-buffered LDS), which
+// keeps the transpose at copy speed.
+constexpr int kTrTiles = 2;
+
 __global__ __launch_bounds__(256) void transpose_planes_kernel(const float* __restrict__ in, float* __restrict__ out,
                                                                int rows, int cols, int tiles_r, int tiles_c,
                                                                unsigned nblocks) {
-    __shared__ float tile[64][65];
+    __shared__ float tile[2][64][65];
+    const int groups_c = (tiles_c + kTrTiles - 1) / kTrTiles;
     const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
-    const int per_plane = tiles_r * tiles_c;
+    const int per_plane = tiles_r * groups_c;
     const int z = (int)(lb / (unsigned)per_plane);
     const int t = (int)(lb % (unsigned)per_plane);
-    const int tr = t / tiles_c, tc = t % tiles_c;
+    const int tr = t / groups_c, tg = t % groups_c;
     const int64_t plane = (int64_t)z * rows * cols;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
-    const int c0 = tc * 64, r0 = tr * 64;
+    const int r0 = tr * 64;
+    float v[16];
+    auto load = [&](int tc) {
+        const int c = tc * 64 + tx;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const int r = r0 + ty + 4 * k, c = c0 + tx;
-        if (r < rows && c < cols) tile[ty + 4 * k][tx] = in[plane + (int64_t)r * cols + c];
-    }
-    __syncthreads();
+        for (int k = 0; k < 16; ++k) {
+            const int r = r0 + ty + 4 * k;
+            v[k] = (r < rows && c < cols) ? in[plane + (int64_t)r * cols + c] : 0.0f;
+        }
+    };
+    const int tc0 = tg * kTrTiles, tc1 = min(tiles_c, tc0 + kTrTiles);
+    load(tc0);
+    for (int tc = tc0; tc < tc1; ++tc) {
+        const int buf = (tc - tc0) & 1;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const int c = c0 + ty + 4 * k, r = r0 + tx;
-        if (r < rows && c < cols) out[plane + (int64_t)c * rows + r] = tile[tx][ty + 4 * k];
+        for (int k = 0; k < 16; ++k) tile[buf][ty + 4 * k][tx] = v[k];
+        __syncthreads();  // also orders the previous use of the other buffer
+        if (tc + 1 < tc1) load(tc + 1);
+        const int r = r0 + tx;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int c = tc * 64 + ty + 4 * k;
+            if (r < rows && c < cols) out[plane + (int64_t)c * rows + r] = tile[buf][tx][ty + 4 * k];
+        }
     }
 }
 
@@ -778,7 +797,7 @@ hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool 
     const int dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
     const int rows = to_yx ? dy : dx, cols = to_yx ? dx : dy;
     const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
-    const int64_t blocks = (int64_t)tr * tc * dz;
+    const int64_t blocks = (int64_t)tr * ((tc + kTrTiles - 1) / kTrTiles) * dz;
     transpose_planes_kernel<<<(unsigned)blocks, 256, 0, s>>>(in, out, rows, cols, tr, tc, (unsigned)blocks);
     return hipGetLastError();
 }
