@@ -15,6 +15,13 @@
 // compares integers.  k values live in LDS in a lane-interleaved layout
 // (bank-conflict free), are insertion-sorted, and the column is written as
 // one byte per voxel.
+//
+// The walk is latency-bound (candidate id -> XY box -> triangle are three
+// dependent loads), so the ids and boxes of kWalkBatch entries are fetched
+// together before any of them is tested; entries are still tested in list
+// order, so the overflow cut-off sees exactly the same sequence.  Keys are
+// 16-bit when they fit (dz <= 32767), which halves the LDS per block and
+// doubles the waves a CU keeps in flight.
 #include "rhz_device.hpp"
 #include "rhz_internal.hpp"
 
@@ -23,8 +30,10 @@ namespace rhz {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kParityBlock = 64;     // one wave: 32 KB of hit indices in LDS
+constexpr int kParityBlock = 64;     // one wave: 16 or 32 KB of hit keys in LDS
 constexpr int kParityMaxHits = 128;  // LDS capacity per lane (reference limit)
+constexpr int kWalkBatch = 8;        // candidate entries whose loads are issued together
+constexpr int kShortKeyMaxDz = 32767;  // (dz << 1) | 1 fits 16 bits
 
 __device__ __forceinline__ int floor_div_trunc(int a, int b) { return a / b; }  // C semantics
 
@@ -100,13 +109,13 @@ __device__ __forceinline__ int toggle_index(float hit, int mnz, int dz, float h)
 // facing +z: the ray leaves the solid) / -1 (facing -z).  Inside = w != 0,
 // which also classifies overlapping shells (w = 2) that parity calls outside.
 // A column whose crossings do not cancel (w above the top != 0) is deferred
-// like an odd one.
-template <bool SIGNED>
+// like an odd one.  Key = uint16_t when dz <= kShortKeyMaxDz, else int.
+template <bool SIGNED, typename Key>
 __global__ __launch_bounds__(kParityBlock) void parity_kernel(
     const float* __restrict__ tris, const float4* __restrict__ xyab, const int64_t* __restrict__ start,
     const int32_t* __restrict__ ids, int mnx, int mny, int mnz, int dx, int dy, int dz, float h, int sup,
     int sdx, float jx, float jy, int max_hits, uint8_t* __restrict__ sign, int* __restrict__ flag) {
-    __shared__ int kbuf[kParityMaxHits][kParityBlock];
+    __shared__ Key kbuf[kParityMaxHits][kParityBlock];
     const int lane = threadIdx.x;
     const int64_t col = (int64_t)blockIdx.x * kParityBlock + lane;
     const int64_t ncols = (int64_t)dx * dy;
@@ -119,28 +128,38 @@ __global__ __launch_bounds__(kParityBlock) void parity_kernel(
     bool bad = false;
     const int64_t s = (int64_t)(cy / sup) * sdx + (cx / sup);
     const int64_t e0 = start[s], e1 = start[s + 1];
-    for (int64_t e = e0; e < e1; ++e) {
-        int ti = ids[e];
-        float4 ab = xyab[ti];
-        if (rx < ab.x || rx > ab.z || ry < ab.y || ry > ab.w) continue;
-        Tri9 t = load_tri(tris, ti);
-        float d0 = (t.bx - t.ax) * (ry - t.ay) - (t.by - t.ay) * (rx - t.ax);
-        float d1 = (t.cx - t.bx) * (ry - t.by) - (t.cy - t.by) * (rx - t.bx);
-        float d2 = (t.ax - t.cx) * (ry - t.cy) - (t.ay - t.cy) * (rx - t.cx);
-        bool pos = (d0 > 0.0f && d1 > 0.0f && d2 > 0.0f);
-        bool neg = (d0 < 0.0f && d1 < 0.0f && d2 < 0.0f);
-        if (!pos && !neg) continue;
-        float area2 = d0 + d1 + d2;
-        if (fabsf(area2) < 1e-12f) continue;
-        float zh = (d1 * t.az + d2 * t.bz + d0 * t.cz) / area2;
-        if (n < max_hits) {
-            const int k = toggle_index(zh, mnz, dz, h);
-            kbuf[n][lane] = SIGNED ? ((k << 1) | (pos ? 1 : 0)) : k;
-            total += pos ? 1 : -1;
-            ++n;
-        } else {
-            bad = true;
-            break;
+    for (int64_t eb = e0; eb < e1 && !bad; eb += kWalkBatch) {
+        // every index is clamped into [e0, e1), so the loads need no guard
+        int tid[kWalkBatch];
+#pragma unroll
+        for (int j = 0; j < kWalkBatch; ++j) tid[j] = ids[min(eb + j, e1 - 1)];
+        bool in_box[kWalkBatch];
+#pragma unroll
+        for (int j = 0; j < kWalkBatch; ++j) {
+            const float4 ab = xyab[tid[j]];
+            in_box[j] = (eb + j < e1) && !(rx < ab.x || rx > ab.z || ry < ab.y || ry > ab.w);
+        }
+#pragma unroll
+        for (int j = 0; j < kWalkBatch; ++j) {
+            if (!in_box[j] || bad) continue;
+            Tri9 t = load_tri(tris, tid[j]);
+            float d0 = (t.bx - t.ax) * (ry - t.ay) - (t.by - t.ay) * (rx - t.ax);
+            float d1 = (t.cx - t.bx) * (ry - t.by) - (t.cy - t.by) * (rx - t.bx);
+            float d2 = (t.ax - t.cx) * (ry - t.cy) - (t.ay - t.cy) * (rx - t.cx);
+            bool pos = (d0 > 0.0f && d1 > 0.0f && d2 > 0.0f);
+            bool neg = (d0 < 0.0f && d1 < 0.0f && d2 < 0.0f);
+            if (!pos && !neg) continue;
+            float area2 = d0 + d1 + d2;
+            if (fabsf(area2) < 1e-12f) continue;
+            float zh = (d1 * t.az + d2 * t.bz + d0 * t.cz) / area2;
+            if (n < max_hits) {
+                const int k = toggle_index(zh, mnz, dz, h);
+                kbuf[n][lane] = (Key)(SIGNED ? ((k << 1) | (pos ? 1 : 0)) : k);
+                total += pos ? 1 : -1;
+                ++n;
+            } else {
+                bad = true;  // the later entries of this batch are skipped, as the list walk ends here
+            }
         }
     }
 
@@ -158,11 +177,11 @@ __global__ __launch_bounds__(kParityBlock) void parity_kernel(
             kbuf[j + 1][lane] = kbuf[j][lane];
             --j;
         }
-        kbuf[j + 1][lane] = key;
+        kbuf[j + 1][lane] = (Key)key;
     }
     const int shift = SIGNED ? 1 : 0;
     int p = 0;
-    int next = (n > 0) ? (kbuf[0][lane] >> shift) : dz;
+    int next = (n > 0) ? ((int)kbuf[0][lane] >> shift) : dz;
     uint8_t inside = 0;
     int below = 0;  // signed crossings below the voxel centre
     for (int z = 0; z < dz; ++z) {
@@ -172,10 +191,18 @@ __global__ __launch_bounds__(kParityBlock) void parity_kernel(
             else
                 inside ^= 1;
             ++p;
-            next = (p < n) ? (kbuf[p][lane] >> shift) : dz;
+            next = (p < n) ? ((int)kbuf[p][lane] >> shift) : dz;
         }
         out[(int64_t)z * zs] = SIGNED ? (uint8_t)(below != 0) : inside;
     }
+}
+
+template <bool SIGNED, typename Key>
+void launch_parity_t(unsigned blocks, const float* tris, const Grid& g, int sup, int sdx, const CsrBuffers& b,
+                     float jx, float jy, int max_hits, uint8_t* sign, int* flag, hipStream_t s) {
+    parity_kernel<SIGNED, Key><<<blocks, kParityBlock, 0, s>>>(tris, b.xyab, b.start, b.ids, g.mn[0], g.mn[1],
+                                                               g.mn[2], g.dims[0], g.dims[1], g.dims[2], g.h, sup,
+                                                               sdx, jx, jy, max_hits, sign, flag);
 }
 
 }  // namespace
@@ -212,14 +239,18 @@ hipError_t launch_parity(const float* tris, const Grid& g, int sup, const CsrBuf
     int sdx = (g.dims[0] + sup - 1) / sup;
     int64_t ncols = (int64_t)g.dims[0] * g.dims[1];
     const unsigned blocks = (unsigned)((ncols + kParityBlock - 1) / kParityBlock);
-    if (signed_count)
-        parity_kernel<true><<<blocks, kParityBlock, 0, s>>>(tris, b.xyab, b.start, b.ids, g.mn[0], g.mn[1], g.mn[2],
-                                                           g.dims[0], g.dims[1], g.dims[2], g.h, sup, sdx, jx, jy,
-                                                           max_hits, sign, flag);
-    else
-        parity_kernel<false><<<blocks, kParityBlock, 0, s>>>(tris, b.xyab, b.start, b.ids, g.mn[0], g.mn[1],
-                                                            g.mn[2], g.dims[0], g.dims[1], g.dims[2], g.h, sup, sdx,
-                                                            jx, jy, max_hits, sign, flag);
+    const bool short_keys = g.dims[2] <= kShortKeyMaxDz;
+    if (signed_count) {
+        if (short_keys)
+            launch_parity_t<true, uint16_t>(blocks, tris, g, sup, sdx, b, jx, jy, max_hits, sign, flag, s);
+        else
+            launch_parity_t<true, int>(blocks, tris, g, sup, sdx, b, jx, jy, max_hits, sign, flag, s);
+    } else {
+        if (short_keys)
+            launch_parity_t<false, uint16_t>(blocks, tris, g, sup, sdx, b, jx, jy, max_hits, sign, flag, s);
+        else
+            launch_parity_t<false, int>(blocks, tris, g, sup, sdx, b, jx, jy, max_hits, sign, flag, s);
+    }
     return hipGetLastError();
 }
 
