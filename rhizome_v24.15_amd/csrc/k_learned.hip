@@ -138,7 +138,13 @@ __global__ void tri_zrange_kernel(const float* __restrict__ tris, int64_t nt, fl
 // second-nearest distance is capped at 4 h, ray crossings at 2 h, the rest
 // are nearer), so a triangle whose z extent is farther than 4.004 h from the
 // voxel is skipped before its distance is computed; the features are the same
-// bits as with every triangle of the column list.
+// bits as with every triangle of the column list.  The column list is walked
+// kFeatBatch entries at a time, ids and z extents loaded together (the walk
+// is latency-bound); entries are still taken in list order, which the
+// order-dependent features (the first of tied nearest triangles, the first
+// risky one) need.
+constexpr int kFeatBatch = 8;  // column-list entries whose loads are issued together
+
 __global__ void features_kernel(const float* __restrict__ tris, const float4* __restrict__ xyab,
                                 const float2* __restrict__ zr, const int64_t* __restrict__ start,
                                 const int32_t* __restrict__ ids,
@@ -168,43 +174,56 @@ __global__ void features_kernel(const float* __restrict__ tris, const float4* __
     const float half_len = 0.5f * sqrtf(jx * jx + jy * jy) * h;
     const int64_t s = (int64_t)(y / sup) * sdx + (x / sup);
     const float reach = 4.004f * h;
-    for (int64_t e = start[s]; e < start[s + 1]; ++e) {
-        const int ti = ids[e];
-        const float2 q = zr[ti];
-        if (pz < q.x - reach || pz > q.y + reach) continue;
-        const Tri9 t = load_tri(tris, ti);
-        const float d2 = point_tri_dist2(px, py, pz, t);
-        if (!risky && d2 <= (2.0f * half_len + eps) * (2.0f * half_len + eps)) {
-            const float4 tn = reinterpret_cast<const float4*>(tris + 12 * (int64_t)ti)[2];
-            risky = segment_may_cross(px, py, pz, rx, ry, t, tn.y, tn.z, tn.w, eps, half_len);
+    const int64_t l0 = start[s], l1 = start[s + 1];
+    for (int64_t eb = l0; eb < l1; eb += kFeatBatch) {
+        // every index is clamped into [l0, l1), so the loads need no guard
+        int tid[kFeatBatch];
+#pragma unroll
+        for (int j = 0; j < kFeatBatch; ++j) tid[j] = ids[min(eb + j, l1 - 1)];
+        bool in_reach[kFeatBatch];
+#pragma unroll
+        for (int j = 0; j < kFeatBatch; ++j) {
+            const float2 q = zr[tid[j]];
+            in_reach[j] = (eb + j < l1) && !(pz < q.x - reach || pz > q.y + reach);
         }
-        if (d2 < best) {
-            second = best;
-            best = d2;
-            best_t = ti;
-        } else if (d2 < second) {
-            second = d2;
-        }
-        if (d2 <= h2) ++near_cnt;
-        if (d2 <= 2.25f * h2) {
-            const float4 nn = reinterpret_cast<const float4*>(tris + 12 * (int64_t)ti)[2];
-            sx += nn.y;
-            sy += nn.z;
-            sz += nn.w;
-            ++spread_cnt;
-        }
-        // crossing of the jittered column ray (same tests as the scan)
-        const float4 ab = xyab[ti];
-        if (!(rx < ab.x || rx > ab.z || ry < ab.y || ry > ab.w)) {
-            float e0 = (t.bx - t.ax) * (ry - t.ay) - (t.by - t.ay) * (rx - t.ax);
-            float e1 = (t.cx - t.bx) * (ry - t.by) - (t.cy - t.by) * (rx - t.bx);
-            float e2 = (t.ax - t.cx) * (ry - t.cy) - (t.ay - t.cy) * (rx - t.cx);
-            bool pos = (e0 > 0.0f && e1 > 0.0f && e2 > 0.0f);
-            bool neg = (e0 < 0.0f && e1 < 0.0f && e2 < 0.0f);
-            float area2 = e0 + e1 + e2;
-            if ((pos || neg) && fabsf(area2) >= 1e-12f) {
-                float zh = (e1 * t.az + e2 * t.bz + e0 * t.cz) / area2;
-                zhit = fminf(zhit, fabsf(zh - pz));
+#pragma unroll
+        for (int j = 0; j < kFeatBatch; ++j) {
+            if (!in_reach[j]) continue;
+            const int ti = tid[j];
+            const Tri9 t = load_tri(tris, ti);
+            const float d2 = point_tri_dist2(px, py, pz, t);
+            if (!risky && d2 <= (2.0f * half_len + eps) * (2.0f * half_len + eps)) {
+                const float4 tn = reinterpret_cast<const float4*>(tris + 12 * (int64_t)ti)[2];
+                risky = segment_may_cross(px, py, pz, rx, ry, t, tn.y, tn.z, tn.w, eps, half_len);
+            }
+            if (d2 < best) {
+                second = best;
+                best = d2;
+                best_t = ti;
+            } else if (d2 < second) {
+                second = d2;
+            }
+            if (d2 <= h2) ++near_cnt;
+            if (d2 <= 2.25f * h2) {
+                const float4 nn = reinterpret_cast<const float4*>(tris + 12 * (int64_t)ti)[2];
+                sx += nn.y;
+                sy += nn.z;
+                sz += nn.w;
+                ++spread_cnt;
+            }
+            // crossing of the jittered column ray (same tests as the scan)
+            const float4 ab = xyab[ti];
+            if (!(rx < ab.x || rx > ab.z || ry < ab.y || ry > ab.w)) {
+                float e0 = (t.bx - t.ax) * (ry - t.ay) - (t.by - t.ay) * (rx - t.ax);
+                float e1 = (t.cx - t.bx) * (ry - t.by) - (t.cy - t.by) * (rx - t.bx);
+                float e2 = (t.ax - t.cx) * (ry - t.cy) - (t.ay - t.cy) * (rx - t.cx);
+                bool pos = (e0 > 0.0f && e1 > 0.0f && e2 > 0.0f);
+                bool neg = (e0 < 0.0f && e1 < 0.0f && e2 < 0.0f);
+                float area2 = e0 + e1 + e2;
+                if ((pos || neg) && fabsf(area2) >= 1e-12f) {
+                    float zh = (e1 * t.az + e2 * t.bz + e0 * t.cz) / area2;
+                    zhit = fminf(zhit, fabsf(zh - pz));
+                }
             }
         }
     }
