@@ -128,6 +128,10 @@ struct Workspace {
     unsigned long long* d_cnt = nullptr;  // device counters (ull[16] + int[32])
     unsigned long long* h_cnt = nullptr;  // pinned mirror
     bool init = false;
+    // the sign stage's own workspace and stream while it overlaps the
+    // eikonal rounds (created on first use)
+    Workspace* side = nullptr;
+    hipStream_t side_stream = nullptr;
 };
 
 std::mutex g_ws_mu;
@@ -174,6 +178,26 @@ Lane* bake_lane(int lane) {
         L.ready = true;
     }
     return &L;
+}
+
+// The side workspace of `w`: own counters, mirror, buffers and a stream
+// (default priority, non-blocking).
+Workspace* side_of(Workspace* w) {
+    if (w->side) return w->side;
+    Workspace* sw = new Workspace;
+    if (hipMalloc(&sw->d_cnt, 512) != hipSuccess ||
+        hipHostMalloc((void**)&sw->h_cnt, 512, hipHostMallocDefault) != hipSuccess ||
+        hipStreamCreateWithFlags(&w->side_stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        if (sw->d_cnt) (void)hipFree(sw->d_cnt);
+        if (sw->h_cnt) (void)hipHostFree(sw->h_cnt);
+        delete sw;
+        w->side_stream = nullptr;
+        return nullptr;
+    }
+    sw->init = true;
+    w->side = sw;
+    return sw;
 }
 
 // Returns a device buffer of at least `bytes` for `slot` (grow-only).
@@ -458,8 +482,9 @@ int fsm_rounds_batched(Ctx& c, float* b, int cap, int* rounds_out) {
     return RHZ_OK;
 }
 
-// [D] far-field re-distancing with the reference round policy.
-int run_fsm(Ctx& c, const uint8_t* sign) {
+// [D] far-field re-distancing with the reference round policy: the rounds
+// and the audit-gated extension rounds; *far = voxels still at the sentinel.
+int run_fsm_rounds(Ctx& c, int64_t* far_out) {
     const int64_t V = c.g.n();
     float* b = (float*)ws_get(c.w, S_DIST2, sizeof(float) * (size_t)V);
     if (!b) return set_err(RHZ_ERR_NOMEM, "second sweep buffer allocation failed");
@@ -494,15 +519,29 @@ int run_fsm(Ctx& c, const uint8_t* sign) {
     }
     c.st->eikonal_rounds = rounds + extra;
     c.st->extra_rounds = extra;
+    *far_out = far;
+    return RHZ_OK;
+}
+
+// [D3] safe clamp of what the rounds left at the sentinel (needs the signs).
+int run_fsm_clamp(Ctx& c, const uint8_t* sign, int64_t far) {
+    const int64_t V = c.g.n();
     c.st->safe_clamped = 0;
     if (far > 0) {
         c.st->safe_clamped = far;
         HIP_TRY(launch_clamp_far(c.d, sign, V, (float)c.p->band_voxels * c.g.h, c.p->clamp_intended_sign, c.s));
-        rc = audit(c, &far);
+        const int rc = audit(c, &far);
         if (rc) return rc;
     }
     c.st->far_residual = far;
     return RHZ_OK;
+}
+
+int run_fsm(Ctx& c, const uint8_t* sign) {
+    int64_t far = 0;
+    const int rc = run_fsm_rounds(c, &far);
+    if (rc) return rc;
+    return run_fsm_clamp(c, sign, far);
 }
 
 bool int_range_ok(const Grid& g, int64_t nt) {
@@ -937,8 +976,9 @@ float candidate_radius(const rhz_sdf_params* p) {
 // Candidates + features + certificate flags (after splat, signed scan and
 // fixup).  On return *cand / *feats / *risk are workspace buffers holding
 // *n_cand rows; *n_risky counts the flagged ones.
-int learned_candidates(Ctx& c, uint8_t* sign, const CsrBuffers& b, int64_t** cand, float** feats, uint8_t** risk,
-                       int64_t* n_cand, int64_t* n_risky) {
+// Compacts the candidate voxels (band distance within the candidate radius;
+// the last step that reads the band, so it runs before the eikonal rounds).
+int learned_candidate_list(Ctx& c, int64_t** cand, int64_t* n_cand) {
     const int64_t V = c.g.n();
     const float thr = candidate_radius(c.p) * c.g.h;
     int64_t cap = (int64_t)(c.w->bytes[S_CAND] / sizeof(int64_t));
@@ -952,25 +992,31 @@ int learned_candidates(Ctx& c, uint8_t* sign, const CsrBuffers& b, int64_t** can
         if (rc) return rc;
         const int64_t n = (int64_t)hc(c, C_LIST);
         if (n <= cap) {
-            float* f = (float*)ws_get(c.w, S_FEAT, sizeof(float) * 12 * (size_t)std::max<int64_t>(n, 1));
-            uint8_t* r = (uint8_t*)ws_get(c.w, S_RISK, (size_t)std::max<int64_t>(n, 1));
-            float2* zr = (float2*)ws_get(c.w, S_ZRANGE, sizeof(float2) * (size_t)std::max<int64_t>(c.nt, 1));
-            if (!f || !r || !zr) return set_err(RHZ_ERR_NOMEM, "feature buffer allocation failed");
-            HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_RISKY), 0, sizeof(unsigned long long), c.s));
-            HIP_TRY(launch_features(c.tris, c.nt, b, zr, sign, list, n, c.g, c.p->sup, c.p->jitter_x, c.p->jitter_y, f,
-                                    r, cnt_ptr(c.w, C_RISKY), c.s));
-            rc = read_counters(c);
-            if (rc) return rc;
             *cand = list;
-            *feats = f;
-            *risk = r;
             *n_cand = n;
-            *n_risky = (int64_t)hc(c, C_RISKY);
             return RHZ_OK;
         }
         cap = n;
     }
     return set_err(RHZ_ERR_HIP, "candidate compaction did not converge");
+}
+
+// Features of the candidates (and their certificate flags).
+int learned_features(Ctx& c, uint8_t* sign, const CsrBuffers& b, const int64_t* list, int64_t n, float** feats,
+                     uint8_t** risk, int64_t* n_risky) {
+    float* f = (float*)ws_get(c.w, S_FEAT, sizeof(float) * 12 * (size_t)std::max<int64_t>(n, 1));
+    uint8_t* r = (uint8_t*)ws_get(c.w, S_RISK, (size_t)std::max<int64_t>(n, 1));
+    float2* zr = (float2*)ws_get(c.w, S_ZRANGE, sizeof(float2) * (size_t)std::max<int64_t>(c.nt, 1));
+    if (!f || !r || !zr) return set_err(RHZ_ERR_NOMEM, "feature buffer allocation failed");
+    HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_RISKY), 0, sizeof(unsigned long long), c.s));
+    HIP_TRY(launch_features(c.tris, c.nt, b, zr, sign, list, n, c.g, c.p->sup, c.p->jitter_x, c.p->jitter_y, f, r,
+                            cnt_ptr(c.w, C_RISKY), c.s));
+    const int rc = read_counters(c);
+    if (rc) return rc;
+    *feats = f;
+    *risk = r;
+    *n_risky = (int64_t)hc(c, C_RISKY);
+    return RHZ_OK;
 }
 
 // Ranks the candidates and returns the top ones (at least `k`, whole score
@@ -1059,45 +1105,19 @@ int learned_select(Ctx& c, const int64_t* cand, const float* feats, int64_t n, i
 // Fast path with the learned sign gate: signed scanline signs, exact
 // re-probes of the top-ranked candidates, growth around every mismatch,
 // then the reference far-field policy.
-int run_learned(Ctx& c) {
+// The learned sign stage after the candidate list: features, ranking, the
+// exact probes of the ranked and the certificate-flagged candidates and their
+// commits into `sign`.  Reads the triangles, the CSR, `sign` and the
+// candidates, never the distance field, so it may overlap the eikonal rounds.
+int learned_sign_stage(Ctx& c, uint8_t* sign, const CsrBuffers& b, unsigned* bits, const int64_t* cand,
+                       int64_t n_cand, Timer& tm, int mark_sel, int mark_end) {
     const rhz_sdf_params* p = c.p;
     rhz_sdf_stats* st = c.st;
     const int64_t V = c.g.n();
-    if (!int_range_ok(c.g, c.nt) || c.nt <= 0) {
-        st->decline_reason = 1;
-        return RHZ_DECLINED;
-    }
-    uint8_t* sign = (uint8_t*)ws_get(c.w, S_SIGN, (size_t)V);
-    unsigned* bits = (unsigned*)ws_get(c.w, S_BITS, sizeof(unsigned) * (size_t)((V + 31) / 32));
-    if (!sign || !bits) {
-        st->decline_reason = 4;
-        return RHZ_DECLINED;
-    }
-    Timer tm(true);
-    tm.mark(0, c.s);
-    int rc = zero_counters(c);
-    if (rc) return rc;
-    HIP_TRY(launch_fill(c.d, V, kFar, c.s));
-    HIP_TRY(launch_splat(c.tris, c.nt, c.g, (float)p->band_voxels * c.g.h, c.d, c.s));
-    tm.mark(1, c.s);
-    int unknown = 0;
-    CsrBuffers b;
-    rc = run_parity(c, sign, &unknown, true, &b);
-    if (rc) return rc;
-    tm.mark(2, c.s);
-    st->unknown_flag = unknown;
-    if (unknown) {
-        int64_t fixed = 0;
-        rc = fixup_unknown(c, sign, &fixed);
-        if (rc) return rc;
-        st->fixup_voxels = fixed;
-    }
-    tm.mark(3, c.s);
-    int64_t* cand = nullptr;
     float* feats = nullptr;
     uint8_t* risk = nullptr;
-    int64_t n_cand = 0, n_risky = 0;
-    rc = learned_candidates(c, sign, b, &cand, &feats, &risk, &n_cand, &n_risky);
+    int64_t n_risky = 0;
+    int rc = learned_features(c, sign, b, cand, n_cand, &feats, &risk, &n_risky);
     if (rc) return rc;
     st->candidates = n_cand;
     st->risky_candidates = n_risky;
@@ -1108,7 +1128,7 @@ int run_learned(Ctx& c) {
     int64_t n_sel = 0;
     rc = learned_select(c, cand, feats, n_cand, k, &sel, &n_sel);
     if (rc) return rc;
-    tm.mark(4, c.s);
+    tm.mark(mark_sel, c.s);
     HIP_TRY(hipMemsetAsync(bits, 0, sizeof(unsigned) * (size_t)((V + 31) / 32), c.s));
     HIP_TRY(launch_mark_probed(sel, n_sel, bits, c.s));
     int64_t mism_cap = std::max<int64_t>(n_sel, 1 << 16);
@@ -1207,10 +1227,134 @@ int run_learned(Ctx& c) {
     }
     st->sites = (int32_t)std::min<int64_t>(st->mismatches_found, INT32_MAX);
     st->probes = 1;
-    tm.mark(5, c.s);
-    rc = run_fsm(c, sign);
+    tm.mark(mark_end, c.s);
+    return RHZ_OK;
+}
+
+// Lattices from this size on run the sign stage on the side stream while the
+// eikonal rounds run (tuning key "verify_overlap").
+constexpr int64_t kOverlapMinVoxels = 1ll << 24;
+int g_verify_overlap = 1;
+
+int run_learned(Ctx& c) {
+    const rhz_sdf_params* p = c.p;
+    rhz_sdf_stats* st = c.st;
+    const int64_t V = c.g.n();
+    if (!int_range_ok(c.g, c.nt) || c.nt <= 0) {
+        st->decline_reason = 1;
+        return RHZ_DECLINED;
+    }
+    uint8_t* sign = (uint8_t*)ws_get(c.w, S_SIGN, (size_t)V);
+    unsigned* bits = (unsigned*)ws_get(c.w, S_BITS, sizeof(unsigned) * (size_t)((V + 31) / 32));
+    if (!sign || !bits) {
+        st->decline_reason = 4;
+        return RHZ_DECLINED;
+    }
+    Timer tm(true);
+    tm.mark(0, c.s);
+    int rc = zero_counters(c);
     if (rc) return rc;
-    tm.mark(6, c.s);
+    HIP_TRY(launch_fill(c.d, V, kFar, c.s));
+    HIP_TRY(launch_splat(c.tris, c.nt, c.g, (float)p->band_voxels * c.g.h, c.d, c.s));
+    tm.mark(1, c.s);
+    int unknown = 0;
+    CsrBuffers b;
+    rc = run_parity(c, sign, &unknown, true, &b);
+    if (rc) return rc;
+    tm.mark(2, c.s);
+    st->unknown_flag = unknown;
+    if (unknown) {
+        int64_t fixed = 0;
+        rc = fixup_unknown(c, sign, &fixed);
+        if (rc) return rc;
+        st->fixup_voxels = fixed;
+    }
+    tm.mark(3, c.s);
+    int64_t* cand = nullptr;
+    int64_t n_cand = 0;
+    rc = learned_candidate_list(c, &cand, &n_cand);
+    if (rc) return rc;
+    Workspace* sw = (g_verify_overlap && V >= kOverlapMinVoxels) ? side_of(c.w) : nullptr;
+    if (!sw) {
+        rc = learned_sign_stage(c, sign, b, bits, cand, n_cand, tm, 4, 5);
+        if (rc) return rc;
+        rc = run_fsm(c, sign);
+        if (rc) return rc;
+        tm.mark(6, c.s);
+    } else {
+        // fork: the side stream starts where the main stream is (candidates
+        // compacted, signs scanned); the main stream runs the eikonal rounds;
+        // join before the clamp and the apply, which read the signs
+        tm.mark(4, c.s);
+        Ctx cv = c;
+        cv.w = sw;
+        cv.s = c.w->side_stream;
+        cv.sweeps_t = nullptr;
+        cv.trans_t = nullptr;
+        hipEvent_t fork = nullptr, join = nullptr;
+        HIP_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        if (hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess) {
+            (void)hipEventDestroy(fork);
+            return set_err(RHZ_ERR_HIP, "event creation failed");
+        }
+        Timer ts(true);
+        int rc_side = RHZ_OK;
+        std::string side_err;
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipEventRecord(fork, c.s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(cv.s, fork, 0);
+        if (e == hipSuccess) e = hipMemsetAsync(sw->d_cnt, 0, 512, cv.s);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(fork);
+            (void)hipEventDestroy(join);
+            return set_err(RHZ_ERR_HIP, "sign-stage fork failed: %s", hipGetErrorString(e));
+        }
+        auto side_work = [&]() {
+            if (hipSetDevice(dev) != hipSuccess) {
+                rc_side = set_err(RHZ_ERR_HIP, "sign stage: hipSetDevice failed");
+            } else {
+                ts.mark(0, cv.s);
+                rc_side = learned_sign_stage(cv, sign, b, bits, cand, n_cand, ts, 1, 2);
+            }
+            if (rc_side) side_err = rhz_last_error();
+            if (hipEventRecord(join, cv.s) != hipSuccess && rc_side == RHZ_OK) {
+                rc_side = set_err(RHZ_ERR_HIP, "sign stage: event record failed");
+                side_err = rhz_last_error();
+            }
+        };
+        std::thread side;
+        try {
+            side = std::thread(side_work);
+        } catch (...) {
+            side_work();  // no thread: the stage runs first, still on its own stream
+        }
+        int64_t far = 0;
+        rc = run_fsm_rounds(c, &far);
+        if (side.joinable()) side.join();
+        tm.mark(5, c.s);
+        e = hipStreamWaitEvent(c.s, join, 0);
+        (void)hipEventDestroy(fork);
+        (void)hipEventDestroy(join);
+        if (rc_side) return set_err(rc_side, "%s", side_err.c_str());
+        if (rc) return rc;
+        if (e != hipSuccess) return set_err(RHZ_ERR_HIP, "sign-stage join failed: %s", hipGetErrorString(e));
+        rc = run_fsm_clamp(c, sign, far);
+        if (rc) return rc;
+        tm.mark(6, c.s);
+        HIP_TRY(launch_apply(c.d, sign, V, c.s));
+        tm.mark(7, c.s);
+        HIP_TRY(hipStreamSynchronize(c.s));
+        st->ms_splat = tm.ms(0, 1);
+        st->ms_parity = tm.ms(1, 2);
+        st->ms_fixup = tm.ms(2, 3);
+        st->ms_proposer = tm.ms(3, 4) + ts.ms(0, 1);  // candidate list + features and ranking (side)
+        st->ms_verify = ts.ms(1, 2);                   // probes and commits (side, overlapping)
+        st->ms_eikonal = tm.ms(4, 5);
+        st->ms_apply = tm.ms(6, 7);
+        st->ms_total = tm.ms(0, 7);
+        return RHZ_OK;
+    }
     HIP_TRY(launch_apply(c.d, sign, V, c.s));
     tm.mark(7, c.s);
     HIP_TRY(hipStreamSynchronize(c.s));
@@ -1388,6 +1532,11 @@ int rhz_tuning_set(const char* key, int64_t value) {
         set_fsm_batching(t.batch_rounds_max_voxels, (int)value);
         return RHZ_OK;
     }
+    if (!std::strcmp(key, "verify_overlap")) {
+        if (value != 0 && value != 1) return set_err(RHZ_ERR_ARG, "verify_overlap must be 0 or 1");
+        g_verify_overlap = (int)value;
+        return RHZ_OK;
+    }
     static const char* kShapeKeys[3] = {"sweep_shape_x", "sweep_shape_y", "sweep_shape_z"};
     for (int a = 0; a < 3; ++a)
         if (!std::strcmp(key, kShapeKeys[a])) {
@@ -1427,6 +1576,10 @@ int rhz_tuning_get(const char* key, int64_t* value) {
         *value = t.fsm_graph;
         return RHZ_OK;
     }
+    if (!std::strcmp(key, "verify_overlap")) {
+        *value = g_verify_overlap;
+        return RHZ_OK;
+    }
     return set_err(RHZ_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -1444,10 +1597,13 @@ void rhz_release_workspace(void) {
     std::lock_guard<std::mutex> lk(g_ws_mu);
     Workspace* w = workspace();
     if (!w) return;
-    for (int i = 0; i < S_COUNT; ++i) {
-        if (w->p[i]) (void)hipFree(w->p[i]);
-        w->p[i] = nullptr;
-        w->bytes[i] = 0;
+    for (Workspace* x : {w, w->side}) {
+        if (!x) continue;
+        for (int i = 0; i < S_COUNT; ++i) {
+            if (x->p[i]) (void)hipFree(x->p[i]);
+            x->p[i] = nullptr;
+            x->bytes[i] = 0;
+        }
     }
 }
 
@@ -2123,7 +2279,9 @@ int rhz_learned_candidates(const rhz_triangle* d_tris, int64_t n_tris, float h, 
     float* feats = nullptr;
     uint8_t* risk = nullptr;
     int64_t n = 0, n_risky = 0;
-    rc = learned_candidates(c, sign, b, &cand, &feats, &risk, &n, &n_risky);
+    rc = learned_candidate_list(c, &cand, &n);
+    if (rc) return rc;
+    rc = learned_features(c, sign, b, cand, n, &feats, &risk, &n_risky);
     if (rc) return rc;
     const int64_t k = std::min(n, cap);
     if (k > 0 && d_voxels) HIP_TRY(hipMemcpyAsync(d_voxels, cand, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToDevice, c.s));

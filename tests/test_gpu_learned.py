@@ -356,3 +356,41 @@ def test_candidate_features_against_fp64(rhz, orc, torch_gpu, config1):
         assert feat[k, 8] == pytest.approx(np.log1p(np.sum(d2 <= h * h)), rel=1e-6)
         checked += 1
     assert checked > 100
+
+
+def test_sign_stage_overlap_matches_sequential(rhz, geom, torch_gpu):
+    """On a lattice of >= 16M voxels the sign stage (features, ranking, exact
+    probes, commits) runs on a side stream while the eikonal rounds run; the
+    field and every count equal the sequential order bit for bit."""
+    torch = torch_gpu
+    v, f = geom.vessel()
+    tris = rhz.build_triangles(v, f)
+    h = 0.25
+    mn, mx = geom.frame_grid(v, h)
+    V = int(np.prod(dims_of(mn, mx).astype(np.int64)))
+    assert V >= 1 << 24
+    d_t = torch.from_numpy(tris).cuda()
+    saved = rhz.tuning_get("verify_overlap")
+    outs, stats = [], []
+    try:
+        for ov in (0, 1, 1, 0):
+            rhz.tuning_set("verify_overlap", ov)
+            out = torch.empty(V, dtype=torch.float32, device="cuda")
+            rc, st = rhz.compute_sdf_device(d_t, len(tris), h, mn, mx, out,
+                                            rhz.default_params(method=rhz.METHOD_FAST_LEARNED))
+            torch.cuda.synchronize()
+            assert rc == rhz.RHZ_OK
+            outs.append(out)
+            stats.append(st)
+    finally:
+        rhz.tuning_set("verify_overlap", saved)
+    keys = ("candidates", "risky_candidates", "ranked_probes", "ranked_mismatches", "fallback_probes",
+            "fallback_mismatches", "mismatches_found", "reprobed_voxels", "eikonal_rounds", "sweeps",
+            "far_residual", "fixup_voxels")
+    for o, st in zip(outs[1:], stats[1:]):
+        assert torch.equal(o.view(torch.int32), outs[0].view(torch.int32))
+        for k in keys:
+            assert getattr(st, k) == getattr(stats[0], k), k
+    print(stats[1].log_line(), "| overlapped: eikonal", round(stats[1].ms_eikonal, 2), "ms, sign stage",
+          round(stats[1].ms_proposer + stats[1].ms_verify, 2), "ms, total", round(stats[1].ms_total, 2),
+          "| sequential total", round(stats[0].ms_total, 2))
