@@ -145,7 +145,7 @@ typedef struct rhz_sdf_stats {
     int64_t winding_pairs;        /* (voxel, triangle) pairs those kernels evaluated */
     int64_t candidates;           /* FAST_LEARNED: candidate voxels ranked */
     int64_t mismatches_found;     /* FAST_LEARNED: scanline signs the exact kernel corrected */
-    double ms_proposer;           /* FAST_LEARNED: features + network + selection */
+    double ms_proposer;           /* FAST_LEARNED: candidates + features + network + selection */
     int64_t winding_calls;        /* bracketed exact-winding launches behind ms_winding */
     int64_t sweep_calls;          /* sweep launches behind ms_sweeps (each timed on its own) */
     int64_t ranked_probes;        /* FAST_LEARNED: exact probes chosen by the scorer */
@@ -181,7 +181,13 @@ int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
  *                                  the one-line walker;
  *   "fsm_batch_rounds_max_voxels"  lattices up to this size enqueue all FSM
  *                                  rounds without host round trips;
- *   "fsm_graph"                    1: those rounds replay as one captured graph. */
+ *   "fsm_graph"                    1: those rounds replay as one captured graph;
+ *   "verify_overlap"               1 (default): on lattices of >= 16M voxels the
+ *                                  FAST_LEARNED sign stage (features, ranking,
+ *                                  exact probes) runs on a second stream and
+ *                                  host thread while the eikonal rounds run;
+ *                                  ms_proposer / ms_verify then overlap
+ *                                  ms_eikonal, and ms_total is their union. */
 int rhz_tuning_set(const char* key, int64_t value);
 int rhz_tuning_get(const char* key, int64_t* value);
 /* Frees the cached device workspace of the current device. */
