@@ -1236,6 +1236,22 @@ int learned_sign_stage(Ctx& c, uint8_t* sign, const CsrBuffers& b, unsigned* bit
 constexpr int64_t kOverlapMinVoxels = 1ll << 24;
 int g_verify_overlap = 1;
 
+// Parity signs over the candidate-list CSR, then the exact fixup of the
+// deferred columns.
+int learned_scan(Ctx& c, uint8_t* sign, CsrBuffers* b) {
+    int unknown = 0;
+    int rc = run_parity(c, sign, &unknown, true, b);
+    if (rc) return rc;
+    c.st->unknown_flag = unknown;
+    if (unknown) {
+        int64_t fixed = 0;
+        rc = fixup_unknown(c, sign, &fixed);
+        if (rc) return rc;
+        c.st->fixup_voxels = fixed;
+    }
+    return RHZ_OK;
+}
+
 int run_learned(Ctx& c) {
     const rhz_sdf_params* p = c.p;
     rhz_sdf_stats* st = c.st;
@@ -1257,89 +1273,28 @@ int run_learned(Ctx& c) {
     HIP_TRY(launch_fill(c.d, V, kFar, c.s));
     HIP_TRY(launch_splat(c.tris, c.nt, c.g, (float)p->band_voxels * c.g.h, c.d, c.s));
     tm.mark(1, c.s);
-    int unknown = 0;
-    CsrBuffers b;
-    rc = run_parity(c, sign, &unknown, true, &b);
-    if (rc) return rc;
-    tm.mark(2, c.s);
-    st->unknown_flag = unknown;
-    if (unknown) {
-        int64_t fixed = 0;
-        rc = fixup_unknown(c, sign, &fixed);
-        if (rc) return rc;
-        st->fixup_voxels = fixed;
-    }
-    tm.mark(3, c.s);
     int64_t* cand = nullptr;
     int64_t n_cand = 0;
-    rc = learned_candidate_list(c, &cand, &n_cand);
-    if (rc) return rc;
+    CsrBuffers b;
     Workspace* sw = (g_verify_overlap && V >= kOverlapMinVoxels) ? side_of(c.w) : nullptr;
     if (!sw) {
+        int unknown = 0;
+        rc = run_parity(c, sign, &unknown, true, &b);
+        if (rc) return rc;
+        tm.mark(2, c.s);
+        st->unknown_flag = unknown;
+        if (unknown) {
+            int64_t fixed = 0;
+            rc = fixup_unknown(c, sign, &fixed);
+            if (rc) return rc;
+            st->fixup_voxels = fixed;
+        }
+        tm.mark(3, c.s);
+        rc = learned_candidate_list(c, &cand, &n_cand);
+        if (rc) return rc;
         rc = learned_sign_stage(c, sign, b, bits, cand, n_cand, tm, 4, 5);
         if (rc) return rc;
         rc = run_fsm(c, sign);
-        if (rc) return rc;
-        tm.mark(6, c.s);
-    } else {
-        // fork: the side stream starts where the main stream is (candidates
-        // compacted, signs scanned); the main stream runs the eikonal rounds;
-        // join before the clamp and the apply, which read the signs
-        tm.mark(4, c.s);
-        Ctx cv = c;
-        cv.w = sw;
-        cv.s = c.w->side_stream;
-        cv.sweeps_t = nullptr;
-        cv.trans_t = nullptr;
-        hipEvent_t fork = nullptr, join = nullptr;
-        HIP_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-        if (hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess) {
-            (void)hipEventDestroy(fork);
-            return set_err(RHZ_ERR_HIP, "event creation failed");
-        }
-        Timer ts(true);
-        int rc_side = RHZ_OK;
-        std::string side_err;
-        int dev = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipEventRecord(fork, c.s);
-        if (e == hipSuccess) e = hipStreamWaitEvent(cv.s, fork, 0);
-        if (e == hipSuccess) e = hipMemsetAsync(sw->d_cnt, 0, 512, cv.s);
-        if (e != hipSuccess) {
-            (void)hipEventDestroy(fork);
-            (void)hipEventDestroy(join);
-            return set_err(RHZ_ERR_HIP, "sign-stage fork failed: %s", hipGetErrorString(e));
-        }
-        auto side_work = [&]() {
-            if (hipSetDevice(dev) != hipSuccess) {
-                rc_side = set_err(RHZ_ERR_HIP, "sign stage: hipSetDevice failed");
-            } else {
-                ts.mark(0, cv.s);
-                rc_side = learned_sign_stage(cv, sign, b, bits, cand, n_cand, ts, 1, 2);
-            }
-            if (rc_side) side_err = rhz_last_error();
-            if (hipEventRecord(join, cv.s) != hipSuccess && rc_side == RHZ_OK) {
-                rc_side = set_err(RHZ_ERR_HIP, "sign stage: event record failed");
-                side_err = rhz_last_error();
-            }
-        };
-        std::thread side;
-        try {
-            side = std::thread(side_work);
-        } catch (...) {
-            side_work();  // no thread: the stage runs first, still on its own stream
-        }
-        int64_t far = 0;
-        rc = run_fsm_rounds(c, &far);
-        if (side.joinable()) side.join();
-        tm.mark(5, c.s);
-        e = hipStreamWaitEvent(c.s, join, 0);
-        (void)hipEventDestroy(fork);
-        (void)hipEventDestroy(join);
-        if (rc_side) return set_err(rc_side, "%s", side_err.c_str());
-        if (rc) return rc;
-        if (e != hipSuccess) return set_err(RHZ_ERR_HIP, "sign-stage join failed: %s", hipGetErrorString(e));
-        rc = run_fsm_clamp(c, sign, far);
         if (rc) return rc;
         tm.mark(6, c.s);
         HIP_TRY(launch_apply(c.d, sign, V, c.s));
@@ -1348,24 +1303,91 @@ int run_learned(Ctx& c) {
         st->ms_splat = tm.ms(0, 1);
         st->ms_parity = tm.ms(1, 2);
         st->ms_fixup = tm.ms(2, 3);
-        st->ms_proposer = tm.ms(3, 4) + ts.ms(0, 1);  // candidate list + features and ranking (side)
-        st->ms_verify = ts.ms(1, 2);                   // probes and commits (side, overlapping)
-        st->ms_eikonal = tm.ms(4, 5);
+        st->ms_proposer = tm.ms(3, 4);
+        st->ms_verify = tm.ms(4, 5);
+        st->ms_eikonal = tm.ms(5, 6);
         st->ms_apply = tm.ms(6, 7);
         st->ms_total = tm.ms(0, 7);
         return RHZ_OK;
     }
+    // Overlapped: the candidate compaction is the last step that reads the
+    // band, so the main stream runs it and then the eikonal rounds; from the
+    // fork on the side stream runs the scan, the fixup and the sign stage,
+    // which write the signs but never touch the field; both join before the
+    // clamp and the apply, which read the signs.
+    rc = learned_candidate_list(c, &cand, &n_cand);
+    if (rc) return rc;
+    tm.mark(2, c.s);
+    Ctx cv = c;
+    cv.w = sw;
+    cv.s = c.w->side_stream;
+    cv.sweeps_t = nullptr;
+    cv.trans_t = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    if (hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess) {
+        (void)hipEventDestroy(fork);
+        return set_err(RHZ_ERR_HIP, "event creation failed");
+    }
+    Timer ts(true);
+    int rc_side = RHZ_OK;
+    std::string side_err;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipEventRecord(fork, c.s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(cv.s, fork, 0);
+    if (e == hipSuccess) e = hipMemsetAsync(sw->d_cnt, 0, 512, cv.s);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(fork);
+        (void)hipEventDestroy(join);
+        return set_err(RHZ_ERR_HIP, "sign-stage fork failed: %s", hipGetErrorString(e));
+    }
+    auto side_work = [&]() {
+        if (hipSetDevice(dev) != hipSuccess) {
+            rc_side = set_err(RHZ_ERR_HIP, "sign stage: hipSetDevice failed");
+        } else {
+            ts.mark(0, cv.s);
+            rc_side = learned_scan(cv, sign, &b);
+            ts.mark(1, cv.s);
+            if (rc_side == RHZ_OK) rc_side = learned_sign_stage(cv, sign, b, bits, cand, n_cand, ts, 2, 3);
+        }
+        if (rc_side) side_err = rhz_last_error();
+        if (hipEventRecord(join, cv.s) != hipSuccess && rc_side == RHZ_OK) {
+            rc_side = set_err(RHZ_ERR_HIP, "sign stage: event record failed");
+            side_err = rhz_last_error();
+        }
+    };
+    std::thread side;
+    try {
+        side = std::thread(side_work);
+    } catch (...) {
+        side_work();  // no thread: the stage runs first, still on its own stream
+    }
+    int64_t far = 0;
+    rc = run_fsm_rounds(c, &far);
+    if (side.joinable()) side.join();
+    tm.mark(3, c.s);
+    e = hipStreamWaitEvent(c.s, join, 0);
+    (void)hipEventDestroy(fork);
+    (void)hipEventDestroy(join);
+    if (rc_side < 0) return set_err(rc_side, "%s", side_err.c_str());
+    if (rc_side > 0) return rc_side;  // declined (decline_reason says why); the caller falls back
+    if (rc) return rc;
+    if (e != hipSuccess) return set_err(RHZ_ERR_HIP, "sign-stage join failed: %s", hipGetErrorString(e));
+    rc = run_fsm_clamp(c, sign, far);
+    if (rc) return rc;
+    tm.mark(4, c.s);
     HIP_TRY(launch_apply(c.d, sign, V, c.s));
-    tm.mark(7, c.s);
+    tm.mark(5, c.s);
     HIP_TRY(hipStreamSynchronize(c.s));
     st->ms_splat = tm.ms(0, 1);
-    st->ms_parity = tm.ms(1, 2);
-    st->ms_fixup = tm.ms(2, 3);
-    st->ms_proposer = tm.ms(3, 4);
-    st->ms_verify = tm.ms(4, 5);
-    st->ms_eikonal = tm.ms(5, 6);
-    st->ms_apply = tm.ms(6, 7);
-    st->ms_total = tm.ms(0, 7);
+    st->ms_parity = ts.ms(0, 1);                   // scan + fixup (side)
+    st->ms_fixup = 0.0;                            // (inside ms_parity here)
+    st->ms_proposer = tm.ms(1, 2) + ts.ms(1, 2);  // candidate list (main) + features and ranking (side)
+    st->ms_verify = ts.ms(2, 3);                   // exact probes and commits (side)
+    st->ms_eikonal = tm.ms(2, 3);
+    st->ms_apply = tm.ms(4, 5);
+    st->ms_total = tm.ms(0, 5);
     return RHZ_OK;
 }
 

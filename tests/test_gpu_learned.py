@@ -394,3 +394,24 @@ def test_sign_stage_overlap_matches_sequential(rhz, geom, torch_gpu):
     print(stats[1].log_line(), "| overlapped: eikonal", round(stats[1].ms_eikonal, 2), "ms, sign stage",
           round(stats[1].ms_proposer + stats[1].ms_verify, 2), "ms, total", round(stats[1].ms_total, 2),
           "| sequential total", round(stats[0].ms_total, 2))
+
+
+def test_sign_stage_overlap_decline(rhz, geom, torch_gpu):
+    """A decline raised on the side stream (candidate lists over their cap)
+    comes back as RHZ_DECLINED with its reason, and AUTO falls back."""
+    torch = torch_gpu
+    v, f = geom.vessel()
+    tris = rhz.build_triangles(v, f)
+    h = 0.25
+    mn, mx = geom.frame_grid(v, h)
+    V = int(np.prod(dims_of(mn, mx).astype(np.int64)))
+    d_t = torch.from_numpy(tris).cuda()
+    out = torch.empty(V, dtype=torch.float32, device="cuda")
+    saved = rhz.tuning_get("verify_overlap")
+    try:
+        rhz.tuning_set("verify_overlap", 1)
+        rc, st = rhz.compute_sdf_device(d_t, len(tris), h, mn, mx, out,
+                                        rhz.default_params(method=rhz.METHOD_FAST_LEARNED, csr_max_entries=10))
+        assert rc == rhz.RHZ_DECLINED and st.decline_reason == 2
+    finally:
+        rhz.tuning_set("verify_overlap", saved)
