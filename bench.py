@@ -17,6 +17,12 @@ import os
 import sys
 import time
 
+# The batched bake drives several lanes, each with a main and a side stream;
+# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (default 4), read
+# once when the runtime starts, so the value is set before anything touches
+# the GPU.  A value the caller set wins.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
