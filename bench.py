@@ -170,7 +170,10 @@ def batched_bake(rhz, torch, params, n_meshes=64, log=None):
         items.append((torch.from_numpy(tris).cuda(), len(tris), h, mn, mx,
                       torch.empty(V, dtype=torch.float32, device="cuda")))
     best, table = tune.autotune_bake(rhz, torch, items, params, log=log)
-    rhz.bake(items, params, best["n_lanes"])  # warm-up at the chosen shape
+    # warm-up at the chosen shape: lanes take items in any order, so two passes
+    # let every lane's workspace reach the sizes of the batch's largest items
+    for _ in range(2):
+        rhz.bake(items, params, best["n_lanes"])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     rc, stats, ms = rhz.bake(items, params, best["n_lanes"])

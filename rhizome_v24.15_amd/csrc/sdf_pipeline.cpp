@@ -200,15 +200,20 @@ Workspace* side_of(Workspace* w) {
     return sw;
 }
 
-// Returns a device buffer of at least `bytes` for `slot` (grow-only).
+// Returns a device buffer of at least `bytes` for `slot` (grow-only).  A slot
+// that has to grow takes at least 1.5x its old size: a bake lane meets items
+// of many sizes, and every regrowth (a free and an allocation) can stall the
+// whole device.
 void* ws_get(Workspace* w, int slot, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (w->bytes[slot] >= bytes) return w->p[slot];
+    const size_t old = w->bytes[slot];
     if (w->p[slot]) {
         (void)hipFree(w->p[slot]);
         w->p[slot] = nullptr;
         w->bytes[slot] = 0;
     }
+    if (old > 0) bytes = std::max(bytes, old + old / 2);
     size_t rounded = (bytes + 4095) & ~size_t(4095);
     if (hipMalloc(&w->p[slot], rounded) != hipSuccess) {
         (void)hipGetLastError();
