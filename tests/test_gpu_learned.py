@@ -415,3 +415,33 @@ def test_sign_stage_overlap_decline(rhz, geom, torch_gpu):
         assert rc == rhz.RHZ_DECLINED and st.decline_reason == 2
     finally:
         rhz.tuning_set("verify_overlap", saved)
+
+
+def test_sign_stage_overlap_with_safe_clamp(rhz, geom, torch_gpu):
+    """No rounds and no extension rounds leave far voxels at the sentinel, so
+    the safe clamp runs after the join (it reads the signs the side stream
+    wrote): the overlapped build equals the sequential one bit for bit."""
+    torch = torch_gpu
+    v, f = geom.vessel()
+    tris = rhz.build_triangles(v, f)
+    h = 0.25
+    mn, mx = geom.frame_grid(v, h)
+    V = int(np.prod(dims_of(mn, mx).astype(np.int64)))
+    d_t = torch.from_numpy(tris).cuda()
+    saved = rhz.tuning_get("verify_overlap")
+    outs, stats = [], []
+    try:
+        for ov in (0, 1):
+            rhz.tuning_set("verify_overlap", ov)
+            out = torch.empty(V, dtype=torch.float32, device="cuda")
+            p = rhz.default_params(method=rhz.METHOD_FAST_LEARNED, max_rounds=0, max_extra_rounds=0)
+            rc, st = rhz.compute_sdf_device(d_t, len(tris), h, mn, mx, out, p)
+            torch.cuda.synchronize()
+            assert rc == rhz.RHZ_OK
+            outs.append(out)
+            stats.append(st)
+    finally:
+        rhz.tuning_set("verify_overlap", saved)
+    assert stats[1].safe_clamped > 0 and stats[1].safe_clamped == stats[0].safe_clamped
+    assert stats[1].far_residual == stats[0].far_residual
+    assert torch.equal(outs[1].view(torch.int32), outs[0].view(torch.int32))
