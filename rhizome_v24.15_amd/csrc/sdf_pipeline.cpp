@@ -1239,7 +1239,7 @@ int learned_sign_stage(Ctx& c, uint8_t* sign, const CsrBuffers& b, unsigned* bit
 // Lattices from this size on run the sign stage on the side stream while the
 // eikonal rounds run (tuning key "verify_overlap").
 constexpr int64_t kOverlapMinVoxels = 1ll << 24;
-int g_verify_overlap = 1;
+std::atomic<int> g_verify_overlap{1};
 
 // Parity signs over the candidate-list CSR, then the exact fixup of the
 // deferred columns.
@@ -1281,7 +1281,7 @@ int run_learned(Ctx& c) {
     int64_t* cand = nullptr;
     int64_t n_cand = 0;
     CsrBuffers b;
-    Workspace* sw = (g_verify_overlap && V >= kOverlapMinVoxels) ? side_of(c.w) : nullptr;
+    Workspace* sw = (g_verify_overlap.load(std::memory_order_relaxed) && V >= kOverlapMinVoxels) ? side_of(c.w) : nullptr;
     if (!sw) {
         int unknown = 0;
         rc = run_parity(c, sign, &unknown, true, &b);
@@ -1561,7 +1561,7 @@ int rhz_tuning_set(const char* key, int64_t value) {
     }
     if (!std::strcmp(key, "verify_overlap")) {
         if (value != 0 && value != 1) return set_err(RHZ_ERR_ARG, "verify_overlap must be 0 or 1");
-        g_verify_overlap = (int)value;
+        g_verify_overlap.store((int)value, std::memory_order_relaxed);
         return RHZ_OK;
     }
     static const char* kShapeKeys[3] = {"sweep_shape_x", "sweep_shape_y", "sweep_shape_z"};
@@ -1604,7 +1604,7 @@ int rhz_tuning_get(const char* key, int64_t* value) {
         return RHZ_OK;
     }
     if (!std::strcmp(key, "verify_overlap")) {
-        *value = g_verify_overlap;
+        *value = g_verify_overlap.load(std::memory_order_relaxed);
         return RHZ_OK;
     }
     return set_err(RHZ_ERR_ARG, "unknown tuning key '%s'", key);
