@@ -34,6 +34,19 @@ def load_obj(path: str):
     return v, f.astype(np.uint32)
 
 
+def save_obj(path: str, verts, faces) -> None:
+    """Writes ``v x y z`` records (nine significant digits: binary32 values
+    read back exactly) and 1-based ``f a b c`` triangles, the records
+    load_obj reads (the reference's ObjExporter writes the same two kinds)."""
+    v = np.asarray(verts, dtype=np.float32).reshape(-1, 3)
+    f = np.asarray(faces, dtype=np.int64).reshape(-1, 3)
+    with open(path, "w", encoding="utf-8") as fh:
+        for x, y, z in v.tolist():
+            fh.write(f"v {x:.9g} {y:.9g} {z:.9g}\n")
+        for a, b, c in (f + 1).tolist():
+            fh.write(f"f {a} {b} {c}\n")
+
+
 def signed_volume(v, f) -> float:
     v = np.asarray(v, dtype=np.float64)
     a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
