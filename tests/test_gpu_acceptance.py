@@ -266,3 +266,19 @@ def test_reference_test_8_round_trip(rhz, geom, torch_gpu, tmp_path):
           f"vertex max {dv.max():.4f} mm, midpoint max {dm.max():.4f} mm")
     assert np.isfinite(dv).all() and np.isfinite(dm).all()
     assert dv.max() <= 0.100 and dm.max() <= 0.100
+
+
+@pytest.mark.gpu
+def test_reference_test_15_marching_cubes(rhz, torch_gpu):
+    """harness_eikonal_cuda_tests.cpp:360-385: f = |p| - 10 on a 30^3 lattice
+    at 1 mm from (-15, -15, -15) extracts a non-empty surface; here also every
+    vertex lies within 0.02 mm of the sphere (linear interpolation of the exact
+    field along an edge errs by about h^2 / (8 R) = 0.0125 mm)."""
+    n, h, R = 30, 1.0, 10.0
+    ax = (np.arange(n, dtype=np.float64) - 15.0) * h
+    Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")
+    grid = (np.sqrt(X * X + Y * Y + Z * Z) - R).astype(np.float32)
+    verts, tris = rhz.iso_surface(grid.reshape(-1), [n, n, n], h, [-15.0, -15.0, -15.0], 0.0)
+    assert len(tris) > 0
+    dev = np.abs(np.linalg.norm(verts.astype(np.float64), axis=1) - R)
+    assert dev.max() <= 0.02, dev.max()
