@@ -335,7 +335,11 @@ def main():
                    "avg_launch_ms": ms_wind / max(wcalls, 1), "launches": wcalls,
                    "pairs_per_step": pairs / len(stats),
                    "note": "FP32 vector ALU work priced against MI355X FP32 157.3 TFLOP/s "
-                           f"({WINDING_OPS_PER_PAIR:.0f} FP32 ops per voxel-triangle pair)"}
+                           f"({WINDING_OPS_PER_PAIR:.0f} FP32 ops per voxel-triangle pair)"
+                           + ("; on lattices of >= 16M voxels these launches run on the side stream while the "
+                              "eikonal sweeps run, so they share the GPU (plain_exact.winding_tpairs_per_s is "
+                              "the kernel's rate alone)" if V >= (1 << 24) and rhz.tuning_get("verify_overlap")
+                              else "")}
     dominant = rl_fsm
     if rl_wind and (not rl_fsm or ms_wind > ms_sweeps + ms_trans):
         dominant = rl_wind
