@@ -110,3 +110,104 @@ def test_round_policy_reference_cap(orc, small_sphere):
     assert st.safe_clamped > 0 and st.far_residual == 0
     band = 8 * h
     assert set(np.unique(np.abs(out[np.abs(out) == band]))) <= {np.float32(band)}
+
+
+def _converge_octants(orc, d, dims, h, max_rounds=512):
+    dm = orc._i32(dims)
+    for r in range(max_rounds):
+        ch = 0
+        for sz in (1, -1):
+            for sy in (1, -1):
+                for sx in (1, -1):
+                    ch |= orc.lib().orc_fsm_sweep_octant(d, dm, h, sx, sy, sz)
+        if not ch:
+            return r + 1
+    return max_rounds
+
+
+def test_reference_update_fixed_point_depends_on_sweep_order(orc, small_sphere):
+    """The reference's Godunov update halves the squared differences of its
+    3-D discriminant (cuda_rasterizer.cu:416-418), which makes it
+    non-monotone in the largest neighbour: the 6-axis sweeps and the classical
+    8-ordering sweeps (Zhao 2005) converge to different fixed points.  This is
+    why the sweeps are pinned to one schedule bit for bit rather than to a
+    converged solution."""
+    tris, h, mn, mx = small_sphere
+    dims = orc.dims_of(mn, mx)
+    band = orc.splat_band(tris, h, mn, dims)
+    a = band.copy()
+    b = band.copy()
+    orc.fsm_converge(a, dims, h)
+    _converge_octants(orc, b, dims, h)
+    assert np.all(a < 1e29) and np.all(b < 1e29)
+    diff = np.abs(a.astype(np.float64) - b.astype(np.float64))
+    assert diff.max() > 0.1 * h  # measured: 0.31 voxel, in about a fifth of the cells
+
+
+def _godunov_textbook(a, b, c, h):
+    s0, s1, s2 = sorted((a, b, c))
+    u = s0 + h
+    if u > s1:
+        u = (s0 + s1 + np.sqrt(max(2 * h * h - (s1 - s0) ** 2, 0.0))) / 2
+        if u > s2:
+            disc = 3 * h * h - ((s1 - s0) ** 2 + (s2 - s1) ** 2 + (s2 - s0) ** 2)
+            u = (s0 + s1 + s2 + np.sqrt(max(disc, 0.0))) / 3
+    return u
+
+
+def _textbook_sweep(g, h, order):
+    """In-place Gauss-Seidel over the cells in `order` (float64, textbook
+    Godunov update)."""
+    nz, ny, nx = g.shape
+    big = 1e30
+    changed = False
+    for z, y, x in order:
+        ax = min(g[z, y, x - 1] if x > 0 else big, g[z, y, x + 1] if x < nx - 1 else big)
+        ay = min(g[z, y - 1, x] if y > 0 else big, g[z, y + 1, x] if y < ny - 1 else big)
+        az = min(g[z - 1, y, x] if z > 0 else big, g[z + 1, y, x] if z < nz - 1 else big)
+        if min(ax, ay, az) >= big:
+            continue
+        u = _godunov_textbook(ax, ay, az, h)
+        if u < g[z, y, x]:
+            g[z, y, x] = u
+            changed = True
+    return changed
+
+
+def test_textbook_update_fixed_point_is_order_independent():
+    """With the textbook discriminant 3h^2 - sum of squared differences the
+    update is monotone, and the axis-line sweeps and the 8-ordering sweeps
+    reach the same fixed point (the scheme's unique discrete solution)."""
+    n, h = 12, 1.0
+    rng = np.random.default_rng(4)
+    seed = np.full((n, n, n), 1e30)
+    for _ in range(3):
+        z, y, x = rng.integers(0, n, 3)
+        seed[z, y, x] = rng.uniform(0.0, 1.0)
+    idx = [(z, y, x) for z in range(n) for y in range(n) for x in range(n)]
+
+    def axis_orders():
+        for axis in range(3):
+            for d in (1, -1):
+                # lines along `axis`, each walked in direction d (cells of one
+                # line in order; lines one after another)
+                key = [lambda c: (c[0], c[1], c[2] * d), lambda c: (c[0], c[2], c[1] * d),
+                       lambda c: (c[1], c[2], c[0] * d)][axis]
+                yield sorted(idx, key=key)
+
+    def octant_orders():
+        for sz in (1, -1):
+            for sy in (1, -1):
+                for sx in (1, -1):
+                    yield sorted(idx, key=lambda c: (c[0] * sz, c[1] * sy, c[2] * sx))
+
+    a = seed.copy()
+    for _ in range(50):
+        if not any([_textbook_sweep(a, h, o) for o in axis_orders()]):
+            break
+    b = seed.copy()
+    for _ in range(50):
+        if not any([_textbook_sweep(b, h, o) for o in octant_orders()]):
+            break
+    assert np.all(a < 1e29)
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-12)
