@@ -607,8 +607,9 @@ __global__ __launch_bounds__(64 * ROWS) void sweep_lane_kernel(const float* __re
 constexpr int kTrTiles = 2;
 
 __global__ __launch_bounds__(256) void transpose_planes_kernel(const float* __restrict__ in, float* __restrict__ out,
-                                                               int rows, int cols, int tiles_r, int tiles_c,
-                                                               unsigned nblocks) {
+                                                               int rows, int cols, int64_t in_pitch, int64_t in_plane,
+                                                               int64_t out_pitch, int64_t out_plane, int tiles_r,
+                                                               int tiles_c, unsigned nblocks) {
     __shared__ float tile[2][64][65];
     const int groups_c = (tiles_c + kTrTiles - 1) / kTrTiles;
     const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
@@ -616,7 +617,8 @@ __global__ __launch_bounds__(256) void transpose_planes_kernel(const float* __re
     const int z = (int)(lb / (unsigned)per_plane);
     const int t = (int)(lb % (unsigned)per_plane);
     const int tr = t / groups_c, tg = t % groups_c;
-    const int64_t plane = (int64_t)z * rows * cols;
+    const float* inz = in + (int64_t)z * in_plane;
+    float* outz = out + (int64_t)z * out_plane;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
     const int r0 = tr * 64;
     float v[16];
@@ -625,7 +627,7 @@ __global__ __launch_bounds__(256) void transpose_planes_kernel(const float* __re
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int r = r0 + ty + 4 * k;
-            v[k] = (r < rows && c < cols) ? in[plane + (int64_t)r * cols + c] : 0.0f;
+            v[k] = (r < rows && c < cols) ? inz[(int64_t)r * in_pitch + c] : 0.0f;
         }
     };
     const int tc0 = tg * kTrTiles, tc1 = min(tiles_c, tc0 + kTrTiles);
@@ -640,7 +642,7 @@ __global__ __launch_bounds__(256) void transpose_planes_kernel(const float* __re
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int c = tc * 64 + ty + 4 * k;
-            if (r < rows && c < cols) out[plane + (int64_t)c * rows + r] = tile[buf][tx][ty + 4 * k];
+            if (r < rows && c < cols) outz[(int64_t)c * out_pitch + r] = tile[buf][tx][ty + 4 * k];
         }
     }
 }
@@ -793,29 +795,52 @@ bool lane_sweeps_ok(const Grid& g) {
            g.h <= 0x1.0p30f;
 }
 
-hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s) {
+hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s,
+                               int64_t in_pitch, int64_t out_pitch) {
     const int dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
     const int rows = to_yx ? dy : dx, cols = to_yx ? dx : dy;
+    if (in_pitch <= 0) in_pitch = cols;
+    if (out_pitch <= 0) out_pitch = rows;
+    if (in_pitch < cols || out_pitch < rows) return hipErrorInvalidValue;
     const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
     const int64_t blocks = (int64_t)tr * ((tc + kTrTiles - 1) / kTrTiles) * dz;
-    transpose_planes_kernel<<<(unsigned)blocks, 256, 0, s>>>(in, out, rows, cols, tr, tc, (unsigned)blocks);
+    transpose_planes_kernel<<<(unsigned)blocks, 256, 0, s>>>(in, out, rows, cols, in_pitch, in_pitch * rows, out_pitch,
+                                                            out_pitch * cols, tr, tc, (unsigned)blocks);
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir, int* changed, hipStream_t s) {
-    // y-fastest layout: lanes over y (Godunov slot 1), walk x (slot 0, stride dy), cross z (slot 2)
+hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir, int* changed, hipStream_t s,
+                           int64_t pitch) {
+    // y-fastest layout: lanes over y (Godunov slot 1), walk x (slot 0, stride = row pitch), cross z (slot 2)
     const int dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
-    return launch_lane<1, 0, 2>(src, dst, dy, dx, dz, dy, (int64_t)dx * dy, g.h, dir, changed, s);
+    if (pitch <= 0) pitch = dy;
+    return launch_lane<1, 0, 2>(src, dst, dy, dx, dz, pitch, pitch * dx, g.h, dir, changed, s);
 }
 
 hipError_t launch_sweep(const float* src, float* dst, const Grid& g, int axis, int dir, int* changed,
-                        hipStream_t s) {
+                        hipStream_t s, int64_t pitch) {
     const int dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
-    if (axis == 0 || !lane_sweeps_ok(g)) return launch_generic_sweep(src, dst, g, axis, dir, changed, s);
-    if (axis == 1)  // lanes x (slot 0), walk y (slot 1, stride dx), cross z (slot 2)
-        return launch_lane<0, 1, 2>(src, dst, dx, dy, dz, dx, (int64_t)dx * dy, g.h, dir, changed, s);
-    // axis 2: lanes x (slot 0), walk z (slot 2, stride dx*dy), cross y (slot 1)
-    return launch_lane<0, 2, 1>(src, dst, dx, dz, dy, (int64_t)dx * dy, dx, g.h, dir, changed, s);
+    if (pitch <= 0) pitch = dx;
+    if (axis == 0 || !lane_sweeps_ok(g)) {
+        if (pitch != dx) return hipErrorInvalidValue;  // the generic kernels take dense fields only
+        return launch_generic_sweep(src, dst, g, axis, dir, changed, s);
+    }
+    if (axis == 1)  // lanes x (slot 0), walk y (slot 1, stride = row pitch), cross z (slot 2)
+        return launch_lane<0, 1, 2>(src, dst, dx, dy, dz, pitch, pitch * dy, g.h, dir, changed, s);
+    // axis 2: lanes x (slot 0), walk z (slot 2, stride = plane), cross y (slot 1, stride = row pitch)
+    return launch_lane<0, 2, 1>(src, dst, dx, dz, dy, pitch * dy, pitch, g.h, dir, changed, s);
+}
+
+// Row pitches of the FSM's working layouts: rows rounded up to 32 floats
+// (128 B), so that every 64-cell segment of a row spans two cache lines
+// instead of three.  Dense when the padded planes would leave the lane
+// sweeps' 32-bit offset range.
+void fsm_pitches(const Grid& g, int64_t* px, int64_t* py) {
+    const int64_t dx = g.dims[0], dy = g.dims[1];
+    const int64_t ax = (dx + 31) / 32 * 32, ay = (dy + 31) / 32 * 32;
+    const bool ok = lane_sweeps_ok(g) && ax * dy < (int64_t(1) << 27) && ay * dx < (int64_t(1) << 27);
+    *px = ok ? ax : dx;
+    *py = ok ? ay : dy;
 }
 
 hipError_t launch_redblack(float* g, const int32_t dims[3], float h, int parity, hipStream_t s) {

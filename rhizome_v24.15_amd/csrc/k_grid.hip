@@ -1,6 +1,8 @@
 // k_grid.hip -- streaming passes over the voxel grid (HBM-bound byte work):
 // sentinel fill, sign apply, far-sentinel audit and clamp, unknown-voxel
 // compaction, patch-box marking, probe positions, and an exclusive scan.
+#include <algorithm>
+
 #include "rhz_device.hpp"
 #include "rhz_internal.hpp"
 
@@ -73,6 +75,30 @@ __global__ void clamp_far_kernel(float* __restrict__ d, const uint8_t* __restric
 }
 
 // [D2] audit: count d >= 1e29 (cuda_rasterizer.cu:523-534); one atomic per wave.
+// Rows of `len` floats at pitch `sp` copied to rows at pitch `dp` (a block per
+// group of rows, threads along the row: both sides coalesced).
+__global__ void copy_rows_kernel(const float* __restrict__ src, int64_t sp, float* __restrict__ dst, int64_t dp,
+                                 int len, int64_t rows) {
+    for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const float* a = src + r * sp;
+        float* b = dst + r * dp;
+        for (int x = threadIdx.x; x < len; x += blockDim.x) b[x] = a[x];
+    }
+}
+
+// Sentinel count over rows of `len` floats at pitch `p` (the padding between
+// rows is not counted).
+__global__ void count_far_rows_kernel(const float* __restrict__ d, int64_t p, int len, int64_t rows,
+                                      unsigned long long* counter) {
+    unsigned c = 0;
+    for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const float* a = d + r * p;
+        for (int x = threadIdx.x; x < len; x += blockDim.x) c += (a[x] >= kFarTest);
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (__lane_id() == 0 && c) atomicAdd(counter, (unsigned long long)c);
+}
+
 __global__ void count_far_kernel(const float* __restrict__ d, int64_t n, unsigned long long* counter) {
     const int64_t n4 = n >> 2;
     const float4* d4 = reinterpret_cast<const float4*>(d);
@@ -295,6 +321,22 @@ hipError_t launch_clamp_far(float* d, const uint8_t* sign, int64_t n, float bw, 
                             hipStream_t s) {
     if (n <= 0) return hipSuccess;
     clamp_far_kernel<<<stream_blocks(n), kBlock, 0, s>>>(d, sign, n, bw, intended);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_rows(const float* src, int64_t sp, float* dst, int64_t dp, int len, int64_t rows,
+                            hipStream_t s) {
+    if (rows <= 0 || len <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<int64_t>(rows, 16384);
+    copy_rows_kernel<<<blocks, kBlock, 0, s>>>(src, sp, dst, dp, len, rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_far_rows(const float* d, int64_t p, int len, int64_t rows, unsigned long long* counter,
+                                 hipStream_t s) {
+    if (rows <= 0 || len <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<int64_t>(rows, 16384);
+    count_far_rows_kernel<<<blocks, kBlock, 0, s>>>(d, p, len, rows, counter);
     return hipGetLastError();
 }
 

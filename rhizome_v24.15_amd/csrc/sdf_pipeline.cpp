@@ -60,7 +60,7 @@ int set_err(int code, const char* fmt, ...) {
 // per-device workspace
 // ---------------------------------------------------------------------------
 enum Slot {
-    S_DIST2 = 0,   // second FSM buffer
+    S_DIST2 = 0,   // FSM working buffer
     S_SIGN,        // sign volume
     S_XYAB,
     S_COVER,
@@ -117,6 +117,7 @@ enum Slot {
     S_MC_VOFF,     // first vertex id per voxel
     S_MC_VERTS,    // vertices / triangles staged by the host API
     S_MC_TRIS,
+    S_DIST3,       // second FSM working buffer (rows padded to 128 B)
     S_COUNT
 };
 
@@ -386,55 +387,70 @@ int fixup_unknown(Ctx& c, uint8_t* sign, int64_t* fixed) {
     return set_err(RHZ_ERR_HIP, "unknown-voxel compaction did not converge");
 }
 
-int sweep_round(Ctx& c, float* a, float* b, int* changed) {
-    // x+, x-, y+, y-, z+, z- (cuda_rasterizer.cu:1012-1017); six sweeps
-    // alternate the two buffers, so the round ends back in `a`.
-    const int axes[6] = {0, 0, 1, 1, 2, 2};
-    const int dirs[6] = {+1, -1, +1, -1, +1, -1};
-    float* src = a;
-    float* dst = b;
-    // every sweep launch is bracketed by its own events (per-launch timing);
-    // the plane transposes are timed apart from the sweeps
+// One round on the lane kernels: x+, x-, y+, y-, z+, z- (cuda_rasterizer.cu
+// :1012-1017).  `in` (x-fastest, row pitch in_pitch) is transposed into b
+// (y-fastest, row pitch py); x+ b -> a and x- a -> b on that copy; b is
+// transposed back into a (x-fastest, row pitch px); y and z alternate a and
+// b, so the round ends in a.  `in` may be a itself.  Every sweep launch is
+// bracketed by its own events; the transposes are timed apart.
+int sweep_round_p(Ctx& c, const float* in, int64_t in_pitch, float* a, float* b, int64_t px, int64_t py,
+                  int* changed) {
     auto sw0 = [&]() { if (c.sweeps_t) c.sweeps_t->begin(c.s); };
     auto sw1 = [&]() { if (c.sweeps_t) c.sweeps_t->end(c.s); };
     auto tr0 = [&]() { if (c.trans_t) c.trans_t->begin(c.s); };
     auto tr1 = [&]() { if (c.trans_t) c.trans_t->end(c.s); };
-    if (lane_sweeps_ok(c.g)) {
-        // x-sweeps run on a y-fastest copy: a -> b (transpose), x+ b -> a,
-        // x- a -> b, b -> a (transpose back); then y and z on x-fastest.
-        tr0();
-        HIP_TRY(launch_transpose_xy(a, b, c.g, true, c.s));
-        tr1();
+    tr0();
+    HIP_TRY(launch_transpose_xy(in, b, c.g, true, c.s, in_pitch, py));
+    tr1();
+    sw0();
+    HIP_TRY(launch_sweep_yx(b, a, c.g, +1, changed, c.s, py));
+    sw1();
+    sw0();
+    HIP_TRY(launch_sweep_yx(a, b, c.g, -1, changed, c.s, py));
+    sw1();
+    tr0();
+    HIP_TRY(launch_transpose_xy(b, a, c.g, false, c.s, py, px));
+    tr1();
+    const int axes[4] = {1, 1, 2, 2};
+    const int dirs[4] = {+1, -1, +1, -1};
+    float* src = a;
+    float* dst = b;
+    for (int k = 0; k < 4; ++k) {
         sw0();
-        HIP_TRY(launch_sweep_yx(b, a, c.g, +1, changed, c.s));
+        HIP_TRY(launch_sweep(src, dst, c.g, axes[k], dirs[k], changed, c.s, px));
         sw1();
-        sw0();
-        HIP_TRY(launch_sweep_yx(a, b, c.g, -1, changed, c.s));
-        sw1();
-        tr0();
-        HIP_TRY(launch_transpose_xy(b, a, c.g, false, c.s));
-        tr1();
-        for (int k = 2; k < 6; ++k) {
-            sw0();
-            HIP_TRY(launch_sweep(src, dst, c.g, axes[k], dirs[k], changed, c.s));
-            sw1();
-            std::swap(src, dst);
-        }
-    } else {
-        for (int k = 0; k < 6; ++k) {
-            sw0();
-            HIP_TRY(launch_sweep(src, dst, c.g, axes[k], dirs[k], changed, c.s));
-            sw1();
-            std::swap(src, dst);
-        }
+        std::swap(src, dst);
     }
     c.st->sweeps += 6;
     return RHZ_OK;
 }
 
-int audit(Ctx& c, int64_t* far) {
+// One round on a dense field in a (b: the other buffer); the round ends in a.
+int sweep_round(Ctx& c, float* a, float* b, int* changed) {
+    if (lane_sweeps_ok(c.g)) return sweep_round_p(c, a, c.g.dims[0], a, b, c.g.dims[0], c.g.dims[1], changed);
+    const int axes[6] = {0, 0, 1, 1, 2, 2};
+    const int dirs[6] = {+1, -1, +1, -1, +1, -1};
+    float* src = a;
+    float* dst = b;
+    for (int k = 0; k < 6; ++k) {
+        if (c.sweeps_t) c.sweeps_t->begin(c.s);
+        HIP_TRY(launch_sweep(src, dst, c.g, axes[k], dirs[k], changed, c.s));
+        if (c.sweeps_t) c.sweeps_t->end(c.s);
+        std::swap(src, dst);
+    }
+    c.st->sweeps += 6;
+    return RHZ_OK;
+}
+
+// Sentinel count of the x-fastest field d at row pitch `pitch` (the caller's
+// dense field when pitch == dims[0]).
+int audit(Ctx& c, int64_t* far, const float* d = nullptr, int64_t pitch = 0) {
     HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FAR), 0, sizeof(unsigned long long), c.s));
-    HIP_TRY(launch_count_far(c.d, c.g.n(), cnt_ptr(c.w, C_FAR), c.s));
+    if (!d || pitch == c.g.dims[0])
+        HIP_TRY(launch_count_far(d ? d : c.d, c.g.n(), cnt_ptr(c.w, C_FAR), c.s));
+    else
+        HIP_TRY(launch_count_far_rows(d, pitch, c.g.dims[0], (int64_t)c.g.dims[1] * c.g.dims[2], cnt_ptr(c.w, C_FAR),
+                                      c.s));
     int rc = read_counters(c);
     if (rc) return rc;
     *far = (int64_t)hc(c, C_FAR);
@@ -446,7 +462,8 @@ int audit(Ctx& c, int64_t* far) {
 // The policy's round count is the first round that changed nothing (+1): the
 // rounds after it see an unchanged field and change nothing either, so the
 // result equals the round-by-round loop.
-int fsm_rounds_batched(Ctx& c, float* b, int cap, int* rounds_out) {
+template <typename Round>
+int fsm_rounds_batched(Ctx& c, int cap, int* rounds_out, Round&& round) {
     int* flags = flag_ptr(c.w, F_ROUND0);
     const bool graph = sweep_tuning().fsm_graph != 0 && c.s != nullptr;
     EventAcc* sw = c.sweeps_t;
@@ -461,7 +478,7 @@ int fsm_rounds_batched(Ctx& c, float* b, int cap, int* rounds_out) {
     int rc = RHZ_OK;
     hipError_t e = launch_zero_i32(flags, cap, c.s);
     if (e != hipSuccess) rc = set_err(RHZ_ERR_HIP, "flag reset failed: %s", hipGetErrorString(e));
-    for (int r = 0; r < cap && rc == RHZ_OK; ++r) rc = sweep_round(c, c.d, b, flags + r);
+    for (int r = 0; r < cap && rc == RHZ_OK; ++r) rc = round(flags + r);
     if (graph) {
         const hipError_t ec = hipStreamEndCapture(c.s, &gr);
         c.sweeps_t = sw;
@@ -489,22 +506,42 @@ int fsm_rounds_batched(Ctx& c, float* b, int cap, int* rounds_out) {
 
 // [D] far-field re-distancing with the reference round policy: the rounds
 // and the audit-gated extension rounds; *far = voxels still at the sentinel.
+// The rounds run on two working buffers whose rows are padded to 128 B
+// (fsm_pitches); the first round reads the caller's dense field, and the
+// result is copied back into it at the end.  Lattices the lane kernels do not
+// take (or whose rows are already aligned) run in place on the dense field.
 int run_fsm_rounds(Ctx& c, int64_t* far_out) {
     const int64_t V = c.g.n();
-    float* b = (float*)ws_get(c.w, S_DIST2, sizeof(float) * (size_t)V);
-    if (!b) return set_err(RHZ_ERR_NOMEM, "second sweep buffer allocation failed");
+    const int64_t dx = c.g.dims[0], dy = c.g.dims[1], dz = c.g.dims[2];
+    const bool lanes = lane_sweeps_ok(c.g);
+    int64_t px = dx, py = dy;
+    if (lanes) fsm_pitches(c.g, &px, &py);
+    const bool padded = px != dx || py != dy;
+    const size_t cells = padded ? (size_t)(std::max(px * dy, py * dx) * dz) : (size_t)V;
+    float* q = (float*)ws_get(c.w, S_DIST2, sizeof(float) * cells);
+    float* pbuf = padded ? (float*)ws_get(c.w, S_DIST3, sizeof(float) * cells) : c.d;
+    if (!q || !pbuf) return set_err(RHZ_ERR_NOMEM, "sweep buffer allocation failed");
+    const float* cur = c.d;  // the field: the caller's dense buffer until a round has run
+    int64_t cur_pitch = dx;
+    auto round = [&](int* flag) -> int {
+        if (!lanes) return sweep_round(c, c.d, q, flag);
+        const int rc = sweep_round_p(c, cur, cur_pitch, pbuf, q, px, py, flag);
+        cur = pbuf;
+        cur_pitch = px;
+        return rc;
+    };
     int* changed = flag_ptr(c.w, F_CHANGED);
     const bool converge = c.p->fsm_policy == RHZ_FSM_CONVERGE;
     const int cap = converge ? c.p->fsm_round_cap : c.p->max_rounds;
     int rounds = 0;
     const SweepTuning tune = sweep_tuning();
     if (!converge && cap > 0 && cap <= kMaxBatchedRounds && V <= tune.batch_rounds_max_voxels) {
-        int rc = fsm_rounds_batched(c, b, cap, &rounds);
+        int rc = fsm_rounds_batched(c, cap, &rounds, round);
         if (rc) return rc;
     } else
     for (int r = 0; r < cap; ++r) {
         HIP_TRY(hipMemsetAsync(changed, 0, sizeof(int), c.s));
-        int rc = sweep_round(c, c.d, b, changed);
+        int rc = round(changed);
         if (rc) return rc;
         rc = read_counters(c);
         if (rc) return rc;
@@ -512,16 +549,17 @@ int run_fsm_rounds(Ctx& c, int64_t* far_out) {
         if (hf(c, F_CHANGED) == 0) break;
     }
     int64_t far = 0;
-    int rc = audit(c, &far);
+    int rc = audit(c, &far, cur, cur_pitch);
     if (rc) return rc;
     int extra = 0;
     while (far > 0 && extra < c.p->max_extra_rounds) {
-        rc = sweep_round(c, c.d, b, changed);
+        rc = round(changed);
         if (rc) return rc;
         ++extra;
-        rc = audit(c, &far);
+        rc = audit(c, &far, cur, cur_pitch);
         if (rc) return rc;
     }
+    if (cur != c.d) HIP_TRY(launch_copy_rows(cur, cur_pitch, c.d, dx, (int)dx, dy * dz, c.s));
     c.st->eikonal_rounds = rounds + extra;
     c.st->extra_rounds = extra;
     *far_out = far;
