@@ -671,7 +671,7 @@ __global__ void redblack_kernel(float* __restrict__ g, int dx, int dy, int dz, f
 
 // Launch shape of the lane sweeps (steps per load group, waves per block);
 // set by the tuning entry point, read at every launch.
-static SweepTuning g_sweep_tuning = {2, 4, 0, {14, 21, 22}, 16ll << 20, 0};
+static SweepTuning g_sweep_tuning = {2, 4, 0, {14, 21, 21}, 16ll << 20, 0};
 SweepTuning sweep_tuning() { return g_sweep_tuning; }
 bool set_sweep_tuning(int step, int rows, int fmajor) {
     if ((step != 2 && step != 4 && step != 8) || (rows != 4 && rows != 8 && rows != 16) ||
