@@ -796,51 +796,64 @@ bool lane_sweeps_ok(const Grid& g) {
 }
 
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s,
-                               int64_t in_pitch, int64_t out_pitch) {
+                               const FieldLayout& lin, const FieldLayout& lout) {
     const int dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
     const int rows = to_yx ? dy : dx, cols = to_yx ? dx : dy;
-    if (in_pitch <= 0) in_pitch = cols;
-    if (out_pitch <= 0) out_pitch = rows;
-    if (in_pitch < cols || out_pitch < rows) return hipErrorInvalidValue;
+    if (lin.pitch < cols || lout.pitch < rows || lin.plane < lin.pitch * rows || lout.plane < lout.pitch * cols)
+        return hipErrorInvalidValue;
     const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
     const int64_t blocks = (int64_t)tr * ((tc + kTrTiles - 1) / kTrTiles) * dz;
-    transpose_planes_kernel<<<(unsigned)blocks, 256, 0, s>>>(in, out, rows, cols, in_pitch, in_pitch * rows, out_pitch,
-                                                            out_pitch * cols, tr, tc, (unsigned)blocks);
+    transpose_planes_kernel<<<(unsigned)blocks, 256, 0, s>>>(in, out, rows, cols, lin.pitch, lin.plane, lout.pitch,
+                                                            lout.plane, tr, tc, (unsigned)blocks);
     return hipGetLastError();
 }
 
+hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s) {
+    const FieldLayout xy = dense_xy(g), yx = dense_yx(g);
+    return launch_transpose_xy(in, out, g, to_yx, s, to_yx ? xy : yx, to_yx ? yx : xy);
+}
+
 hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir, int* changed, hipStream_t s,
-                           int64_t pitch) {
-    // y-fastest layout: lanes over y (Godunov slot 1), walk x (slot 0, stride = row pitch), cross z (slot 2)
+                           const FieldLayout& L) {
+    // y-fastest layout: lanes over y (Godunov slot 1), walk x (slot 0, stride = row pitch), cross z (slot 2, plane)
     const int dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
-    if (pitch <= 0) pitch = dy;
-    return launch_lane<1, 0, 2>(src, dst, dy, dx, dz, pitch, pitch * dx, g.h, dir, changed, s);
+    return launch_lane<1, 0, 2>(src, dst, dy, dx, dz, L.pitch, L.plane, g.h, dir, changed, s);
+}
+
+hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir, int* changed, hipStream_t s) {
+    return launch_sweep_yx(src, dst, g, dir, changed, s, dense_yx(g));
 }
 
 hipError_t launch_sweep(const float* src, float* dst, const Grid& g, int axis, int dir, int* changed,
-                        hipStream_t s, int64_t pitch) {
+                        hipStream_t s, const FieldLayout& L) {
     const int dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
-    if (pitch <= 0) pitch = dx;
     if (axis == 0 || !lane_sweeps_ok(g)) {
-        if (pitch != dx) return hipErrorInvalidValue;  // the generic kernels take dense fields only
+        const FieldLayout d = dense_xy(g);
+        if (L.pitch != d.pitch || L.plane != d.plane) return hipErrorInvalidValue;  // generic kernels: dense only
         return launch_generic_sweep(src, dst, g, axis, dir, changed, s);
     }
-    if (axis == 1)  // lanes x (slot 0), walk y (slot 1, stride = row pitch), cross z (slot 2)
-        return launch_lane<0, 1, 2>(src, dst, dx, dy, dz, pitch, pitch * dy, g.h, dir, changed, s);
+    if (axis == 1)  // lanes x (slot 0), walk y (slot 1, stride = row pitch), cross z (slot 2, plane)
+        return launch_lane<0, 1, 2>(src, dst, dx, dy, dz, L.pitch, L.plane, g.h, dir, changed, s);
     // axis 2: lanes x (slot 0), walk z (slot 2, stride = plane), cross y (slot 1, stride = row pitch)
-    return launch_lane<0, 2, 1>(src, dst, dx, dz, dy, pitch * dy, pitch, g.h, dir, changed, s);
+    return launch_lane<0, 2, 1>(src, dst, dx, dz, dy, L.plane, L.pitch, g.h, dir, changed, s);
 }
 
-// Row pitches of the FSM's working layouts: rows rounded up to 32 floats
-// (128 B), so that every 64-cell segment of a row spans two cache lines
-// instead of three.  Dense when the padded planes would leave the lane
-// sweeps' 32-bit offset range.
-void fsm_pitches(const Grid& g, int64_t* px, int64_t* py) {
+hipError_t launch_sweep(const float* src, float* dst, const Grid& g, int axis, int dir, int* changed,
+                        hipStream_t s) {
+    return launch_sweep(src, dst, g, axis, dir, changed, s, dense_xy(g));
+}
+
+// The FSM's working layouts: rows rounded up to 32 floats (128 B), so that
+// every 64-cell segment of a row spans two cache lines instead of three;
+// planes hold the lattice's own row count (rounding it up as well measured
+// the same).  Dense when the padded planes would leave the lane sweeps'
+// 32-bit offset range.
+void fsm_layouts(const Grid& g, FieldLayout* xy, FieldLayout* yx) {
     const int64_t dx = g.dims[0], dy = g.dims[1];
-    const int64_t ax = (dx + 31) / 32 * 32, ay = (dy + 31) / 32 * 32;
-    const bool ok = lane_sweeps_ok(g) && ax * dy < (int64_t(1) << 27) && ay * dx < (int64_t(1) << 27);
-    *px = ok ? ax : dx;
-    *py = ok ? ay : dy;
+    const int64_t px = (dx + 31) / 32 * 32, py = (dy + 31) / 32 * 32;
+    const bool ok = lane_sweeps_ok(g) && px * dy < (int64_t(1) << 27) && py * dx < (int64_t(1) << 27);
+    *xy = ok ? FieldLayout{px, px * dy} : dense_xy(g);
+    *yx = ok ? FieldLayout{py, py * dx} : dense_yx(g);
 }
 
 hipError_t launch_redblack(float* g, const int32_t dims[3], float h, int parity, hipStream_t s) {

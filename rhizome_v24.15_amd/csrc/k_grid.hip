@@ -77,22 +77,29 @@ __global__ void clamp_far_kernel(float* __restrict__ d, const uint8_t* __restric
 // [D2] audit: count d >= 1e29 (cuda_rasterizer.cu:523-534); one atomic per wave.
 // Rows of `len` floats at pitch `sp` copied to rows at pitch `dp` (a block per
 // group of rows, threads along the row: both sides coalesced).
-__global__ void copy_rows_kernel(const float* __restrict__ src, int64_t sp, float* __restrict__ dst, int64_t dp,
-                                 int len, int64_t rows) {
+// Row r of a field with rpp rows per plane sits at plane (r / rpp) and row
+// (r % rpp) of that plane.
+__device__ __forceinline__ int64_t row_offset(int64_t r, int rpp, int64_t pitch, int64_t plane) {
+    const int64_t z = r / rpp;
+    return z * plane + (r - z * rpp) * pitch;
+}
+
+__global__ void copy_rows_kernel(const float* __restrict__ src, int64_t sp, int64_t splane, float* __restrict__ dst,
+                                 int64_t dp, int64_t dplane, int len, int rpp, int64_t rows) {
     for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
-        const float* a = src + r * sp;
-        float* b = dst + r * dp;
+        const float* a = src + row_offset(r, rpp, sp, splane);
+        float* b = dst + row_offset(r, rpp, dp, dplane);
         for (int x = threadIdx.x; x < len; x += blockDim.x) b[x] = a[x];
     }
 }
 
-// Sentinel count over rows of `len` floats at pitch `p` (the padding between
-// rows is not counted).
-__global__ void count_far_rows_kernel(const float* __restrict__ d, int64_t p, int len, int64_t rows,
-                                      unsigned long long* counter) {
+// Sentinel count over rows of `len` floats (the padding between rows and
+// between planes is not counted).
+__global__ void count_far_rows_kernel(const float* __restrict__ d, int64_t p, int64_t plane, int len, int rpp,
+                                      int64_t rows, unsigned long long* counter) {
     unsigned c = 0;
     for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
-        const float* a = d + r * p;
+        const float* a = d + row_offset(r, rpp, p, plane);
         for (int x = threadIdx.x; x < len; x += blockDim.x) c += (a[x] >= kFarTest);
     }
     for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off);
@@ -324,19 +331,23 @@ hipError_t launch_clamp_far(float* d, const uint8_t* sign, int64_t n, float bw, 
     return hipGetLastError();
 }
 
-hipError_t launch_copy_rows(const float* src, int64_t sp, float* dst, int64_t dp, int len, int64_t rows,
-                            hipStream_t s) {
-    if (rows <= 0 || len <= 0) return hipSuccess;
+hipError_t launch_copy_rows(const float* src, int64_t sp, int64_t splane, float* dst, int64_t dp, int64_t dplane,
+                            int len, int rpp, int planes, hipStream_t s) {
+    if (len <= 0 || rpp <= 0 || planes <= 0) return hipSuccess;
+    if (sp < len || dp < len || splane < sp * rpp || dplane < dp * rpp) return hipErrorInvalidValue;
+    const int64_t rows = (int64_t)rpp * planes;
     const unsigned blocks = (unsigned)std::min<int64_t>(rows, 16384);
-    copy_rows_kernel<<<blocks, kBlock, 0, s>>>(src, sp, dst, dp, len, rows);
+    copy_rows_kernel<<<blocks, kBlock, 0, s>>>(src, sp, splane, dst, dp, dplane, len, rpp, rows);
     return hipGetLastError();
 }
 
-hipError_t launch_count_far_rows(const float* d, int64_t p, int len, int64_t rows, unsigned long long* counter,
-                                 hipStream_t s) {
-    if (rows <= 0 || len <= 0) return hipSuccess;
+hipError_t launch_count_far_rows(const float* d, int64_t p, int64_t plane, int len, int rpp, int planes,
+                                 unsigned long long* counter, hipStream_t s) {
+    if (len <= 0 || rpp <= 0 || planes <= 0) return hipSuccess;
+    if (p < len || plane < p * rpp) return hipErrorInvalidValue;
+    const int64_t rows = (int64_t)rpp * planes;
     const unsigned blocks = (unsigned)std::min<int64_t>(rows, 16384);
-    count_far_rows_kernel<<<blocks, kBlock, 0, s>>>(d, p, len, rows, counter);
+    count_far_rows_kernel<<<blocks, kBlock, 0, s>>>(d, p, plane, len, rpp, rows, counter);
     return hipGetLastError();
 }
 

@@ -24,12 +24,13 @@ hipError_t launch_clamp_far(float* d, const uint8_t* sign, int64_t n, float bw, 
                             hipStream_t s);
 // counter[0] += number of d[i] >= 1e29
 hipError_t launch_count_far(const float* d, int64_t n, unsigned long long* counter, hipStream_t s);
-// `rows` rows of `len` floats from pitch sp to pitch dp (floats).
-hipError_t launch_copy_rows(const float* src, int64_t sp, float* dst, int64_t dp, int len, int64_t rows,
-                            hipStream_t s);
-// Sentinel count over `rows` rows of `len` floats at pitch p (padding skipped).
-hipError_t launch_count_far_rows(const float* d, int64_t p, int len, int64_t rows, unsigned long long* counter,
-                                 hipStream_t s);
+// The rows of an x-fastest field (len = dims[0] floats, rpp = dims[1] rows
+// per plane, planes = dims[2]) copied between strides (row pitch, plane).
+hipError_t launch_copy_rows(const float* src, int64_t sp, int64_t splane, float* dst, int64_t dp, int64_t dplane,
+                            int len, int rpp, int planes, hipStream_t s);
+// Sentinel count over such rows (the padding between them is skipped).
+hipError_t launch_count_far_rows(const float* d, int64_t p, int64_t plane, int len, int rpp, int planes,
+                                 unsigned long long* counter, hipStream_t s);
 // Appends i for every sign[i] == 2 to list (capacity cap); counter += count.
 hipError_t launch_compact_unknown(const uint8_t* sign, int64_t n, int64_t* list, int64_t cap,
                                   unsigned long long* counter, hipStream_t s);
@@ -76,10 +77,21 @@ hipError_t launch_parity(const float* tris, const Grid& g, int sup, const CsrBuf
                          hipStream_t s, bool signed_count = false);
 
 // ---- k_fsm.hip ------------------------------------------------------------
-// pitch: row pitch of the x-fastest field in floats (0: dense, dims[0]); the
-// plane stride is pitch * dims[1].  Axis 0 (generic kernels) takes dense only.
+// Strides of a field layout in floats: between rows of the fastest axis and
+// between z planes.  x-fastest: (x, y, z) at z plane + y pitch + x;
+// y-fastest: (x, y, z) at z plane + x pitch + y.
+struct FieldLayout {
+    int64_t pitch;
+    int64_t plane;
+};
+inline FieldLayout dense_xy(const Grid& g) { return {g.dims[0], (int64_t)g.dims[0] * g.dims[1]}; }
+inline FieldLayout dense_yx(const Grid& g) { return {g.dims[1], (int64_t)g.dims[0] * g.dims[1]}; }
+// Sweep of an x-fastest field (dense, or layout L).  Axis 0 (generic
+// kernels) takes dense fields only.
 hipError_t launch_sweep(const float* src, float* dst, const Grid& g, int axis, int dir,
-                        int* changed, hipStream_t s, int64_t pitch = 0);
+                        int* changed, hipStream_t s);
+hipError_t launch_sweep(const float* src, float* dst, const Grid& g, int axis, int dir,
+                        int* changed, hipStream_t s, const FieldLayout& L);
 // True when the lane-walker sweeps apply (every axis has >= 2 cells, a plane
 // has < 2^27 cells, 2^-30 <= h <= 2^30); otherwise launch_sweep uses generic kernels.
 bool lane_sweeps_ok(const Grid& g);
@@ -98,17 +110,17 @@ SweepTuning sweep_tuning();
 bool set_sweep_tuning(int step, int rows, int fmajor);
 bool set_sweep_shape(int axis, int shape);
 void set_fsm_batching(int64_t max_voxels, int graph);
-// Per-plane transpose between the x-fastest and the y-fastest layouts; the
-// pitches are the row strides of the input and the output in floats (0:
-// dense), each plane holding (rows of the other axis) x pitch.
+// Per-plane transpose between the x-fastest and the y-fastest layouts (dense,
+// or the layouts of the input and the output).
+hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s);
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s,
-                               int64_t in_pitch = 0, int64_t out_pitch = 0);
-// x-sweep on the y-fastest layout (src/dst both y-fastest, row pitch `pitch`, 0: dense).
+                               const FieldLayout& lin, const FieldLayout& lout);
+// x-sweep on the y-fastest layout (src/dst both y-fastest; dense, or layout L).
+hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir, int* changed, hipStream_t s);
 hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir, int* changed, hipStream_t s,
-                           int64_t pitch = 0);
-// Row pitches of the FSM's padded working copies: x-fastest (px) and
-// y-fastest (py), rounded up to 128 B when the lane sweeps apply.
-void fsm_pitches(const Grid& g, int64_t* px, int64_t* py);
+                           const FieldLayout& L);
+// The padded layouts of the FSM's working copies (x-fastest and y-fastest).
+void fsm_layouts(const Grid& g, FieldLayout* xy, FieldLayout* yx);
 // ---- k_mc.hip -------------------------------------------------------------
 // Marching cubes in three passes: classify (in: inside byte per voxel; info: u8 per voxel; blk_v / blk_t:
 // u32 per block of mc_blocks(dims)), prefix sums of the block totals into

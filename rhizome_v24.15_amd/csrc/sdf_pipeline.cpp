@@ -388,28 +388,28 @@ int fixup_unknown(Ctx& c, uint8_t* sign, int64_t* fixed) {
 }
 
 // One round on the lane kernels: x+, x-, y+, y-, z+, z- (cuda_rasterizer.cu
-// :1012-1017).  `in` (x-fastest, row pitch in_pitch) is transposed into b
-// (y-fastest, row pitch py); x+ b -> a and x- a -> b on that copy; b is
-// transposed back into a (x-fastest, row pitch px); y and z alternate a and
-// b, so the round ends in a.  `in` may be a itself.  Every sweep launch is
-// bracketed by its own events; the transposes are timed apart.
-int sweep_round_p(Ctx& c, const float* in, int64_t in_pitch, float* a, float* b, int64_t px, int64_t py,
-                  int* changed) {
+// :1012-1017).  `in` (x-fastest, layout lin) is transposed into b (y-fastest,
+// layout ly); x+ b -> a and x- a -> b on that copy; b is transposed back into
+// a (x-fastest, layout lx); y and z alternate a and b, so the round ends in
+// a.  `in` may be a itself.  Every sweep launch is bracketed by its own
+// events; the transposes are timed apart.
+int sweep_round_p(Ctx& c, const float* in, const FieldLayout& lin, float* a, float* b, const FieldLayout& lx,
+                  const FieldLayout& ly, int* changed) {
     auto sw0 = [&]() { if (c.sweeps_t) c.sweeps_t->begin(c.s); };
     auto sw1 = [&]() { if (c.sweeps_t) c.sweeps_t->end(c.s); };
     auto tr0 = [&]() { if (c.trans_t) c.trans_t->begin(c.s); };
     auto tr1 = [&]() { if (c.trans_t) c.trans_t->end(c.s); };
     tr0();
-    HIP_TRY(launch_transpose_xy(in, b, c.g, true, c.s, in_pitch, py));
+    HIP_TRY(launch_transpose_xy(in, b, c.g, true, c.s, lin, ly));
     tr1();
     sw0();
-    HIP_TRY(launch_sweep_yx(b, a, c.g, +1, changed, c.s, py));
+    HIP_TRY(launch_sweep_yx(b, a, c.g, +1, changed, c.s, ly));
     sw1();
     sw0();
-    HIP_TRY(launch_sweep_yx(a, b, c.g, -1, changed, c.s, py));
+    HIP_TRY(launch_sweep_yx(a, b, c.g, -1, changed, c.s, ly));
     sw1();
     tr0();
-    HIP_TRY(launch_transpose_xy(b, a, c.g, false, c.s, py, px));
+    HIP_TRY(launch_transpose_xy(b, a, c.g, false, c.s, ly, lx));
     tr1();
     const int axes[4] = {1, 1, 2, 2};
     const int dirs[4] = {+1, -1, +1, -1};
@@ -417,7 +417,7 @@ int sweep_round_p(Ctx& c, const float* in, int64_t in_pitch, float* a, float* b,
     float* dst = b;
     for (int k = 0; k < 4; ++k) {
         sw0();
-        HIP_TRY(launch_sweep(src, dst, c.g, axes[k], dirs[k], changed, c.s, px));
+        HIP_TRY(launch_sweep(src, dst, c.g, axes[k], dirs[k], changed, c.s, lx));
         sw1();
         std::swap(src, dst);
     }
@@ -427,7 +427,7 @@ int sweep_round_p(Ctx& c, const float* in, int64_t in_pitch, float* a, float* b,
 
 // One round on a dense field in a (b: the other buffer); the round ends in a.
 int sweep_round(Ctx& c, float* a, float* b, int* changed) {
-    if (lane_sweeps_ok(c.g)) return sweep_round_p(c, a, c.g.dims[0], a, b, c.g.dims[0], c.g.dims[1], changed);
+    if (lane_sweeps_ok(c.g)) return sweep_round_p(c, a, dense_xy(c.g), a, b, dense_xy(c.g), dense_yx(c.g), changed);
     const int axes[6] = {0, 0, 1, 1, 2, 2};
     const int dirs[6] = {+1, -1, +1, -1, +1, -1};
     float* src = a;
@@ -442,15 +442,16 @@ int sweep_round(Ctx& c, float* a, float* b, int* changed) {
     return RHZ_OK;
 }
 
-// Sentinel count of the x-fastest field d at row pitch `pitch` (the caller's
-// dense field when pitch == dims[0]).
-int audit(Ctx& c, int64_t* far, const float* d = nullptr, int64_t pitch = 0) {
+// Sentinel count of the x-fastest field d in layout L (the caller's dense
+// field when d is null).
+int audit(Ctx& c, int64_t* far, const float* d = nullptr, const FieldLayout* L = nullptr) {
     HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FAR), 0, sizeof(unsigned long long), c.s));
-    if (!d || pitch == c.g.dims[0])
+    const FieldLayout dense = dense_xy(c.g);
+    if (!d || !L || (L->pitch == dense.pitch && L->plane == dense.plane))
         HIP_TRY(launch_count_far(d ? d : c.d, c.g.n(), cnt_ptr(c.w, C_FAR), c.s));
     else
-        HIP_TRY(launch_count_far_rows(d, pitch, c.g.dims[0], (int64_t)c.g.dims[1] * c.g.dims[2], cnt_ptr(c.w, C_FAR),
-                                      c.s));
+        HIP_TRY(launch_count_far_rows(d, L->pitch, L->plane, c.g.dims[0], c.g.dims[1], c.g.dims[2],
+                                      cnt_ptr(c.w, C_FAR), c.s));
     int rc = read_counters(c);
     if (rc) return rc;
     *far = (int64_t)hc(c, C_FAR);
@@ -506,28 +507,29 @@ int fsm_rounds_batched(Ctx& c, int cap, int* rounds_out, Round&& round) {
 
 // [D] far-field re-distancing with the reference round policy: the rounds
 // and the audit-gated extension rounds; *far = voxels still at the sentinel.
-// The rounds run on two working buffers whose rows are padded to 128 B
-// (fsm_pitches); the first round reads the caller's dense field, and the
+// The rounds run on two working buffers with padded rows and planes
+// (fsm_layouts); the first round reads the caller's dense field, and the
 // result is copied back into it at the end.  Lattices the lane kernels do not
-// take (or whose rows are already aligned) run in place on the dense field.
+// take (or that are already aligned) run in place on the dense field.
 int run_fsm_rounds(Ctx& c, int64_t* far_out) {
     const int64_t V = c.g.n();
-    const int64_t dx = c.g.dims[0], dy = c.g.dims[1], dz = c.g.dims[2];
+    const int64_t dx = c.g.dims[0], dz = c.g.dims[2];
     const bool lanes = lane_sweeps_ok(c.g);
-    int64_t px = dx, py = dy;
-    if (lanes) fsm_pitches(c.g, &px, &py);
-    const bool padded = px != dx || py != dy;
-    const size_t cells = padded ? (size_t)(std::max(px * dy, py * dx) * dz) : (size_t)V;
+    const FieldLayout dense = dense_xy(c.g);
+    FieldLayout lx = dense, ly = dense_yx(c.g);
+    if (lanes) fsm_layouts(c.g, &lx, &ly);
+    const bool padded = lx.plane != dense.plane || ly.plane != dense.plane;
+    const size_t cells = padded ? (size_t)(std::max(lx.plane, ly.plane) * dz) : (size_t)V;
     float* q = (float*)ws_get(c.w, S_DIST2, sizeof(float) * cells);
     float* pbuf = padded ? (float*)ws_get(c.w, S_DIST3, sizeof(float) * cells) : c.d;
     if (!q || !pbuf) return set_err(RHZ_ERR_NOMEM, "sweep buffer allocation failed");
     const float* cur = c.d;  // the field: the caller's dense buffer until a round has run
-    int64_t cur_pitch = dx;
+    FieldLayout cur_l = dense;
     auto round = [&](int* flag) -> int {
         if (!lanes) return sweep_round(c, c.d, q, flag);
-        const int rc = sweep_round_p(c, cur, cur_pitch, pbuf, q, px, py, flag);
+        const int rc = sweep_round_p(c, cur, cur_l, pbuf, q, lx, ly, flag);
         cur = pbuf;
-        cur_pitch = px;
+        cur_l = lx;
         return rc;
     };
     int* changed = flag_ptr(c.w, F_CHANGED);
@@ -549,17 +551,19 @@ int run_fsm_rounds(Ctx& c, int64_t* far_out) {
         if (hf(c, F_CHANGED) == 0) break;
     }
     int64_t far = 0;
-    int rc = audit(c, &far, cur, cur_pitch);
+    int rc = audit(c, &far, cur, &cur_l);
     if (rc) return rc;
     int extra = 0;
     while (far > 0 && extra < c.p->max_extra_rounds) {
         rc = round(changed);
         if (rc) return rc;
         ++extra;
-        rc = audit(c, &far, cur, cur_pitch);
+        rc = audit(c, &far, cur, &cur_l);
         if (rc) return rc;
     }
-    if (cur != c.d) HIP_TRY(launch_copy_rows(cur, cur_pitch, c.d, dx, (int)dx, dy * dz, c.s));
+    if (cur != c.d)
+        HIP_TRY(launch_copy_rows(cur, cur_l.pitch, cur_l.plane, c.d, dense.pitch, dense.plane, (int)dx, c.g.dims[1],
+                                 (int)dz, c.s));
     c.st->eikonal_rounds = rounds + extra;
     c.st->extra_rounds = extra;
     *far_out = far;
