@@ -74,9 +74,6 @@ __global__ void clamp_far_kernel(float* __restrict__ d, const uint8_t* __restric
     }
 }
 
-// [D2] audit: count d >= 1e29 (cuda_rasterizer.cu:523-534); one atomic per wave.
-// Rows of `len` floats at pitch `sp` copied to rows at pitch `dp` (a block per
-// group of rows, threads along the row: both sides coalesced).
 // Row r of a field with rpp rows per plane sits at plane (r / rpp) and row
 // (r % rpp) of that plane.
 __device__ __forceinline__ int64_t row_offset(int64_t r, int rpp, int64_t pitch, int64_t plane) {
@@ -84,12 +81,27 @@ __device__ __forceinline__ int64_t row_offset(int64_t r, int rpp, int64_t pitch,
     return z * plane + (r - z * rpp) * pitch;
 }
 
+// Rows of `len` floats copied between two layouts (a block per group of
+// rows, threads along the row: both sides coalesced).
 __global__ void copy_rows_kernel(const float* __restrict__ src, int64_t sp, int64_t splane, float* __restrict__ dst,
                                  int64_t dp, int64_t dplane, int len, int rpp, int64_t rows) {
     for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
         const float* a = src + row_offset(r, rpp, sp, splane);
         float* b = dst + row_offset(r, rpp, dp, dplane);
         for (int x = threadIdx.x; x < len; x += blockDim.x) b[x] = a[x];
+    }
+}
+
+// copy_rows_kernel into the dense field (row r at r * len) with apply_one on
+// the way, the sign read at the same dense index.
+__global__ void copy_apply_rows_kernel(const float* __restrict__ src, int64_t sp, int64_t splane,
+                                       float* __restrict__ dst, const uint8_t* __restrict__ sign, int len, int rpp,
+                                       int64_t rows) {
+    for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const float* a = src + row_offset(r, rpp, sp, splane);
+        float* b = dst + r * len;
+        const uint8_t* g = sign + r * len;
+        for (int x = threadIdx.x; x < len; x += blockDim.x) b[x] = apply_one(a[x], g[x]);
     }
 }
 
@@ -106,6 +118,7 @@ __global__ void count_far_rows_kernel(const float* __restrict__ d, int64_t p, in
     if (__lane_id() == 0 && c) atomicAdd(counter, (unsigned long long)c);
 }
 
+// [D2] audit: count d >= 1e29 (cuda_rasterizer.cu:523-534); one atomic per wave.
 __global__ void count_far_kernel(const float* __restrict__ d, int64_t n, unsigned long long* counter) {
     const int64_t n4 = n >> 2;
     const float4* d4 = reinterpret_cast<const float4*>(d);
@@ -338,6 +351,16 @@ hipError_t launch_copy_rows(const float* src, int64_t sp, int64_t splane, float*
     const int64_t rows = (int64_t)rpp * planes;
     const unsigned blocks = (unsigned)std::min<int64_t>(rows, 16384);
     copy_rows_kernel<<<blocks, kBlock, 0, s>>>(src, sp, splane, dst, dp, dplane, len, rpp, rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_apply_rows(const float* src, int64_t sp, int64_t splane, float* dst, const uint8_t* sign,
+                                  int len, int rpp, int planes, hipStream_t s) {
+    if (len <= 0 || rpp <= 0 || planes <= 0) return hipSuccess;
+    if (sp < len || splane < sp * rpp) return hipErrorInvalidValue;
+    const int64_t rows = (int64_t)rpp * planes;
+    const unsigned blocks = (unsigned)std::min<int64_t>(rows, 16384);
+    copy_apply_rows_kernel<<<blocks, kBlock, 0, s>>>(src, sp, splane, dst, sign, len, rpp, rows);
     return hipGetLastError();
 }
 

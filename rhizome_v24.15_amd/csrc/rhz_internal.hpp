@@ -28,6 +28,10 @@ hipError_t launch_count_far(const float* d, int64_t n, unsigned long long* count
 // per plane, planes = dims[2]) copied between strides (row pitch, plane).
 hipError_t launch_copy_rows(const float* src, int64_t sp, int64_t splane, float* dst, int64_t dp, int64_t dplane,
                             int len, int rpp, int planes, hipStream_t s);
+// The same rows copied into the dense field d with [F] applied on the way
+// (sign in the dense layout): the copy and launch_apply in one pass.
+hipError_t launch_copy_apply_rows(const float* src, int64_t sp, int64_t splane, float* dst, const uint8_t* sign,
+                                  int len, int rpp, int planes, hipStream_t s);
 // Sentinel count over such rows (the padding between them is skipped).
 hipError_t launch_count_far_rows(const float* d, int64_t p, int64_t plane, int len, int rpp, int planes,
                                  unsigned long long* counter, hipStream_t s);

@@ -505,13 +505,21 @@ int fsm_rounds_batched(Ctx& c, int cap, int* rounds_out, Round&& round) {
     return RHZ_OK;
 }
 
+// Where the FSM rounds left the field: the caller's dense buffer, or a padded
+// working copy in layout l.
+struct FsmField {
+    const float* d;
+    FieldLayout l;
+};
+
 // [D] far-field re-distancing with the reference round policy: the rounds
 // and the audit-gated extension rounds; *far = voxels still at the sentinel.
-// The rounds run on two working buffers with padded rows and planes
-// (fsm_layouts); the first round reads the caller's dense field, and the
-// result is copied back into it at the end.  Lattices the lane kernels do not
+// The rounds run on two working buffers with padded rows (fsm_layouts); the
+// first round reads the caller's dense field, and the result is copied back
+// into it at the end, or, when `keep` is given, left where it is and named in
+// *keep (run_fsm_finish writes it back).  Lattices the lane kernels do not
 // take (or that are already aligned) run in place on the dense field.
-int run_fsm_rounds(Ctx& c, int64_t* far_out) {
+int run_fsm_rounds(Ctx& c, int64_t* far_out, FsmField* keep = nullptr) {
     const int64_t V = c.g.n();
     const int64_t dx = c.g.dims[0], dz = c.g.dims[2];
     const bool lanes = lane_sweeps_ok(c.g);
@@ -561,9 +569,12 @@ int run_fsm_rounds(Ctx& c, int64_t* far_out) {
         rc = audit(c, &far, cur, &cur_l);
         if (rc) return rc;
     }
-    if (cur != c.d)
+    if (keep) {
+        *keep = FsmField{cur, cur_l};
+    } else if (cur != c.d) {
         HIP_TRY(launch_copy_rows(cur, cur_l.pitch, cur_l.plane, c.d, dense.pitch, dense.plane, (int)dx, c.g.dims[1],
                                  (int)dz, c.s));
+    }
     c.st->eikonal_rounds = rounds + extra;
     c.st->extra_rounds = extra;
     *far_out = far;
@@ -589,6 +600,29 @@ int run_fsm(Ctx& c, const uint8_t* sign) {
     const int rc = run_fsm_rounds(c, &far);
     if (rc) return rc;
     return run_fsm_clamp(c, sign, far);
+}
+
+// [D3] and [F] after run_fsm_rounds kept its result in field f.  With voxels
+// left at the sentinel, f goes back into the dense field and the clamp and the
+// apply run there; otherwise (the clamp has nothing to do) the write-back
+// applies the signs on the way: one pass over the field instead of two.
+// mark() runs on the stream just before the apply.
+template <typename Mark>
+int run_fsm_finish(Ctx& c, const uint8_t* sign, int64_t far, const FsmField& f, Mark&& mark) {
+    const FieldLayout dense = dense_xy(c.g);
+    const bool fused = f.d != c.d && far == 0;
+    if (f.d != c.d && !fused)
+        HIP_TRY(launch_copy_rows(f.d, f.l.pitch, f.l.plane, c.d, dense.pitch, dense.plane, c.g.dims[0], c.g.dims[1],
+                                 c.g.dims[2], c.s));
+    const int rc = run_fsm_clamp(c, sign, far);
+    if (rc) return rc;
+    mark();
+    if (fused)
+        HIP_TRY(launch_copy_apply_rows(f.d, f.l.pitch, f.l.plane, c.d, sign, c.g.dims[0], c.g.dims[1], c.g.dims[2],
+                                       c.s));
+    else
+        HIP_TRY(launch_apply(c.d, sign, c.g.n(), c.s));
+    return RHZ_OK;
 }
 
 bool int_range_ok(const Grid& g, int64_t nt) {
@@ -775,11 +809,15 @@ int run_plain_exact(Ctx& c) {
     c.st->reprobed_voxels = V;
     tm.mark(2, c.s);
     tm.mark(3, c.s);
-    rc = run_fsm(c, sign);
+    FsmField fld{};
+    int64_t far = 0;
+    rc = run_fsm_rounds(c, &far, &fld);
     if (rc) return rc;
-    tm.mark(4, c.s);
-    tm.mark(5, c.s);
-    HIP_TRY(launch_apply(c.d, sign, V, c.s));
+    rc = run_fsm_finish(c, sign, far, fld, [&] {
+        tm.mark(4, c.s);
+        tm.mark(5, c.s);
+    });
+    if (rc) return rc;
     tm.mark(6, c.s);
     HIP_TRY(hipStreamSynchronize(c.s));
     c.st->ms_splat = tm.ms(0, 1);
@@ -1341,10 +1379,12 @@ int run_learned(Ctx& c) {
         if (rc) return rc;
         rc = learned_sign_stage(c, sign, b, bits, cand, n_cand, tm, 4, 5);
         if (rc) return rc;
-        rc = run_fsm(c, sign);
+        FsmField fld{};
+        int64_t far = 0;
+        rc = run_fsm_rounds(c, &far, &fld);
         if (rc) return rc;
-        tm.mark(6, c.s);
-        HIP_TRY(launch_apply(c.d, sign, V, c.s));
+        rc = run_fsm_finish(c, sign, far, fld, [&] { tm.mark(6, c.s); });
+        if (rc) return rc;
         tm.mark(7, c.s);
         HIP_TRY(hipStreamSynchronize(c.s));
         st->ms_splat = tm.ms(0, 1);
@@ -1411,7 +1451,8 @@ int run_learned(Ctx& c) {
         side_work();  // no thread: the stage runs first, still on its own stream
     }
     int64_t far = 0;
-    rc = run_fsm_rounds(c, &far);
+    FsmField fld{};
+    rc = run_fsm_rounds(c, &far, &fld);
     if (side.joinable()) side.join();
     tm.mark(3, c.s);
     e = hipStreamWaitEvent(c.s, join, 0);
@@ -1421,10 +1462,8 @@ int run_learned(Ctx& c) {
     if (rc_side > 0) return rc_side;  // declined (decline_reason says why); the caller falls back
     if (rc) return rc;
     if (e != hipSuccess) return set_err(RHZ_ERR_HIP, "sign-stage join failed: %s", hipGetErrorString(e));
-    rc = run_fsm_clamp(c, sign, far);
+    rc = run_fsm_finish(c, sign, far, fld, [&] { tm.mark(4, c.s); });
     if (rc) return rc;
-    tm.mark(4, c.s);
-    HIP_TRY(launch_apply(c.d, sign, V, c.s));
     tm.mark(5, c.s);
     HIP_TRY(hipStreamSynchronize(c.s));
     st->ms_splat = tm.ms(0, 1);
