@@ -133,7 +133,20 @@ struct Workspace {
     // eikonal rounds (created on first use)
     Workspace* side = nullptr;
     hipStream_t side_stream = nullptr;
+    // buffers a slot outgrew: hipFree waits for the whole device, so they are
+    // freed by ws_reap between calls, never while a call's work is queued
+    std::vector<void*> retired;
 };
+
+// Frees the buffers that w and its side workspace retired.
+void ws_reap(Workspace* w) {
+    if (!w) return;
+    for (Workspace* x : {w, w->side}) {
+        if (!x) continue;
+        for (void* p : x->retired) (void)hipFree(p);
+        x->retired.clear();
+    }
+}
 
 std::mutex g_ws_mu;
 Workspace g_ws[64];
@@ -153,6 +166,7 @@ Workspace* workspace() {
         if (hipHostMalloc((void**)&w.h_cnt, 512, hipHostMallocDefault) != hipSuccess) return nullptr;
         w.init = true;
     }
+    ws_reap(&w);  // the previous call's retired buffers (callers hold g_ws_mu)
     return &w;
 }
 
@@ -202,21 +216,31 @@ Workspace* side_of(Workspace* w) {
 }
 
 // Returns a device buffer of at least `bytes` for `slot` (grow-only).  A slot
-// that has to grow takes at least 1.5x its old size: a bake lane meets items
-// of many sizes, and every regrowth (a free and an allocation) can stall the
-// whole device.
+// that has to grow takes at least 1.5x its old size (a bake lane meets items
+// of many sizes), and the buffer it outgrew is retired, not freed: a hipFree
+// here would wait for every stream on the device, the other bake lanes and
+// the side stream included.
 void* ws_get(Workspace* w, int slot, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (w->bytes[slot] >= bytes) return w->p[slot];
     const size_t old = w->bytes[slot];
     if (w->p[slot]) {
-        (void)hipFree(w->p[slot]);
+        w->retired.push_back(w->p[slot]);
         w->p[slot] = nullptr;
         w->bytes[slot] = 0;
     }
     if (old > 0) bytes = std::max(bytes, old + old / 2);
     size_t rounded = (bytes + 4095) & ~size_t(4095);
-    if (hipMalloc(&w->p[slot], rounded) != hipSuccess) {
+    hipError_t e = hipMalloc(&w->p[slot], rounded);
+    if (e != hipSuccess && !w->retired.empty()) {
+        // out of memory: give the retired buffers back first (hipFree waits
+        // for the device, so the work still queued on them completes)
+        (void)hipGetLastError();
+        for (void* p : w->retired) (void)hipFree(p);
+        w->retired.clear();
+        e = hipMalloc(&w->p[slot], rounded);
+    }
+    if (e != hipSuccess) {
         (void)hipGetLastError();
         w->p[slot] = nullptr;
         return nullptr;
@@ -1703,7 +1727,7 @@ int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
 
 void rhz_release_workspace(void) {
     std::lock_guard<std::mutex> lk(g_ws_mu);
-    Workspace* w = workspace();
+    Workspace* w = workspace();  // frees the retired buffers too
     if (!w) return;
     for (Workspace* x : {w, w->side}) {
         if (!x) continue;
@@ -1802,6 +1826,7 @@ int rhz_sdf_bake(const rhz_bake_item* items, int32_t n_items, const rhz_sdf_para
     for (int l = 1; l < n_lanes; ++l) th.emplace_back(worker, l);
     worker(0);
     for (auto& t : th) t.join();
+    for (Lane* L : lanes) ws_reap(&L->ws);  // every item has synchronised its lane
     int first = RHZ_OK;
     for (int i = 0; i < n_items; ++i) {
         if (stats) stats[i] = st[(size_t)i];
