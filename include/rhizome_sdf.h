@@ -190,7 +190,8 @@ int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
  *                                  ms_eikonal, and ms_total is their union. */
 int rhz_tuning_set(const char* key, int64_t value);
 int rhz_tuning_get(const char* key, int64_t* value);
-/* Frees the cached device workspace of the current device. */
+/* Frees the cached device workspaces of the current device: the one the
+ * single-lattice calls share and the batched bake lanes' own. */
 void rhz_release_workspace(void);
 
 /* Signed distance field of triangles on the lattice [min_c, max_c] (inclusive,

@@ -273,6 +273,12 @@ def device_count() -> int:
     return lib().rhz_device_count()
 
 
+def release_workspace() -> None:
+    """Frees the cached device workspaces of the current device
+    (rhz_release_workspace): the single-lattice calls' and the bake lanes'."""
+    lib().rhz_release_workspace()
+
+
 # ----------------------------------------------------------------------------
 # pipeline-level API (host buffers)
 # ----------------------------------------------------------------------------

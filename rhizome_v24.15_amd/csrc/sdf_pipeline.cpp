@@ -1725,10 +1725,9 @@ int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
     return RHZ_OK;
 }
 
-void rhz_release_workspace(void) {
-    std::lock_guard<std::mutex> lk(g_ws_mu);
-    Workspace* w = workspace();  // frees the retired buffers too
-    if (!w) return;
+// Frees every slot of w and of its side workspace, and what they retired.
+static void ws_free_slots(Workspace* w) {
+    ws_reap(w);
     for (Workspace* x : {w, w->side}) {
         if (!x) continue;
         for (int i = 0; i < S_COUNT; ++i) {
@@ -1737,6 +1736,20 @@ void rhz_release_workspace(void) {
             x->bytes[i] = 0;
         }
     }
+}
+
+void rhz_release_workspace(void) {
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        Workspace* w = workspace();
+        if (w) ws_free_slots(w);
+    }
+    // the bake lanes' workspaces (their counters and streams stay)
+    std::lock_guard<std::mutex> lk(g_lane_mu);
+    const int d = current_device();
+    if (d < 0 || d >= 64) return;
+    for (int l = 0; l < kMaxLanes; ++l)
+        if (g_lanes[d][l].ready) ws_free_slots(&g_lanes[d][l].ws);
 }
 
 int rhz_sdf_compute_device(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],

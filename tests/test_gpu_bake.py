@@ -39,6 +39,46 @@ def test_bake_matches_single_builds(rhz, geom, torch_gpu):
                 assert getattr(st, k) == getattr(rs, k), k
 
 
+def test_bake_after_growth_and_release(rhz, geom, torch_gpu):
+    """Lanes that outgrow their workspace retire the old buffers and free them
+    after the call; a batch of larger lattices after a small one, and a bake
+    after the workspaces were released, still equal the single builds."""
+    torch = torch_gpu
+    p = rhz.default_params(method=rhz.METHOD_FAST_LEARNED)
+
+    def batch(seed, side_range):
+        items, refs = [], []
+        for v, f, h in geom.bake_batch(n=6, seed=seed, tri_range=(2_000, 12_000), side_range=side_range):
+            tris = rhz.build_triangles(v, f)
+            mn, mx = geom.frame_grid(v, h)
+            V = int(np.prod([mx[i] - mn[i] + 1 for i in range(3)]))
+            d_t = torch.from_numpy(tris).cuda()
+            ref = torch.empty(V, dtype=torch.float32, device="cuda")
+            rc, _ = rhz.compute_sdf_device(d_t, len(tris), h, mn, mx, ref, p, stream=torch.cuda.current_stream())
+            assert rc == rhz.RHZ_OK
+            items.append((d_t, len(tris), h, mn, mx, torch.full((V,), np.nan, dtype=torch.float32, device="cuda")))
+            refs.append(ref)
+        torch.cuda.synchronize()
+        return items, refs
+
+    def check(items, refs):
+        rc, stats, _ = rhz.bake(items, p, n_lanes=3)
+        assert rc == rhz.RHZ_OK
+        for it, ref in zip(items, refs):
+            assert torch.equal(it[5].view(torch.int32), ref.view(torch.int32))
+
+    rhz.release_workspace()
+    small = batch(91, (24, 40))
+    check(*small)
+    large = batch(92, (64, 112))
+    check(*large)
+    rhz.release_workspace()
+    for it in large[0]:
+        it[5].fill_(float("nan"))
+    check(*large)
+    check(*small)
+
+
 def test_bake_rejects_bad_lane_count(rhz, geom, torch_gpu):
     with pytest.raises(rhz.RhzError):
         rhz.bake([], rhz.default_params(), n_lanes=0)
