@@ -7,7 +7,7 @@ Workload: the vessel mesh (thick-walled cup R 40 / wall 3 / height 90 / floor
 buffers: triangles resident in HBM when the timed region starts, signed fp32
 field resident in HBM when it ends.
 
-Prints one JSON line (stdout).  --gpus N > 1 exits: this build runs on one GPU.
+Prints one JSON line (stdout).  --gpus N > 1 exits: the library runs on one GPU.
 """
 from __future__ import annotations
 
@@ -256,7 +256,7 @@ def main():
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding is a design note only)"}))
+        print(json.dumps({"error": "the library runs on one GPU; --gpus must be 1 (sharding is a design note only)"}))
         sys.exit(2)
 
     import torch
