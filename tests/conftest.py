@@ -54,7 +54,7 @@ def small_sphere(geom, orc):
 
 
 @pytest.fixture(scope="session")
-def config1(geom, orc):
+def ico5_128(geom, orc):
     """The plain-method small workload: ico^5 (20,480 tris) on 128^3 at 0.5 mm."""
     v, f = geom.perturbed_icosphere(seed=1)
     tris = orc.build_triangles(v, f)

@@ -84,7 +84,7 @@ def test_bake_rejects_bad_lane_count(rhz, geom, torch_gpu):
         rhz.bake([], rhz.default_params(), n_lanes=0)
 
 
-@pytest.mark.parametrize("case", ["small_sphere", "config1"])
+@pytest.mark.parametrize("case", ["small_sphere", "ico5_128"])
 def test_fsm_batched_rounds_and_graph_match_round_loop(rhz, torch_gpu, request, case):
     """All FSM rounds enqueued at once, and the same rounds captured and
     replayed as a hipGraph, give the round-by-round loop's field and round

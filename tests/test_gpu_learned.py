@@ -146,8 +146,8 @@ def test_learned_small_sphere(rhz, small_sphere, scorer):
     assert st.reprobed_voxels < est.reprobed_voxels
 
 
-def test_learned_config1(rhz, config1):
-    tris, h, mn, mx = config1
+def test_learned_ico5_128(rhz, ico5_128):
+    tris, h, mn, mx = ico5_128
     st, est, _ = _compare_learned(rhz, tris, h, mn, mx)
     V = int(np.prod(dims_of(mn, mx).astype(np.int64)))
     # the exact kernel probes a small fraction of what the plain path probes
@@ -180,7 +180,7 @@ def _plain_signs(rhz, tris, h, mn, mx):
     return ref.reshape(-1)
 
 
-@pytest.mark.parametrize("case", ["small_sphere", "config1", "vessel_1mm"])
+@pytest.mark.parametrize("case", ["small_sphere", "ico5_128", "vessel_1mm"])
 def test_certificate_covers_every_scanline_error(rhz, geom, torch_gpu, request, case):
     """Every voxel whose scanline sign differs from the exact winding sign is
     a candidate (band distance within the jitter reach) and is flagged by the
@@ -246,7 +246,7 @@ def test_ranked_probes_alone_can_miss(rhz, small_sphere):
     assert res[1] < res[2], res
 
 
-def test_network_gain_over_blind_probes(rhz, config1):
+def test_network_gain_over_blind_probes(rhz, ico5_128):
     """The trained proposer against the method without it.  Config: the
     perturbed icosphere on 128^3 (a shape family the proposer never saw in
     training), top 0.1% of voxels re-probed exactly, certificate off: the
@@ -256,7 +256,7 @@ def test_network_gain_over_blind_probes(rhz, config1):
     the planar-margin rule."""
     if rhz.Proposer.builtin() is None:
         pytest.skip("library built without trained proposer weights")
-    tris, h, mn, mx = config1
+    tris, h, mn, mx = ico5_128
     ref = _plain_signs(rhz, tris, h, mn, mx)
     on_surface = np.abs(ref) <= 1e-5 * h
     resid, stats = {}, {}
@@ -282,13 +282,13 @@ def test_network_gain_over_blind_probes(rhz, config1):
     assert ranked[0][2] <= ranked[1][2]
 
 
-@pytest.mark.parametrize("case", ["config1", "two_cubes", "vessel_coarse"])
+@pytest.mark.parametrize("case", ["ico5_128", "two_cubes", "vessel_coarse"])
 def test_far_field_certificate_keeps_every_decision(rhz, geom, orc, request, case):
     """The far-field certificate (clusters beyond beta x radius as bounded
     dipoles, undecided voxels to the brute-force kernel) changes no sign:
     fields are bit-identical for beta = 0 (brute force), 2, 6 and 20."""
-    if case == "config1":
-        tris, h, mn, mx = request.getfixturevalue("config1")
+    if case == "ico5_128":
+        tris, h, mn, mx = request.getfixturevalue("ico5_128")
     elif case == "two_cubes":
         v, f = geom.two_cubes()
         tris = orc.build_triangles(v, f)
@@ -315,13 +315,13 @@ def test_far_field_certificate_keeps_every_decision(rhz, geom, orc, request, cas
     assert st6.far_decided >= 0.9 * st6.reprobed_voxels
 
 
-def test_candidate_features_against_fp64(rhz, orc, torch_gpu, config1):
+def test_candidate_features_against_fp64(rhz, orc, torch_gpu, ico5_128):
     """Features 0 (nearest distance / h), 2-4 (nearest normal) and 8
     (log1p of the triangles within h) of a sample of candidates equal a
     binary64 evaluation over EVERY triangle of the mesh: the kernel's column
     lists and its z-extent reject drop no triangle these features need."""
     torch = torch_gpu
-    tris, h, mn, mx = config1
+    tris, h, mn, mx = ico5_128
     d_tris = torch.from_numpy(tris).to("cuda")
     cap = 1 << 20
     d_vox = torch.empty(cap, dtype=torch.int64, device="cuda")

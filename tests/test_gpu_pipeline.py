@@ -50,8 +50,8 @@ def test_fast_path_small_converged(rhz, orc, small_sphere):
     assert orc.fsm_converge(d.copy(), dims, h, max_rounds=1) == 1
 
 
-def test_fast_path_config1(rhz, orc, config1):
-    tris, h, mn, mx = config1
+def test_fast_path_ico5_128(rhz, orc, ico5_128):
+    tris, h, mn, mx = ico5_128
     st, ost, got, ref = _compare_fast(rhz, orc, tris, h, mn, mx, {}, {})
     print(st.log_line())
 

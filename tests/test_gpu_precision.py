@@ -1,4 +1,4 @@
-"""Precision study (config 3): the config-1 icosphere with 1% of its triangles
+"""Precision study: the ico^5 icosphere (20,480 triangles) with 1% of its triangles
 split into slivers of aspect ~1e4 (still watertight), 128^3 at 0.5 mm.
 
 * Band distances: the HIP splat (binary32 point-triangle distances) against
@@ -21,16 +21,16 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def config3(geom, orc):
+def sliver_ico5_128(geom, orc):
     v, f = geom.perturbed_icosphere(seed=1)
     v, f = geom.with_slivers(v, f, frac=0.01, eps=1e-4, seed=3)
     tris = orc.build_triangles(v, f)
     return tris, 0.5, (-64, -64, -64), (63, 63, 63)
 
 
-def test_precision_study(rhz, orc, torch_gpu, config3):
+def test_precision_study(rhz, orc, torch_gpu, sliver_ico5_128):
     torch = torch_gpu
-    tris, h, mn, mx = config3
+    tris, h, mn, mx = sliver_ico5_128
     dims = orc.dims_of(mn, mx)
     V = int(np.prod(dims.astype(np.int64)))
     d_t = torch.from_numpy(tris).cuda()
