@@ -149,10 +149,11 @@ def mug_specimen(rhz, torch, params, steps, warmup):
             "speedup_vs_published": 14489.0 / ms}
 
 
-def batched_bake(rhz, torch, params, n_meshes=64, log=None):
+def batched_bake(rhz, torch, params, n_meshes=64, log=None, fixed=None):
     """The batched bake: 64 seeded vessels (2k-500k triangles, log-uniform),
     longest lattice side 64-384 at 0.25-0.5 mm, built by rhz_sdf_bake with the
-    lane count and sweep shape the autotuner picks on a 16-mesh sample."""
+    lane count and sweep shape the autotuner picks on a 16-mesh sample (or
+    the configuration `fixed`, without tuning)."""
     import importlib
 
     geom = entry.load_geometry()
@@ -169,7 +170,11 @@ def batched_bake(rhz, torch, params, n_meshes=64, log=None):
         tris_n.append(len(tris))
         items.append((torch.from_numpy(tris).cuda(), len(tris), h, mn, mx,
                       torch.empty(V, dtype=torch.float32, device="cuda")))
-    best, table = tune.autotune_bake(rhz, torch, items, params, log=log)
+    if fixed is None:
+        best, table = tune.autotune_bake(rhz, torch, items, params, log=log)
+    else:
+        tune.apply_config(rhz, fixed)
+        best, table = dict(fixed), []
     # warm-up at the chosen shape: lanes take items in any order, so two passes
     # let every lane's workspace reach the sizes of the batch's largest items
     for _ in range(2):
@@ -240,6 +245,9 @@ def main():
     ap.add_argument("--certify", action="store_true",
                     help="also build the whole lattice with the plain exact-winding method and count sign "
                          "mismatches (minutes at 0.10 mm)")
+    ap.add_argument("--bake-only", action="store_true",
+                    help="only the batched bake, at 8 lanes with the one-line walker (no autotuning): two "
+                         "warm-up bakes and one timed bake; prints its JSON record")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 2 GiB device buffer 3 times (reference kernel for rocprofv3 --pmc "
                          "FETCH_SIZE / WRITE_SIZE scaling) and exit")
@@ -266,6 +274,12 @@ def main():
         raise SystemExit("bench.py needs a GPU")
     method = {"fast": rhz.METHOD_FAST, "learned": rhz.METHOD_FAST_LEARNED, "plain": rhz.METHOD_PLAIN_EXACT,
               "legacy": rhz.METHOD_LEGACY}[args.method]
+    if args.bake_only:
+        fixed = {"n_lanes": 8, "sweep_shape_x": 1, "sweep_shape_y": 1, "sweep_shape_z": 1, "sweep_step": 8,
+                 "fsm_batch_rounds_max_voxels": 16 << 20, "fsm_graph": 1}
+        rec = batched_bake(rhz, torch, rhz.default_params(method=method), log=log, fixed=fixed)
+        print(json.dumps(rec))
+        return
     fsm = rhz.FSM_REFERENCE if args.fsm == "reference" else rhz.FSM_CONVERGE
 
     t0 = time.perf_counter()

@@ -30,7 +30,9 @@ def _time_bake(rhz, torch, items, params, n_lanes):
     return (time.perf_counter() - t0) * 1e3
 
 
-def _apply(rhz, cfg):
+def apply_config(rhz, cfg):
+    """Sets the library's tuning keys from a configuration (its n_lanes is the
+    caller's to pass to bake)."""
     for k in ("sweep_shape_x", "sweep_shape_y", "sweep_shape_z", "sweep_step"):
         rhz.tuning_set(k, cfg[k])
     rhz.tuning_set("fsm_batch_rounds_max_voxels", cfg["fsm_batch_rounds_max_voxels"])
@@ -48,7 +50,7 @@ def autotune_bake(rhz, torch, items, params, sample=16, log=None):
     table = []
 
     def trial(c):
-        _apply(rhz, c)
+        apply_config(rhz, c)
         ms = _time_bake(rhz, torch, sub, params, c["n_lanes"])
         row = dict(c, ms=ms)
         table.append(row)
@@ -56,7 +58,7 @@ def autotune_bake(rhz, torch, items, params, sample=16, log=None):
             log(f"autotune: {row}")
         return ms
 
-    _apply(rhz, cfg)
+    apply_config(rhz, cfg)
     _time_bake(rhz, torch, sub, params, max(LANES))  # warm the lanes' workspaces
     best_ms = None
     for nl in LANES:
@@ -74,5 +76,5 @@ def autotune_bake(rhz, torch, items, params, sample=16, log=None):
         ms = trial(c)
         if ms < best_ms:
             best_ms, cfg = ms, c
-    _apply(rhz, cfg)
+    apply_config(rhz, cfg)
     return dict(cfg, ms=best_ms), table
