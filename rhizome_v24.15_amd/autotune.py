@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import time
 
-LANES = (1, 2, 4, 8, 12, 16)  # the library takes up to 16 lanes
+LANES = (1, 2, 4, 8)
 # (sweep_shape_x, sweep_shape_y, sweep_shape_z, sweep_step)
 SWEEP_SHAPES = ((14, 21, 21, 2), (14, 21, 22, 2), (12, 12, 14, 2), (1, 1, 1, 8), (14, 1, 21, 8))
 FSM_MODES = ((16 << 20, 0), (16 << 20, 1), (0, 0))  # (fsm_batch_rounds_max_voxels, fsm_graph)
