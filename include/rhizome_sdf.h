@@ -7,8 +7,10 @@
  * codes: 0 = OK, >0 = declined (the caller falls back, as the reference's
  * fast path does), <0 = error (rhz_last_error() has the message).
  *
- * Threading: calls on one device are serialised by the caller; each call
- * uses the HIP stream it is given (NULL = the legacy default stream).
+ * Threading: calls on one device that share its cached workspace are
+ * serialised by a library lock (held for the whole call, host copies
+ * included); each call uses the HIP stream it is given (NULL = the legacy
+ * default stream).
  */
 #ifndef RHIZOME_SDF_H
 #define RHIZOME_SDF_H
@@ -228,9 +230,12 @@ typedef struct rhz_bake_item {
  * one cached workspace per lane, 1..16).  stats[i] / ms_item[i] (host wall
  * time of item i, start to completion) are filled when non-NULL.  Returns 0
  * when every item succeeded; a declined item reports RHZ_DECLINED in
- * stats[i].status.  Results are identical to rhz_sdf_compute_device per item. */
+ * stats[i].status.  Results are identical to rhz_sdf_compute_device per item.
+ * Ordering: the lanes start after the work already queued on `stream`
+ * (hipStream_t, NULL = the legacy default stream), and the call returns when
+ * every item is complete. */
 int rhz_sdf_bake(const rhz_bake_item* items, int32_t n_items, const rhz_sdf_params* params, int32_t n_lanes,
-                 rhz_sdf_stats* stats, double* ms_item);
+                 rhz_sdf_stats* stats, double* ms_item, void* stream);
 
 /* Indexed mesh -> triangles, dropping |(v1-v0)x(v2-v0)|^2 <= 1e-12 and
  * out-of-range indices, in index order.  Replaces gpu_build_cuda_triangles_kernel
@@ -379,6 +384,32 @@ int rhz_stage_winding_voxels(const rhz_triangle* d_tris, int64_t n_tris, float h
                              const int32_t min_c[3], const int32_t dims[3],
                              const int64_t* d_voxels, int64_t n_voxels, int32_t precision,
                              double* d_w, void* stream);
+
+/* ---- accuracy report (no counterpart in the reference) --------------------
+ * Error of a finished signed field against binary64 ground truth at n_samples
+ * jittered stratified voxels (sample i in [i V / n, (i + 1) V / n), position
+ * from `seed`): the distance to the nearest triangle in binary64 (brute force
+ * over all triangles, lattice point as the field uses it) and the exact
+ * winding sign (compensated FP32, binary64 where ||w| - 0.5| < 1e-3).
+ * Errors are (|field| - distance) / h, in voxels; "band" samples are those
+ * whose true distance is <= band_voxels h, the rest are "far". */
+typedef struct rhz_accuracy {
+    int64_t samples;
+    int64_t band_samples;
+    int64_t far_samples;
+    int64_t unreached;         /* |field| >= 1e5 (the 1e6 marker): not in the error figures */
+    int64_t on_surface;        /* true distance <= 1e-5 h: sign not checked (w = 1/2 there) */
+    int64_t sign_mismatches;   /* field sign != exact winding sign (on-surface samples excluded) */
+    double band_max_err;       /* max | |field| - d | / h */
+    double band_rms_err;
+    double band_mean_err;      /* signed mean: < 0 where the sweeps lowered band values */
+    double far_max_err;
+    double far_rms_err;
+    double far_mean_err;       /* > 0: far distances overestimated (rounds capped) */
+} rhz_accuracy;
+int rhz_sdf_accuracy(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],
+                     const int32_t max_c[3], const float* d_field, int64_t n_samples, int32_t band_voxels,
+                     uint64_t seed, void* stream, rhz_accuracy* out);
 
 #ifdef __cplusplus
 }

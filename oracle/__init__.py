@@ -61,6 +61,14 @@ class Stats(C.Structure):
         ("patch_rounds", C.c_int),
         ("last_probe_m", C.c_int),
         ("csr_entries", C.c_int64),
+        ("ms_splat", C.c_double),
+        ("ms_parity", C.c_double),
+        ("ms_fixup", C.c_double),
+        ("ms_eikonal", C.c_double),
+        ("ms_verify", C.c_double),
+        ("ms_patch", C.c_double),
+        ("ms_apply", C.c_double),
+        ("ms_total", C.c_double),
     ]
 
     def as_dict(self):
@@ -116,6 +124,10 @@ def lib():
         L.orc_fsm_sweep.restype = C.c_int
         L.orc_fsm_sweep_db.argtypes = [_f32p, _f32p, _i32p, C.c_float, C.c_int, C.c_int]
         L.orc_fsm_sweep_db.restype = C.c_int
+        L.orc_fsm_round.argtypes = [_f32p, _i32p, C.c_float]
+        L.orc_fsm_round.restype = C.c_int
+        L.orc_winding_voxels.argtypes = [_f32p, C.c_int64, C.c_float, _i32p, _i32p, _i64p, C.c_int64, C.c_int,
+                                         _f64p]
         L.orc_fsm_sweep_inplace.argtypes = [_f32p, _i32p, C.c_float, C.c_int, C.c_int]
         L.orc_fsm_sweep_inplace.restype = C.c_int
         L.orc_fsm_sweep_octant.argtypes = [_f32p, _i32p, C.c_float, C.c_int, C.c_int, C.c_int]
@@ -211,9 +223,19 @@ def parity_signs(tris, h, mn, dims, jx=0.037, jy=0.051, max_hits=128, with_hits=
 
 
 def winding_at_voxels(tris, h, mn, dims, voxels, mode=0):
+    """Winding numbers at lattice voxels (threads over the list)."""
     t = _f32(tris).reshape(-1)
-    return np.array([lib().orc_winding_at_voxel(t, len(tris), h, _i32(mn), _i32(dims), int(v), mode)
-                     for v in np.asarray(voxels).reshape(-1)], dtype=np.float64)
+    v = np.ascontiguousarray(np.asarray(voxels).reshape(-1), dtype=np.int64)
+    out = np.empty(len(v), dtype=np.float64)
+    if len(v):
+        lib().orc_winding_voxels(t, len(tris), h, _i32(mn), _i32(dims), v, len(v), mode, out)
+    return out
+
+
+def fsm_round(d, dims, h):
+    """One round x+, x-, y+, y-, z+, z- of the pinned schedule, in place on
+    float32 ``d``; returns 1 if a voxel decreased."""
+    return lib().orc_fsm_round(d, _i32(dims), h)
 
 
 def fsm_converge(d, dims, h, max_rounds=512):

@@ -10,6 +10,7 @@
 
 #include <math.h>
 #include <pthread.h>
+#include <time.h>
 #include <stdatomic.h>
 #include <stdlib.h>
 #include <string.h>
@@ -358,8 +359,25 @@ void orc_frame_grid(const float* verts, int64_t nv, float h, int32_t min_c[3],
 /* stages                                                               */
 /* ------------------------------------------------------------------ */
 /* gpu_fill_float_kernel; cuda_rasterizer.cu:194-203. */
+typedef struct {
+    float* d;
+    const uint8_t* sign;
+    float v;
+    int intended;
+    atomic_llong count;
+} elem_ctx;
+
+static void fill_range(void* vctx, int64_t b, int64_t e, int tid) {
+    (void)tid;
+    elem_ctx* c = (elem_ctx*)vctx;
+    for (int64_t i = b; i < e; ++i) c->d[i] = c->v;
+}
+
 void orc_fill(float* d, int64_t n, float v) {
-    for (int64_t i = 0; i < n; ++i) d[i] = v;
+    elem_ctx c;
+    c.d = d;
+    c.v = v;
+    orc_parallel_for(n, 1 << 20, fill_range, &c);
 }
 
 typedef struct {
@@ -728,6 +746,30 @@ static void fixup_range(void* vctx, int64_t b, int64_t e, int tid) {
     atomic_fetch_add(&c->count, k);
 }
 
+typedef struct {
+    const orc_tri* t;
+    int64_t nt;
+    float h;
+    const int32_t* mn;
+    const int32_t* dm;
+    const int64_t* list;
+    double* w;
+    int mode;
+} wlist_ctx;
+
+static void wlist_range(void* vctx, int64_t b, int64_t e, int tid) {
+    (void)tid;
+    wlist_ctx* c = (wlist_ctx*)vctx;
+    for (int64_t i = b; i < e; ++i)
+        c->w[i] = orc_winding_at_voxel(c->t, c->nt, c->h, c->mn, c->dm, c->list[i], c->mode);
+}
+
+void orc_winding_voxels(const orc_tri* t, int64_t nt, float h, const int32_t min_c[3],
+                        const int32_t dims[3], const int64_t* list, int64_t n, int mode, double* w) {
+    wlist_ctx c = {t, nt, h, min_c, dims, list, w, mode};
+    orc_parallel_for(n, 16, wlist_range, &c);
+}
+
 /* gpu_exact_winding_fixup_kernel; cuda_rasterizer.cu:374-400. */
 int64_t orc_winding_fixup(const orc_tri* t, int64_t nt, float h,
                           const int32_t min_c[3], const int32_t dims[3], uint8_t* sign, int mode) {
@@ -757,64 +799,121 @@ int64_t orc_winding_fixup(const orc_tri* t, int64_t nt, float h,
  * neighbour behind the walker sees the value just written.  dst receives the
  * whole swept field.  The HIP sweeps implement the same schedule, so the two
  * agree bit-for-bit after any number of sweeps. */
-int orc_fsm_sweep_db(const float* src, float* dst, const int32_t dims[3], float h,
-                     int axis, int dir) {
+typedef struct {
+    const float* src;
+    float* dst;
+    int dx, dy, dz, axis, dir, ntx;
+    float h;
+    atomic_int changed;
+} sweepdb_ctx;
+
+#define ORC_SWEEP_TILE 64
+
+/* Lines are independent under this schedule (they read only src and write
+ * their own cells), so they are split over threads.  An x-sweep walks one
+ * line per group; y- and z-sweeps walk ORC_SWEEP_TILE adjacent lines of one
+ * x-row together, step by step, so every step reads contiguous rows.  The
+ * arithmetic of each cell is the same in every arrangement. */
+static void sweepdb_range(void* vctx, int64_t b, int64_t e, int tid) {
+    (void)tid;
+    sweepdb_ctx* c = (sweepdb_ctx*)vctx;
     const float S = 1e30f;
-    const int dx = dims[0], dy = dims[1], dz = dims[2];
-    const size_t xs = 1, ys = (size_t)dx, zs = (size_t)dx * (size_t)dy;
-    const int len = axis == 0 ? dx : (axis == 1 ? dy : dz);
-    const size_t as = axis == 0 ? xs : (axis == 1 ? ys : zs);
-    const int64_t nl = axis == 0 ? (int64_t)dy * dz : (axis == 1 ? (int64_t)dx * dz : (int64_t)dx * dy);
+    const int dx = c->dx, dy = c->dy, dz = c->dz, axis = c->axis, dir = c->dir;
+    const size_t ys = (size_t)dx, zs = (size_t)dx * (size_t)dy;
+    const float h = c->h;
+    const float* src = c->src;
+    float* dst = c->dst;
     int changed = 0;
-    for (int64_t line = 0; line < nl; ++line) {
-        int x = 0, y = 0, z = 0;
+    float behind[ORC_SWEEP_TILE];
+    for (int64_t g = b; g < e; ++g) {
         if (axis == 0) {
-            y = (int)(line % dy);
-            z = (int)(line / dy);
-        } else if (axis == 1) {
-            x = (int)(line % dx);
-            z = (int)(line / dx);
-        } else {
-            x = (int)(line % dx);
-            y = (int)(line / dx);
+            const int y = (int)(g % dy), z = (int)(g / dy);
+            const size_t row = (size_t)z * zs + (size_t)y * ys;
+            float bh = S; /* new value of the previous cell along the walk */
+            for (int s = 0; s < dx; ++s) {
+                const int x = dir > 0 ? s : dx - 1 - s;
+                const size_t i = row + (size_t)x;
+                const float cur = src[i];
+                const float lo = x > 0 ? (dir > 0 ? bh : src[i - 1]) : S;
+                const float hi = x < dx - 1 ? (dir > 0 ? src[i + 1] : bh) : S;
+                const float ax = fminf(lo, hi);
+                const float ay = fminf(y > 0 ? src[i - ys] : S, y < dy - 1 ? src[i + ys] : S);
+                const float az = fminf(z > 0 ? src[i - zs] : S, z < dz - 1 ? src[i + zs] : S);
+                const float u = orc_godunov(ax, ay, az, h);
+                float nv = cur;
+                if (u < cur) {
+                    nv = u;
+                    changed = 1;
+                }
+                dst[i] = nv;
+                bh = nv;
+            }
+            continue;
         }
-        float behind = S; /* new value of the previous cell along the walk */
+        const int xt = (int)(g % c->ntx), o = (int)(g / c->ntx);
+        const int x0 = xt * ORC_SWEEP_TILE;
+        const int x1 = x0 + ORC_SWEEP_TILE < dx ? x0 + ORC_SWEEP_TILE : dx;
+        const int len = axis == 1 ? dy : dz;
+        const size_t as = axis == 1 ? ys : zs;
+        int y = axis == 2 ? o : 0, z = axis == 1 ? o : 0;
+        for (int k = 0; k < x1 - x0; ++k) behind[k] = S;
         for (int s = 0; s < len; ++s) {
-            int q = dir > 0 ? s : len - 1 - s;
-            if (axis == 0) x = q;
-            else if (axis == 1) y = q;
+            const int q = dir > 0 ? s : len - 1 - s;
+            if (axis == 1) y = q;
             else z = q;
-            size_t i = (size_t)z * zs + (size_t)y * ys + (size_t)x * xs;
-            float cur = src[i];
-            /* neighbours along the swept axis: behind (new) and ahead (old) */
-            int lo_ok = q > 0, hi_ok = q < len - 1;
-            float lo = lo_ok ? (dir > 0 ? behind : src[i - as]) : S;
-            float hi = hi_ok ? (dir > 0 ? src[i + as] : behind) : S;
-            float ax, ay, az;
-            if (axis == 0) {
-                ax = fminf(lo, hi);
-                ay = fminf(y > 0 ? src[i - ys] : S, y < dy - 1 ? src[i + ys] : S);
-                az = fminf(z > 0 ? src[i - zs] : S, z < dz - 1 ? src[i + zs] : S);
-            } else if (axis == 1) {
-                ax = fminf(x > 0 ? src[i - xs] : S, x < dx - 1 ? src[i + xs] : S);
-                ay = fminf(lo, hi);
-                az = fminf(z > 0 ? src[i - zs] : S, z < dz - 1 ? src[i + zs] : S);
-            } else {
-                ax = fminf(x > 0 ? src[i - xs] : S, x < dx - 1 ? src[i + xs] : S);
-                ay = fminf(y > 0 ? src[i - ys] : S, y < dy - 1 ? src[i + ys] : S);
-                az = fminf(lo, hi);
+            const size_t row = (size_t)z * zs + (size_t)y * ys;
+            const int lo_ok = q > 0, hi_ok = q < len - 1;
+            for (int x = x0; x < x1; ++x) {
+                const size_t i = row + (size_t)x;
+                const float cur = src[i];
+                const float lo = lo_ok ? (dir > 0 ? behind[x - x0] : src[i - as]) : S;
+                const float hi = hi_ok ? (dir > 0 ? src[i + as] : behind[x - x0]) : S;
+                const float ax = fminf(x > 0 ? src[i - 1] : S, x < dx - 1 ? src[i + 1] : S);
+                float ay, az;
+                if (axis == 1) {
+                    ay = fminf(lo, hi);
+                    az = fminf(z > 0 ? src[i - zs] : S, z < dz - 1 ? src[i + zs] : S);
+                } else {
+                    ay = fminf(y > 0 ? src[i - ys] : S, y < dy - 1 ? src[i + ys] : S);
+                    az = fminf(lo, hi);
+                }
+                const float u = orc_godunov(ax, ay, az, h);
+                float nv = cur;
+                if (u < cur) {
+                    nv = u;
+                    changed = 1;
+                }
+                dst[i] = nv;
+                behind[x - x0] = nv;
             }
-            float u = orc_godunov(ax, ay, az, h);
-            float nv = cur;
-            if (u < cur) {
-                nv = u;
-                changed = 1;
-            }
-            dst[i] = nv;
-            behind = nv;
         }
     }
-    return changed;
+    if (changed) atomic_store(&c->changed, 1);
+}
+
+int orc_fsm_sweep_db(const float* src, float* dst, const int32_t dims[3], float h,
+                     int axis, int dir) {
+    sweepdb_ctx c;
+    c.src = src;
+    c.dst = dst;
+    c.dx = dims[0];
+    c.dy = dims[1];
+    c.dz = dims[2];
+    c.axis = axis;
+    c.dir = dir;
+    c.h = h;
+    c.ntx = (dims[0] + ORC_SWEEP_TILE - 1) / ORC_SWEEP_TILE;
+    atomic_init(&c.changed, 0);
+    int64_t groups;
+    int64_t chunk = 1;
+    if (axis == 0) {
+        groups = (int64_t)dims[1] * dims[2];
+        chunk = 64;
+    } else {
+        groups = (int64_t)c.ntx * (axis == 1 ? dims[2] : dims[1]);
+    }
+    orc_parallel_for(groups, chunk, sweepdb_range, &c);
+    return atomic_load(&c.changed);
 }
 
 /* In-place convenience wrapper of the same schedule. */
@@ -898,10 +997,20 @@ int orc_fsm_sweep_octant(float* g, const int32_t dims[3], float h, int sx, int s
 }
 
 /* gpu_count_far_sentinels_kernel; cuda_rasterizer.cu:523-534. */
-int64_t orc_count_far(const float* d, int64_t n) {
+static void count_far_range(void* vctx, int64_t b, int64_t e, int tid) {
+    (void)tid;
+    elem_ctx* c = (elem_ctx*)vctx;
     int64_t k = 0;
-    for (int64_t i = 0; i < n; ++i) k += (d[i] >= 1e29f);
-    return k;
+    for (int64_t i = b; i < e; ++i) k += (c->d[i] >= 1e29f);
+    atomic_fetch_add(&c->count, k);
+}
+
+int64_t orc_count_far(const float* d, int64_t n) {
+    elem_ctx c;
+    c.d = (float*)d;
+    atomic_init(&c.count, 0);
+    orc_parallel_for(n, 1 << 20, count_far_range, &c);
+    return atomic_load(&c.count);
 }
 
 /* gpu_clamp_far_sentinels_kernel; cuda_rasterizer.cu:544-556.  intended=1
@@ -943,44 +1052,71 @@ void orc_mark_patch(uint8_t* sign, const int64_t* sites, int n_sites, int r,
 }
 
 /* gpu_apply_signs_kernel; cuda_rasterizer.cu:650-661. */
-void orc_apply_signs(float* d, const uint8_t* sign, int64_t n) {
-    for (int64_t i = 0; i < n; ++i) {
-        float v = d[i];
+static void apply_range(void* vctx, int64_t b, int64_t e, int tid) {
+    (void)tid;
+    elem_ctx* c = (elem_ctx*)vctx;
+    for (int64_t i = b; i < e; ++i) {
+        float v = c->d[i];
         if (v > 1e29f) v = 1e6f;
-        d[i] = (sign[i] == 1) ? -v : v;
+        c->d[i] = (c->sign[i] == 1) ? -v : v;
     }
+}
+
+void orc_apply_signs(float* d, const uint8_t* sign, int64_t n) {
+    elem_ctx c;
+    c.d = d;
+    c.sign = sign;
+    orc_parallel_for(n, 1 << 20, apply_range, &c);
+}
+
+static double now_ms(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec * 1e3 + (double)ts.tv_nsec * 1e-6;
 }
 
 /* FSM driver: reference round structure (:1010-1063) with optional
  * run-to-convergence. */
+/* One round x+, x-, y+, y-, z+, z- through the second buffer tmp: six
+ * sweeps, so the field is back in d afterwards. */
+static int fsm_round_db(float* d, float* tmp, const int32_t dims[3], float h) {
+    int ch = 0;
+    ch |= orc_fsm_sweep_db(d, tmp, dims, h, 0, +1);
+    ch |= orc_fsm_sweep_db(tmp, d, dims, h, 0, -1);
+    ch |= orc_fsm_sweep_db(d, tmp, dims, h, 1, +1);
+    ch |= orc_fsm_sweep_db(tmp, d, dims, h, 1, -1);
+    ch |= orc_fsm_sweep_db(d, tmp, dims, h, 2, +1);
+    ch |= orc_fsm_sweep_db(tmp, d, dims, h, 2, -1);
+    return ch;
+}
+
+int orc_fsm_round(float* d, const int32_t dims[3], float h) {
+    size_t n = (size_t)dims[0] * (size_t)dims[1] * (size_t)dims[2];
+    float* tmp = (float*)malloc(sizeof(float) * n);
+    int ch = fsm_round_db(d, tmp, dims, h);
+    free(tmp);
+    return ch;
+}
+
 static void fsm_rounds(float* d, const int32_t dims[3], float h, float bw,
                        const uint8_t* sign, const orc_params* p, orc_stats* st) {
     int64_t n = (int64_t)dims[0] * dims[1] * dims[2];
+    float* tmp = (float*)malloc(sizeof(float) * (size_t)n);
     int rounds = 0;
     int cap = p->fsm_converge ? p->fsm_round_cap : p->max_rounds;
     for (int r = 0; r < cap; ++r) {
-        int ch = 0;
-        ch |= orc_fsm_sweep(d, dims, h, 0, +1);
-        ch |= orc_fsm_sweep(d, dims, h, 0, -1);
-        ch |= orc_fsm_sweep(d, dims, h, 1, +1);
-        ch |= orc_fsm_sweep(d, dims, h, 1, -1);
-        ch |= orc_fsm_sweep(d, dims, h, 2, +1);
-        ch |= orc_fsm_sweep(d, dims, h, 2, -1);
+        int ch = fsm_round_db(d, tmp, dims, h);
         rounds = r + 1;
         if (!ch) break;
     }
     int64_t resid = orc_count_far(d, n);
     int extra = 0;
     while (resid > 0 && extra < p->max_extra_rounds) {
-        orc_fsm_sweep(d, dims, h, 0, +1);
-        orc_fsm_sweep(d, dims, h, 0, -1);
-        orc_fsm_sweep(d, dims, h, 1, +1);
-        orc_fsm_sweep(d, dims, h, 1, -1);
-        orc_fsm_sweep(d, dims, h, 2, +1);
-        orc_fsm_sweep(d, dims, h, 2, -1);
+        (void)fsm_round_db(d, tmp, dims, h);
         ++extra;
         resid = orc_count_far(d, n);
     }
+    free(tmp);
     st->eikonal_rounds = rounds + extra;
     st->extra_rounds = extra;
     st->safe_clamped = 0;
@@ -1019,51 +1155,73 @@ int orc_fast_sdf(const orc_tri* t, int64_t nt, float h, const int32_t min_c[3],
         st->decline_reason = 2;
         return 1;
     }
+    double t0 = now_ms();
     float* d = (float*)malloc(sizeof(float) * (size_t)n);
     uint8_t* sg = (uint8_t*)malloc((size_t)n);
     orc_fill(d, n, 1e30f);
     orc_splat_band(t, nt, h, bw, min_c, dims, d);
+    double t1 = now_ms();
+    st->ms_splat = t1 - t0;
     st->unknown_flag = orc_parity_signs(t, nt, h, p->jitter_x, p->jitter_y, p->parity_max_hits,
                                         min_c, dims, sg, NULL);
+    double t2 = now_ms();
+    st->ms_parity = t2 - t1;
     if (st->unknown_flag) st->fixup_voxels = orc_winding_fixup(t, nt, h, min_c, dims, sg, p->winding_mode);
+    double t3 = now_ms();
+    st->ms_fixup = t3 - t2;
     fsm_rounds(d, dims, h, bw, sg, p, st);
+    double t4 = now_ms();
+    st->ms_eikonal = t4 - t3;
 
     int ok = 0;
     int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)p->verify_samples);
     int64_t* sites = (int64_t*)malloc(sizeof(int64_t) * (size_t)p->verify_samples);
+    double* wv = (double*)malloc(sizeof(double) * (size_t)p->verify_samples);
     for (int gen = 0; gen < p->verify_gens; ++gen) {
+        double tv = now_ms();
         orc_verify_indices(n, p->verify_samples, gen, idx);
+        /* every sample's winding number (in parallel), then the mismatches in
+         * sample order */
+        int64_t ns = 0;
+        for (int i = 0; i < p->verify_samples; ++i)
+            if (sg[idx[i]] != 2) idx[ns++] = idx[i];
+        orc_winding_voxels(t, nt, h, min_c, dims, idx, ns, p->winding_mode, wv);
         int m = 0;
-        for (int i = 0; i < p->verify_samples; ++i) {
-            uint8_t s = sg[idx[i]];
-            if (s == 2) continue;
-            double w = orc_winding_at_voxel(t, nt, h, min_c, dims, idx[i], p->winding_mode);
-            int inside = fabs(w) >= 0.5;
-            if ((s == 1) != inside) sites[m++] = idx[i];
+        for (int64_t i = 0; i < ns; ++i) {
+            int inside = fabs(wv[i]) >= 0.5;
+            if ((sg[idx[i]] == 1) != inside) sites[m++] = idx[i];
         }
         st->probes++;
         st->last_probe_m = m;
+        st->ms_verify += now_ms() - tv;
         if (m == 0) {
             ok = 1;
             break;
         }
         if (m > p->max_sites_per_probe) break;
         if (gen == p->verify_gens - 1) break;
+        double tp = now_ms();
         orc_mark_patch(sg, sites, m, p->patch_radius, dims);
         st->patched_voxels += orc_winding_fixup(t, nt, h, min_c, dims, sg, p->winding_mode);
+        st->ms_patch += now_ms() - tp;
         st->sites += m;
         st->patch_rounds++;
     }
     free(idx);
     free(sites);
+    free(wv);
     if (!ok) {
         free(d);
         free(sg);
         st->status = 1;
         st->decline_reason = 3;
+        st->ms_total = now_ms() - t0;
         return 1;
     }
+    double ta = now_ms();
     orc_apply_signs(d, sg, n);
+    st->ms_apply = now_ms() - ta;
+    st->ms_total = now_ms() - t0;
     memcpy(out, d, sizeof(float) * (size_t)n);
     if (sign_out) memcpy(sign_out, sg, (size_t)n);
     free(d);

@@ -116,6 +116,15 @@ typedef struct orc_stats {
     int patch_rounds;
     int last_probe_m;
     int64_t csr_entries;
+    /* wall time of each stage of orc_fast_sdf (ms) */
+    double ms_splat;           /* fill + band splat */
+    double ms_parity;
+    double ms_fixup;
+    double ms_eikonal;         /* rounds, audit, extension, clamp */
+    double ms_verify;          /* exact winding at the stratified samples */
+    double ms_patch;           /* patch boxes and their exact re-evaluation */
+    double ms_apply;
+    double ms_total;
 } orc_stats;
 
 void orc_params_default(orc_params* p);
@@ -166,11 +175,16 @@ int64_t orc_winding_fixup(const orc_tri* t, int64_t nt, float h,
 double orc_winding_at_voxel(const orc_tri* t, int64_t nt, float h,
                             const int32_t min_c[3], const int32_t dims[3],
                             int64_t v, int mode);
+/* orc_winding_at_voxel for a list of voxels (threads over the list). */
+void orc_winding_voxels(const orc_tri* t, int64_t nt, float h, const int32_t min_c[3],
+                        const int32_t dims[3], const int64_t* list, int64_t n, int mode, double* w);
 /* One directional sweep (axis 0/1/2, dir +1/-1) with the pinned schedule:
  * Gauss-Seidel along the swept line, cross-line neighbours from src.
  * Writes every voxel of dst; returns 1 if any voxel decreased. */
 int orc_fsm_sweep_db(const float* src, float* dst, const int32_t dims[3], float h,
                      int axis, int dir);
+/* One round x+, x-, y+, y-, z+, z- of that schedule, in place. */
+int orc_fsm_round(float* d, const int32_t dims[3], float h);
 /* Same schedule, in place (uses a temporary copy). */
 int orc_fsm_sweep(float* d, const int32_t dims[3], float h, int axis, int dir);
 /* Line-after-line in-place variant (a different legitimate interleaving). */

@@ -155,6 +155,27 @@ class Stats(C.Structure):
                 f" {self.far_residual}, extra rounds {self.extra_rounds}, safe-clamped {self.safe_clamped}")
 
 
+class Accuracy(C.Structure):
+    """rhz_accuracy: error of a finished field against binary64 ground truth."""
+    _fields_ = [
+        ("samples", C.c_int64),
+        ("band_samples", C.c_int64),
+        ("far_samples", C.c_int64),
+        ("unreached", C.c_int64),
+        ("on_surface", C.c_int64),
+        ("sign_mismatches", C.c_int64),
+        ("band_max_err", C.c_double),
+        ("band_rms_err", C.c_double),
+        ("band_mean_err", C.c_double),
+        ("far_max_err", C.c_double),
+        ("far_rms_err", C.c_double),
+        ("far_mean_err", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -209,7 +230,8 @@ def lib():
     L.rhz_stage_apply.argtypes = [vp, vp, i64, vp]
     L.rhz_stage_winding_voxels.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, vp, vp]
     L.rhz_sdf_bake.argtypes = [C.POINTER(BakeItem), i32, C.POINTER(Params), i32, C.POINTER(Stats),
-                               C.POINTER(C.c_double)]
+                               C.POINTER(C.c_double), vp]
+    L.rhz_sdf_accuracy.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, C.c_uint64, vp, C.POINTER(Accuracy)]
     L.rhz_tuning_set.argtypes = [C.c_char_p, i64]
     L.rhz_tuning_get.argtypes = [C.c_char_p, C.POINTER(C.c_int64)]
     L.rhz_mlp_param_count.restype = C.c_int
@@ -311,10 +333,13 @@ def compute_sdf_device(d_tris, n_tris, h, min_c, max_c, d_out, params: Params | 
     return rc, st
 
 
-def bake(items, params: Params | None = None, n_lanes: int = 4, **kw):
+def bake(items, params: Params | None = None, n_lanes: int = 4, stream=None, **kw):
     """Batched bake.  ``items``: sequence of (d_tris, n_tris, h, min_c, max_c,
-    d_out) with device tensors / pointers.  Returns (rc, [Stats], ms per item)."""
+    d_out) with device tensors / pointers.  The bake starts after the work
+    queued on ``stream`` (a torch stream or hipStream_t; None = the legacy
+    default stream, torch's default).  Returns (rc, [Stats], ms per item)."""
     p = params if params is not None else default_params(**kw)
+    s = stream.cuda_stream if stream is not None and hasattr(stream, "cuda_stream") else stream
     n = len(items)
     arr = (BakeItem * max(n, 1))()
     for i, (t, nt, h, mn, mx, out) in enumerate(items):
@@ -327,8 +352,20 @@ def bake(items, params: Params | None = None, n_lanes: int = 4, **kw):
         arr[i].d_out = _ptr(out)
     st = (Stats * max(n, 1))()
     ms = (C.c_double * max(n, 1))()
-    rc = check(lib().rhz_sdf_bake(arr, n, C.byref(p), int(n_lanes), st, ms), "rhz_sdf_bake")
+    rc = check(lib().rhz_sdf_bake(arr, n, C.byref(p), int(n_lanes), st, ms, s), "rhz_sdf_bake")
     return rc, [st[i] for i in range(n)], [ms[i] for i in range(n)]
+
+
+def accuracy(d_tris, n_tris, h, min_c, max_c, d_field, n_samples=262144, band_voxels=8, seed=1, stream=None):
+    """Error of the device field ``d_field`` against binary64 ground truth at
+    ``n_samples`` jittered stratified voxels (rhz_sdf_accuracy)."""
+    mn, mx = _np3(min_c), _np3(max_c)
+    a = Accuracy()
+    s = stream.cuda_stream if stream is not None and hasattr(stream, "cuda_stream") else stream
+    check(lib().rhz_sdf_accuracy(_ptr(d_tris), int(n_tris), float(h), mn.ctypes.data, mx.ctypes.data,
+                                 _ptr(d_field), int(n_samples), int(band_voxels), int(seed), s, C.byref(a)),
+          "rhz_sdf_accuracy")
+    return a
 
 
 def build_triangles(verts, idx):

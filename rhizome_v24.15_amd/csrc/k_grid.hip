@@ -202,6 +202,12 @@ __global__ void gather_u8_kernel(const uint8_t* __restrict__ src, const int64_t*
     if (i < n) dst[i] = src[idx[i]];
 }
 
+__global__ void gather_f32_kernel(const float* __restrict__ src, const int64_t* __restrict__ idx, int64_t n,
+                                  float* __restrict__ dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
 __global__ void scatter_decision_kernel(const double* __restrict__ w, const int64_t* __restrict__ idx,
                                         int64_t n, uint8_t* __restrict__ sign) {
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -404,6 +410,12 @@ hipError_t launch_gather_u8(const uint8_t* src, const int64_t* idx, int64_t n, u
                             hipStream_t s) {
     if (n <= 0) return hipSuccess;
     gather_u8_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(src, idx, n, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_f32(const float* src, const int64_t* idx, int64_t n, float* dst, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    gather_f32_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(src, idx, n, dst);
     return hipGetLastError();
 }
 

@@ -322,6 +322,47 @@ __global__ __launch_bounds__(kWBlock) void point_distance_kernel(const float* __
     if (live) out[i] = sqrtf(best);
 }
 
+// ---- binary64 distance to the nearest triangle (accuracy yardstick) -------------
+// d2[i] = min over all triangles of the binary64 squared distance from voxel
+// idx[i]'s lattice point (binary32 coordinates, as the field uses).  Blocks
+// split the triangles into ranges; each lane keeps its range's minimum and
+// merges it with an integer atomic min on the bit pattern (non-negative
+// binary64 values order like their bit patterns), so the result does not
+// depend on the order of the ranges.
+__global__ __launch_bounds__(kWBlock) void distance_f64_kernel(const float* __restrict__ tris, int64_t nt,
+                                                               PointSrc src, int64_t n, int64_t per_split,
+                                                               unsigned long long* __restrict__ d2) {
+    __shared__ float s[9 * kWTile];
+    const int64_t i = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
+    const int64_t t0 = (int64_t)blockIdx.y * per_split;
+    const int64_t t1 = min(nt, t0 + per_split);
+    const bool live = i < n;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (live) {
+        float fx, fy, fz;
+        src.pos(src.voxel(i), fx, fy, fz);
+        px = fx;
+        py = fy;
+        pz = fz;
+    }
+    double best = __longlong_as_double(0x7ff0000000000000ll);  // +inf
+    for (int64_t tb = t0; tb < t1; tb += kWTile) {
+        const int cnt = (int)min((int64_t)kWTile, t1 - tb);
+        __syncthreads();
+        stage_tile(tris, tb, cnt, s);
+        __syncthreads();
+        if (live) {
+            for (int k = 0; k < cnt; ++k) best = fmin(best, point_tri_dist2_f64(px, py, pz, tile_tri(s, k)));
+        }
+    }
+    if (live) atomicMin(d2 + i, (unsigned long long)__double_as_longlong(best));
+}
+
+__global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int64_t n, unsigned long long v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 inline int split_count(int64_t n, int64_t nt) {
     int64_t pblocks = (n + kWBlock - 1) / kWBlock;
     int64_t want = (kTargetBlocks + pblocks - 1) / pblocks;
@@ -394,6 +435,23 @@ hipError_t launch_legacy(const float* tris, int64_t nt, const Grid& g, float* ou
     if (ntiles > 0) legacy_tile_bbox_kernel<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(tris, nt, bb);
     PointSrc src = make_src(g, nullptr, 0);
     legacy_kernel<<<(unsigned)((n + kWBlock - 1) / kWBlock), kWBlock, 0, s>>>(tris, nt, bb, src, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_distance_f64(const float* tris, int64_t nt, const Grid& g, const int64_t* idx, int64_t n,
+                               double* d2, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    unsigned long long* u = reinterpret_cast<unsigned long long*>(d2);
+    fill_u64_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(u, n, 0x7ff0000000000000ull);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nt <= 0) return e;
+    PointSrc src = make_src(g, idx, 0);
+    int nsplit = split_count(n, nt);
+    int64_t per_split = (nt + nsplit - 1) / nsplit;
+    per_split = ((per_split + kWTile - 1) / kWTile) * kWTile;
+    nsplit = (int)((nt + per_split - 1) / per_split);
+    dim3 grid((unsigned)((n + kWBlock - 1) / kWBlock), (unsigned)nsplit);
+    distance_f64_kernel<<<grid, kWBlock, 0, s>>>(tris, nt, src, n, per_split, u);
     return hipGetLastError();
 }
 

@@ -84,6 +84,52 @@ __device__ __forceinline__ float point_tri_dist2(float px, float py, float pz, c
     return dot3(qx, qy, qz, qx, qy, qz);
 }
 
+// The same closest-point regions in binary64 (the yardstick of the accuracy
+// report; Ericson, Real-Time Collision Detection, 5.1.5).
+__device__ __forceinline__ double point_tri_dist2_f64(double px, double py, double pz, const Tri9& t) {
+    const double ax = t.ax, ay = t.ay, az = t.az;
+    const double abx = (double)t.bx - ax, aby = (double)t.by - ay, abz = (double)t.bz - az;
+    const double acx = (double)t.cx - ax, acy = (double)t.cy - ay, acz = (double)t.cz - az;
+    const double apx = px - ax, apy = py - ay, apz = pz - az;
+    const double d1 = abx * apx + aby * apy + abz * apz;
+    const double d2 = acx * apx + acy * apy + acz * apz;
+    if (d1 <= 0.0 && d2 <= 0.0) return apx * apx + apy * apy + apz * apz;
+    const double bpx = px - t.bx, bpy = py - t.by, bpz = pz - t.bz;
+    const double d3 = abx * bpx + aby * bpy + abz * bpz;
+    const double d4 = acx * bpx + acy * bpy + acz * bpz;
+    if (d3 >= 0.0 && d4 <= d3) return bpx * bpx + bpy * bpy + bpz * bpz;
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        const double v = d1 / (d1 - d3);
+        const double qx = apx - v * abx, qy = apy - v * aby, qz = apz - v * abz;
+        return qx * qx + qy * qy + qz * qz;
+    }
+    const double cpx = px - t.cx, cpy = py - t.cy, cpz = pz - t.cz;
+    const double d5 = abx * cpx + aby * cpy + abz * cpz;
+    const double d6 = acx * cpx + acy * cpy + acz * cpz;
+    if (d6 >= 0.0 && d5 <= d6) return cpx * cpx + cpy * cpy + cpz * cpz;
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        const double w = d2 / (d2 - d6);
+        const double qx = apx - w * acx, qy = apy - w * acy, qz = apz - w * acz;
+        return qx * qx + qy * qy + qz * qz;
+    }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        const double qx = bpx - w * ((double)t.cx - t.bx);
+        const double qy = bpy - w * ((double)t.cy - t.by);
+        const double qz = bpz - w * ((double)t.cz - t.bz);
+        return qx * qx + qy * qy + qz * qz;
+    }
+    const double den = 1.0 / (va + vb + vc);
+    const double v = vb * den, w = vc * den;
+    const double qx = apx - abx * v - acx * w;
+    const double qy = apy - aby * v - acy * w;
+    const double qz = apz - abz * v - acz * w;
+    return qx * qx + qy * qy + qz * qz;
+}
+
 // atan2 in FP32 for the winding sums: octant reduction to t = min/max in
 // [0, 1], then atan(t) = t * P(t^2) with a degree-7 polynomial fitted to
 // atan(t)/t on [0, 1] by iteratively reweighted least squares (Lawson's

@@ -43,6 +43,7 @@ hipError_t launch_mark_patch(uint8_t* sign, const int64_t* sites, int n_sites, i
 hipError_t launch_verify_indices(int64_t total, int samples, int gen, int64_t* out, hipStream_t s);
 hipError_t launch_gather_u8(const uint8_t* src, const int64_t* idx, int64_t n, uint8_t* dst,
                             hipStream_t s);
+hipError_t launch_gather_f32(const float* src, const int64_t* idx, int64_t n, float* dst, hipStream_t s);
 hipError_t launch_scatter_decision(const double* w, const int64_t* idx, int64_t n, uint8_t* sign,
                                    hipStream_t s);
 // sign[base + i] = |w[i]| >= 0.5
@@ -155,6 +156,10 @@ hipError_t launch_winding(const float* tris, int64_t nt, const Grid& g, const in
 hipError_t launch_winding_fp64(const float* tris, int64_t nt, const Grid& g, const int64_t* idx,
                                int64_t base, const int64_t* fp64_list, int64_t n_fp64, double* w,
                                hipStream_t s);
+// Binary64 squared distance from lattice voxels idx[0..n) to the nearest of
+// the nt triangles (brute force; +inf when nt = 0).
+hipError_t launch_distance_f64(const float* tris, int64_t nt, const Grid& g, const int64_t* idx, int64_t n,
+                               double* d2, hipStream_t s);
 // Legacy exact per-voxel kernel (distance + winding sign).
 hipError_t launch_legacy(const float* tris, int64_t nt, const Grid& g, float* out, void* scratch,
                          hipStream_t s);

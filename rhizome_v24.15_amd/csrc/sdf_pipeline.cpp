@@ -118,6 +118,7 @@ enum Slot {
     S_MC_VERTS,    // vertices / triangles staged by the host API
     S_MC_TRIS,
     S_DIST3,       // second FSM working buffer (rows padded to 128 B)
+    S_ACC,         // accuracy report: field values at the samples
     S_COUNT
 };
 
@@ -229,6 +230,7 @@ void* ws_get(Workspace* w, int slot, size_t bytes) {
         w->p[slot] = nullptr;
         w->bytes[slot] = 0;
     }
+    const size_t exact = (bytes + 4095) & ~size_t(4095);
     if (old > 0) bytes = std::max(bytes, old + old / 2);
     size_t rounded = (bytes + 4095) & ~size_t(4095);
     hipError_t e = hipMalloc(&w->p[slot], rounded);
@@ -238,6 +240,12 @@ void* ws_get(Workspace* w, int slot, size_t bytes) {
         (void)hipGetLastError();
         for (void* p : w->retired) (void)hipFree(p);
         w->retired.clear();
+        e = hipMalloc(&w->p[slot], rounded);
+    }
+    if (e != hipSuccess && rounded > exact) {
+        // the 1.5x growth margin does not fit: take exactly what was asked
+        (void)hipGetLastError();
+        rounded = exact;
         e = hipMalloc(&w->p[slot], rounded);
     }
     if (e != hipSuccess) {
@@ -1752,6 +1760,24 @@ void rhz_release_workspace(void) {
         if (g_lanes[d][l].ready) ws_free_slots(&g_lanes[d][l].ws);
 }
 
+// The body of rhz_sdf_compute_device; the caller holds g_ws_mu.
+static int compute_device_locked(const rhz_triangle* d_tris, int64_t n_tris, const Grid& g,
+                                 const rhz_sdf_params* params, float* d_out, hipStream_t stream,
+                                 rhz_sdf_stats* st, std::chrono::steady_clock::time_point t0) {
+    std::memset(st, 0, sizeof(*st));
+    for (int j = 0; j < 3; ++j) st->dims[j] = g.dims[j];
+    st->n_triangles = n_tris;
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    if (n_tris == 0) return set_err(RHZ_ERR_ARG, "no triangles");
+    Ctx c{reinterpret_cast<const float*>(d_tris), n_tris, g, params, d_out, stream, st, w};
+    int rc = dispatch(c);
+    st->status = rc;
+    st->ms_host_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
 int rhz_sdf_compute_device(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],
                            const int32_t max_c[3], const rhz_sdf_params* params, float* d_out, void* stream,
                            rhz_sdf_stats* stats) {
@@ -1765,24 +1791,12 @@ int rhz_sdf_compute_device(const rhz_triangle* d_tris, int64_t n_tris, float h, 
     if (rc) return rc;
     if (n_tris < 0 || (n_tris > 0 && !d_tris) || !d_out) return set_err(RHZ_ERR_ARG, "null buffer");
     rhz_sdf_stats local;
-    rhz_sdf_stats* st = stats ? stats : &local;
-    std::memset(st, 0, sizeof(*st));
-    for (int j = 0; j < 3; ++j) st->dims[j] = g.dims[j];
-    st->n_triangles = n_tris;
     std::lock_guard<std::mutex> lk(g_ws_mu);
-    Workspace* w = workspace();
-    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
-    Ctx c{reinterpret_cast<const float*>(d_tris), n_tris, g, params, d_out, (hipStream_t)stream, st, w};
-    if (n_tris == 0) return set_err(RHZ_ERR_ARG, "no triangles");
-    rc = dispatch(c);
-    st->status = rc;
-    st->ms_host_total =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return rc;
+    return compute_device_locked(d_tris, n_tris, g, params, d_out, (hipStream_t)stream, stats ? stats : &local, t0);
 }
 
 int rhz_sdf_bake(const rhz_bake_item* items, int32_t n_items, const rhz_sdf_params* params, int32_t n_lanes,
-                 rhz_sdf_stats* stats, double* ms_item) {
+                 rhz_sdf_stats* stats, double* ms_item, void* stream) {
     int rc = need_device();
     if (rc) return rc;
     rc = check_params(params);
@@ -1808,6 +1822,17 @@ int rhz_sdf_bake(const rhz_bake_item* items, int32_t n_items, const rhz_sdf_para
         lanes[(size_t)l] = bake_lane(l);
         if (!lanes[(size_t)l]) return set_err(RHZ_ERR_NOMEM, "bake lane %d initialisation failed", l);
     }
+    // Every lane starts after the work the caller queued on `stream` (its
+    // inputs may still be in flight there); the call returns only when every
+    // item is complete, so work queued on `stream` afterwards follows the bake.
+    hipEvent_t entry = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&entry, hipEventDisableTiming));
+    struct EventGuard {
+        hipEvent_t e;
+        ~EventGuard() { (void)hipEventDestroy(e); }
+    } entry_guard{entry};
+    HIP_TRY(hipEventRecord(entry, (hipStream_t)stream));
+    for (Lane* L : lanes) HIP_TRY(hipStreamWaitEvent(L->stream, entry, 0));
     std::vector<rhz_sdf_stats> st((size_t)n_items);
     std::vector<double> lat((size_t)n_items, 0.0);
     std::vector<int> rcs((size_t)n_items, RHZ_OK);
@@ -1863,19 +1888,17 @@ int rhz_sdf_compute(const rhz_triangle* tris, int64_t n_tris, float h, const int
     if (n_tris <= 0 || !tris || !out_host) return set_err(RHZ_ERR_ARG, "null or empty buffer");
     rhz_sdf_stats local;
     rhz_sdf_stats* st = stats ? stats : &local;
-    float* d_tris = nullptr;
-    float* d_out = nullptr;
-    {
-        std::lock_guard<std::mutex> lk(g_ws_mu);
-        Workspace* w = workspace();
-        if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
-        d_tris = (float*)ws_get(w, S_TRIS, sizeof(rhz_triangle) * (size_t)n_tris);
-        d_out = (float*)ws_get(w, S_OUT, sizeof(float) * (size_t)g.n());
-        if (!d_tris || !d_out) return set_err(RHZ_ERR_NOMEM, "device buffer allocation failed");
-    }
+    // One lock over staging, compute and the copy back: the staged slots are
+    // shared with every other host entry point of this device.
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    float* d_tris = (float*)ws_get(w, S_TRIS, sizeof(rhz_triangle) * (size_t)n_tris);
+    float* d_out = (float*)ws_get(w, S_OUT, sizeof(float) * (size_t)g.n());
+    if (!d_tris || !d_out) return set_err(RHZ_ERR_NOMEM, "device buffer allocation failed");
     HIP_TRY(hipMemcpy(d_tris, tris, sizeof(rhz_triangle) * (size_t)n_tris, hipMemcpyHostToDevice));
-    rc = rhz_sdf_compute_device(reinterpret_cast<const rhz_triangle*>(d_tris), n_tris, h, min_c, max_c, params,
-                                d_out, nullptr, st);
+    rc = compute_device_locked(reinterpret_cast<const rhz_triangle*>(d_tris), n_tris, g, params, d_out, nullptr, st,
+                               t0);
     if (rc == RHZ_OK) HIP_TRY(hipMemcpy(out_host, d_out, sizeof(float) * (size_t)g.n(), hipMemcpyDeviceToHost));
     st->ms_host_total =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2349,6 +2372,109 @@ int rhz_stage_winding_voxels(const rhz_triangle* d_tris, int64_t n_tris, float h
     return RHZ_OK;
 }
 
+// Jittered stratified voxel ids: sample i lies in [i V / n, (i + 1) V / n),
+// at a position drawn from a splitmix64 hash of (seed, i).
+static void accuracy_sample_ids(int64_t V, int64_t n, uint64_t seed, std::vector<int64_t>& ids) {
+    ids.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+        int64_t v = (int64_t)(((double)i + u) * (double)V / (double)n);
+        ids[(size_t)i] = std::min<int64_t>(std::max<int64_t>(v, 0), V - 1);
+    }
+}
+
+int rhz_sdf_accuracy(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],
+                     const int32_t max_c[3], const float* d_field, int64_t n_samples, int32_t band_voxels,
+                     uint64_t seed, void* stream, rhz_accuracy* out) {
+    int rc = need_device();
+    if (rc) return rc;
+    Grid g;
+    rc = make_grid(min_c, max_c, h, &g);
+    if (rc) return rc;
+    if (!out || !d_field || n_tris <= 0 || !d_tris) return set_err(RHZ_ERR_ARG, "null or empty buffer");
+    if (n_samples < 1 || n_samples > (int64_t)1 << 26) return set_err(RHZ_ERR_ARG, "n_samples must be in [1, 2^26]");
+    if (band_voxels < 0) return set_err(RHZ_ERR_ARG, "band_voxels must be >= 0");
+    const int64_t n = std::min<int64_t>(n_samples, g.n());
+    std::vector<int64_t> ids;
+    accuracy_sample_ids(g.n(), n, seed, ids);
+    rhz_sdf_params p;
+    rhz_sdf_params_default(&p);
+    p.winding_precision = RHZ_WINDING_FP32_KAHAN;
+    rhz_sdf_stats st;
+    std::memset(&st, 0, sizeof(st));
+    std::vector<float> f((size_t)n);
+    std::vector<double> d2((size_t)n), wv((size_t)n);
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        Workspace* w = workspace();
+        if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+        Ctx c{reinterpret_cast<const float*>(d_tris), n_tris, g, &p, nullptr, (hipStream_t)stream, &st, w};
+        int64_t* d_ids = (int64_t*)ws_get(w, S_SAMPLES, sizeof(int64_t) * (size_t)n);
+        double* d_w = (double*)ws_get(w, S_MISC, sizeof(double) * (size_t)n);
+        double* d_d2 = (double*)ws_get(w, S_MISC2, sizeof(double) * (size_t)n);
+        float* d_f = (float*)ws_get(w, S_ACC, sizeof(float) * (size_t)n);
+        void* scr = ws_get(w, S_WSCR, winding_scratch_bytes(n, n_tris));
+        int64_t* f64 = (int64_t*)ws_get(w, S_FP64, sizeof(int64_t) * (size_t)n);
+        if (!d_ids || !d_w || !d_d2 || !d_f || !scr || !f64) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+        HIP_TRY(hipMemcpyAsync(d_ids, ids.data(), sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, c.s));
+        HIP_TRY(launch_gather_f32(d_field, d_ids, n, d_f, c.s));
+        HIP_TRY(launch_distance_f64(c.tris, n_tris, g, d_ids, n, d_d2, c.s));
+        rc = zero_counters(c);
+        if (rc) return rc;
+        HIP_TRY(launch_winding(c.tris, n_tris, g, d_ids, 0, n, p.winding_precision, p.fp64_band, d_w, scr, f64,
+                               cnt_ptr(w, C_FP64), c.s));
+        rc = read_counters(c);
+        if (rc) return rc;
+        HIP_TRY(launch_winding_fp64(c.tris, n_tris, g, d_ids, 0, f64, (int64_t)hc(c, C_FP64), d_w, c.s));
+        HIP_TRY(hipMemcpyAsync(f.data(), d_f, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c.s));
+        HIP_TRY(hipMemcpyAsync(d2.data(), d_d2, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c.s));
+        HIP_TRY(hipMemcpyAsync(wv.data(), d_w, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c.s));
+        HIP_TRY(hipStreamSynchronize(c.s));
+    }
+    std::memset(out, 0, sizeof(*out));
+    out->samples = n;
+    const double hd = (double)h, band = (double)band_voxels * (double)h;
+    double bs = 0, bq = 0, fs = 0, fq = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double truth = std::sqrt(d2[(size_t)i]);
+        const float v = f[(size_t)i];
+        if (truth <= 1e-5 * hd) {
+            ++out->on_surface;  // w is 1/2 to rounding there: the sign is not checked
+        } else if (std::signbit(v) != (std::fabs(wv[(size_t)i]) >= 0.5)) {
+            ++out->sign_mismatches;
+        }
+        const double mag = std::fabs((double)v);
+        if (!(mag < 1e5)) {  // the unreached marker (or NaN) is not a distance
+            ++out->unreached;
+            continue;
+        }
+        const double e = (mag - truth) / hd;
+        if (truth <= band) {
+            ++out->band_samples;
+            bs += e;
+            bq += e * e;
+            out->band_max_err = std::max(out->band_max_err, std::fabs(e));
+        } else {
+            ++out->far_samples;
+            fs += e;
+            fq += e * e;
+            out->far_max_err = std::max(out->far_max_err, std::fabs(e));
+        }
+    }
+    if (out->band_samples) {
+        out->band_mean_err = bs / (double)out->band_samples;
+        out->band_rms_err = std::sqrt(bq / (double)out->band_samples);
+    }
+    if (out->far_samples) {
+        out->far_mean_err = fs / (double)out->far_samples;
+        out->far_rms_err = std::sqrt(fq / (double)out->far_samples);
+    }
+    return RHZ_OK;
+}
 
 int rhz_mlp_param_count(void) { return mlp_param_count(); }
 int rhz_mlp_blob_count(void) { return mlp_blob_count(); }

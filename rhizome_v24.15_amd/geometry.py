@@ -218,8 +218,10 @@ def vessel(seed: int = 2, step: float = 0.84, jitter_mm: float = 0.02,
     return _as_mesh(jitter(v, jitter_mm, seed), f)
 
 
-def random_vessel(seed: int, target_tris: int):
-    """Perturbed cup and/or torus of about ``target_tris`` triangles."""
+def random_vessel(seed: int, target_tris: int, tri_bounds=None):
+    """Perturbed cup and/or torus of about ``target_tris`` triangles.  With
+    ``tri_bounds = (lo, hi)`` the edge step is corrected (up to four times)
+    until the count lies in [lo, hi]."""
     rng = np.random.default_rng(seed)
     kind = rng.integers(0, 3)  # 0 cup+torus, 1 cup, 2 torus pair
     outer = rng.uniform(15.0, 40.0)
@@ -235,16 +237,31 @@ def random_vessel(seed: int, target_tris: int):
     if kind in (0, 2):
         area += 4 * math.pi ** 2 * Rt * rt * (1 if kind == 0 else 2)
     step = math.sqrt(2.0 * area / max(target_tris, 100))
-    parts = []
-    if kind in (0, 1):
-        parts.append(cup(outer, wall, height, floor, step=step))
-    if kind == 0:
-        parts.append(torus(Rt, rt, center=(outer * rng.uniform(0.4, 1.0), 0.0, height * 0.5),
-                           axis="y", step=step))
-    if kind == 2:
-        parts.append(torus(Rt, rt, center=(0.0, 0.0, 0.0), axis="z", step=step))
-        parts.append(torus(Rt, rt, center=(Rt, 0.0, 0.0), axis="y", step=step))
-    v, f = merge(*parts)
+    handle_x = outer * rng.uniform(0.4, 1.0) if kind == 0 else 0.0
+
+    def build(step):
+        parts = []
+        if kind in (0, 1):
+            parts.append(cup(outer, wall, height, floor, step=step))
+        if kind == 0:
+            parts.append(torus(Rt, rt, center=(handle_x, 0.0, height * 0.5), axis="y", step=step))
+        if kind == 2:
+            parts.append(torus(Rt, rt, center=(0.0, 0.0, 0.0), axis="z", step=step))
+            parts.append(torus(Rt, rt, center=(Rt, 0.0, 0.0), axis="y", step=step))
+        return merge(*parts)
+
+    v, f = build(step)
+    if tri_bounds is not None:
+        lo, hi = tri_bounds
+        want = min(max(target_tris, lo), hi)
+        for _ in range(4):
+            if lo <= len(f) <= hi:
+                break
+            # triangle count ~ 1 / step^2
+            step *= math.sqrt(len(f) / (0.97 * want if len(f) > hi else 1.03 * want))
+            v, f = build(step)
+        if not lo <= len(f) <= hi:
+            raise RuntimeError(f"random_vessel({seed}): {len(f)} triangles outside {tri_bounds}")
     v = v.astype(np.float64)
     # low-frequency radial wobble + jitter
     k = rng.uniform(0.5, 2.0)
@@ -268,7 +285,7 @@ def bake_batch(n: int = 64, seed: int = 5000, tri_range=(2_000, 500_000), side_r
         tri = int(math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1]))))
         side = int(rng.integers(side_range[0], side_range[1] + 1))
         h = float(np.float32(rng.uniform(*h_range)))
-        v, f = random_vessel(seed + i, tri)
+        v, f = random_vessel(seed + i, tri, tri_bounds=tri_range)
         ext = float(np.max(v.max(0) - v.min(0)))
         scale = (side - 22) * h / ext  # frame_grid pads 10 h on each side
         c = (v.max(0) + v.min(0)) / 2

@@ -75,6 +75,7 @@ def test_bake_after_growth_and_release(rhz, geom, torch_gpu):
     rhz.release_workspace()
     for it in large[0]:
         it[5].fill_(float("nan"))
+    torch.cuda.synchronize()
     check(*large)
     check(*small)
 
