@@ -115,3 +115,49 @@ def test_fsm_batched_rounds_and_graph_match_round_loop(rhz, torch_gpu, request, 
     for o in outs[1:]:
         assert torch.equal(o.view(torch.int32), outs[0].view(torch.int32))
     assert len(set(rounds)) == 1, rounds
+
+
+@pytest.mark.timeout(600)
+def test_full_batch_matches_single_builds_and_winding(rhz, geom, torch_gpu):
+    """The full batched-bake workload: 64 seeded vessels (2k-500k triangles,
+    longest side 64-384 at 0.25-0.5 mm; lattices of >= 16M voxels run the
+    overlapped sign stage inside their lane).  Every field equals its single
+    build bit for bit, and at every candidate voxel of every item (the only
+    voxels whose scanline sign can be wrong) the sign equals the brute-force
+    exact winding sign (on-surface voxels within 1e-5 voxel exempt)."""
+    torch = torch_gpu
+    p = rhz.default_params(method=rhz.METHOD_FAST_LEARNED)
+    items, refs, total = [], [], 0
+    for v, f, h in geom.bake_batch(n=64):
+        tris = rhz.build_triangles(v, f)
+        assert 2_000 <= len(f) <= 500_000
+        mn, mx = geom.frame_grid(v, h)
+        V = int(np.prod([mx[i] - mn[i] + 1 for i in range(3)]))
+        total += V
+        d_t = torch.from_numpy(tris).cuda()
+        ref = torch.empty(V, dtype=torch.float32, device="cuda")
+        rc, _ = rhz.compute_sdf_device(d_t, len(tris), h, mn, mx, ref, p)
+        assert rc == rhz.RHZ_OK
+        items.append((d_t, len(tris), h, mn, mx, torch.full((V,), np.nan, dtype=torch.float32, device="cuda")))
+        refs.append(ref)
+    torch.cuda.synchronize()
+    assert max(it[5].numel() for it in items) >= 16 << 20
+    rc, stats, ms = rhz.bake(items, p, n_lanes=8)
+    assert rc == rhz.RHZ_OK
+    print(f"bake: {len(items)} items, {total} voxels, slowest item {max(ms):.1f} ms", flush=True)
+    cap = 1 << 22
+    d_vox = torch.empty(cap, dtype=torch.int64, device="cuda")
+    checked = 0
+    for k, ((d_t, nt, h, mn, mx, out), ref, st) in enumerate(zip(items, refs, stats)):
+        assert st.status == rhz.RHZ_OK
+        assert torch.equal(out.view(torch.int32), ref.view(torch.int32)), k
+        dims = [mx[i] - mn[i] + 1 for i in range(3)]
+        n = rhz.Proposer.candidates(d_t, nt, h, mn, mx, d_vox, None, cap, params=p)
+        assert 0 < n <= cap
+        w = torch.empty(n, dtype=torch.float64, device="cuda")
+        rhz.Stages.winding_voxels(d_t, nt, h, mn, dims, d_vox, n, w)
+        fv = out[d_vox[:n]]
+        wrong = (torch.signbit(fv) != (w.abs() >= 0.5)) & (fv.abs() > 1e-5 * h)
+        assert int(wrong.sum()) == 0, (k, int(wrong.sum()))
+        checked += n
+    print(f"candidate voxels checked against brute-force winding: {checked}", flush=True)
