@@ -385,6 +385,20 @@ int rhz_stage_winding_voxels(const rhz_triangle* d_tris, int64_t n_tris, float h
                              const int64_t* d_voxels, int64_t n_voxels, int32_t precision,
                              double* d_w, void* stream);
 
+/* rhz_stage_winding_voxels without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * Same kernels and results; the scratch (recompute counter, binary64
+ * recompute list, split partials) is a fixed 24 MiB + 256 B device array per
+ * GPU, held until the process ends, so n_voxels <= 2^20 (larger calls return
+ * RHZ_ERR_ARG).  The work is enqueued on `stream` and the call returns
+ * without waiting.  Calls that overlap on one GPU share that array and race:
+ * order them (one stream). */
+int rhz_stage_winding_voxels_noalloc(const rhz_triangle* d_tris, int64_t n_tris, float h,
+                                     const int32_t min_c[3], const int32_t dims[3],
+                                     const int64_t* d_voxels, int64_t n_voxels, int32_t precision,
+                                     double* d_w, void* stream);
+
 /* ---- accuracy report (no counterpart in the reference) --------------------
  * Error of a finished signed field against binary64 ground truth at n_samples
  * jittered stratified voxels (sample i in [i V / n, (i + 1) V / n), position

@@ -231,6 +231,7 @@ def lib():
     L.rhz_stage_winding_voxels.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, vp, vp]
     L.rhz_sdf_bake.argtypes = [C.POINTER(BakeItem), i32, C.POINTER(Params), i32, C.POINTER(Stats),
                                C.POINTER(C.c_double), vp]
+    L.rhz_stage_winding_voxels_noalloc.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, vp, vp]
     L.rhz_sdf_accuracy.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, C.c_uint64, vp, C.POINTER(Accuracy)]
     L.rhz_tuning_set.argtypes = [C.c_char_p, i64]
     L.rhz_tuning_get.argtypes = [C.c_char_p, C.POINTER(C.c_int64)]
@@ -561,6 +562,16 @@ class Stages:
         check(lib().rhz_stage_winding_voxels(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c),
                                              _i32x3(dims), _ptr(d_voxels), int(n_voxels),
                                              int(precision), _ptr(d_w), stream), "rhz_stage_winding_voxels")
+
+    @staticmethod
+    def winding_voxels_noalloc(d_tris, n_tris, h, min_c, dims, d_voxels, n_voxels, d_w,
+                               precision=WINDING_FP32_KAHAN, stream=None):
+        """rhz_stage_winding_voxels_noalloc: no device allocation, at most
+        2^20 voxels, enqueued on ``stream`` without waiting."""
+        s = stream.cuda_stream if stream is not None and hasattr(stream, "cuda_stream") else stream
+        check(lib().rhz_stage_winding_voxels_noalloc(_ptr(d_tris), int(n_tris), float(h), _i32x3(min_c),
+                                                     _i32x3(dims), _ptr(d_voxels), int(n_voxels), int(precision),
+                                                     _ptr(d_w), s), "rhz_stage_winding_voxels_noalloc")
 
 
 # ----------------------------------------------------------------------------

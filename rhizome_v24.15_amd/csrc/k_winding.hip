@@ -159,6 +159,23 @@ __global__ __launch_bounds__(kWBlock) void winding_fp64_kernel(const float* __re
     w[i] = sum / (4.0 * 3.14159265358979323846);
 }
 
+// The same recompute with the list length read on the device (no host round
+// trip): the grid covers the list's capacity and lanes past the count exit.
+__global__ __launch_bounds__(kWBlock) void winding_fp64_count_kernel(const float* __restrict__ tris, int64_t nt,
+                                                                     PointSrc src, const int64_t* __restrict__ list,
+                                                                     const unsigned long long* __restrict__ count,
+                                                                     double* __restrict__ w) {
+    const int64_t j = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
+    if ((unsigned long long)j >= *count) return;
+    const int64_t i = list[j];
+    float fx, fy, fz;
+    src.pos(src.voxel(i), fx, fy, fz);
+    const double px = fx, py = fy, pz = fz;
+    double sum = 0.0;
+    for (int64_t t = 0; t < nt; ++t) sum += solid_angle_f64(px, py, pz, load_tri(tris, t));
+    w[i] = sum / (4.0 * 3.14159265358979323846);
+}
+
 // ---- legacy exact kernel -------------------------------------------------------
 constexpr int kLegacyTile = 128;  // RASTER_TILE_SIZE (core/math_utils.cuh:35)
 
@@ -422,6 +439,16 @@ hipError_t launch_winding_fp64(const float* tris, int64_t nt, const Grid& g, con
     PointSrc src = make_src(g, idx, base);
     winding_fp64_kernel<<<(unsigned)((n_fp64 + kWBlock - 1) / kWBlock), kWBlock, 0, s>>>(tris, nt, src, fp64_list,
                                                                                         n_fp64, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_winding_fp64_count(const float* tris, int64_t nt, const Grid& g, const int64_t* idx, int64_t base,
+                                     const int64_t* fp64_list, const unsigned long long* count, int64_t cap, double* w,
+                                     hipStream_t s) {
+    if (cap <= 0) return hipSuccess;
+    PointSrc src = make_src(g, idx, base);
+    winding_fp64_count_kernel<<<(unsigned)((cap + kWBlock - 1) / kWBlock), kWBlock, 0, s>>>(tris, nt, src, fp64_list,
+                                                                                           count, w);
     return hipGetLastError();
 }
 

@@ -15,6 +15,9 @@ struct Grid {
     int64_t n() const { return (int64_t)dims[0] * dims[1] * dims[2]; }
 };
 
+// Records msg as the calling thread's rhz_last_error() and returns code.
+int set_error_message(int code, const char* msg);
+
 // ---- k_grid.hip -----------------------------------------------------------
 hipError_t launch_fill(float* d, int64_t n, float v, hipStream_t s);
 // Zeroes n ints with a kernel (safe inside captured graphs, unlike memset nodes).
@@ -160,6 +163,11 @@ hipError_t launch_winding_fp64(const float* tris, int64_t nt, const Grid& g, con
 // the nt triangles (brute force; +inf when nt = 0).
 hipError_t launch_distance_f64(const float* tris, int64_t nt, const Grid& g, const int64_t* idx, int64_t n,
                                double* d2, hipStream_t s);
+// The same with the recompute count read on the device from *count (the grid
+// covers cap list entries): no host round trip.
+hipError_t launch_winding_fp64_count(const float* tris, int64_t nt, const Grid& g, const int64_t* idx,
+                                     int64_t base, const int64_t* fp64_list, const unsigned long long* count,
+                                     int64_t cap, double* w, hipStream_t s);
 // Legacy exact per-voxel kernel (distance + winding sign).
 hipError_t launch_legacy(const float* tris, int64_t nt, const Grid& g, float* out, void* scratch,
                          hipStream_t s);

@@ -49,6 +49,12 @@ int set_err(int code, const char* fmt, ...) {
     return code;
 }
 
+}  // namespace
+
+int set_error_message(int code, const char* msg) { return set_err(code, "%s", msg); }
+
+namespace {
+
 #define HIP_TRY(expr)                                                                         \
     do {                                                                                      \
         hipError_t _e = (expr);                                                               \
