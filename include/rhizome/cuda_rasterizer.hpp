@@ -103,8 +103,20 @@ public:
             std::fprintf(stderr, "[CudaRasterizer] compute_mesh_sdf failed (%d): %s\n", rc, rhz_last_error());
             return false;
         }
+        if (st.method_used == RHZ_METHOD_FAST) {
+            // the reference logs its fast-path summary on channel "CUDA"
+            // (cuda_rasterizer.cu:1145-1157); out_time_ms is the whole call's
+            st.ms_host_total = out_time_ms;
+            char msg[512];
+            if (rhz_sdf_log_line(&st, p.band_voxels, msg, sizeof(msg)) == RHZ_OK) log_sink()("CUDA", msg);
+        }
         return true;
     }
+
+    // Where compute_mesh_sdf sends its fast-path summary line (the reference
+    // calls its RhizomeLog(channel, msg)); default: "[channel] msg" on stderr.
+    using LogSink = void (*)(const char* channel, const char* msg);
+    static void set_log_sink(LogSink sink) { log_sink() = sink ? sink : &stderr_sink; }
 
     static std::shared_ptr<CudaVolumeGrid> build_from_mesh(const TriangleMesh& mesh, float voxel_size,
                                                            float& out_time_ms) {
@@ -252,6 +264,11 @@ private:
     static uint64_t next_version() {
         static std::atomic<uint64_t> v{1};
         return ++v;
+    }
+    static void stderr_sink(const char* channel, const char* msg) { std::fprintf(stderr, "[%s] %s\n", channel, msg); }
+    static LogSink& log_sink() {
+        static LogSink sink = &stderr_sink;
+        return sink;
     }
 };
 

@@ -165,6 +165,13 @@ typedef struct rhz_sdf_stats {
 
 void rhz_sdf_params_default(rhz_sdf_params* p);
 const char* rhz_last_error(void);
+/* The reference's fast-path log line (cuda_rasterizer.cu:1145-1157) from a
+ * call's counters, into buf (cap bytes, NUL-terminated, truncated to fit):
+ * "Fast SDF path OK: <total> ms total (splat+parity .. | eikonal <r> rounds ..
+ * | verify+sign+copy ..) | <dims> voxels, <n> tris, band <b> vox, fixup .. |
+ * verify .. probes, .. sites, .. voxels patched in .. rounds | far residual ..,
+ * extra rounds .., safe-clamped ..", total = ms_host_total. */
+int rhz_sdf_log_line(const rhz_sdf_stats* st, int32_t band_voxels, char* buf, int64_t cap);
 /* Number of visible devices (0 when no GPU). */
 int rhz_device_count(void);
 /* Free and total memory of the current device.  Replaces

@@ -144,15 +144,12 @@ class Stats(C.Structure):
             d[k] = list(v) if k == "dims" else v
         return d
 
-    def log_line(self) -> str:
-        """The reference's one-line summary (cuda_rasterizer.cu:1150-1156)."""
-        d = self.dims
-        return (f"Fast SDF path OK: {self.ms_total:.0f} ms total (splat+parity {self.ms_splat + self.ms_parity:.0f} ms"
-                f" | eikonal {self.eikonal_rounds} rounds {self.ms_eikonal:.0f} ms | verify+sign"
-                f" {self.ms_verify + self.ms_apply:.0f} ms) | {d[0]}x{d[1]}x{d[2]} voxels, {self.n_triangles} tris,"
-                f" fixup {self.fixup_voxels} | verify {self.probes} probes, {self.sites} sites,"
-                f" {self.patched_voxels} voxels patched in {self.patch_rounds} rounds | far residual"
-                f" {self.far_residual}, extra rounds {self.extra_rounds}, safe-clamped {self.safe_clamped}")
+    def log_line(self, band_voxels: int = 8) -> str:
+        """The reference's one-line summary (cuda_rasterizer.cu:1145-1157),
+        formatted by rhz_sdf_log_line."""
+        buf = C.create_string_buffer(512)
+        check(lib().rhz_sdf_log_line(C.byref(self), int(band_voxels), buf, 512), "rhz_sdf_log_line")
+        return buf.value.decode()
 
 
 class Accuracy(C.Structure):
@@ -232,6 +229,7 @@ def lib():
     L.rhz_sdf_bake.argtypes = [C.POINTER(BakeItem), i32, C.POINTER(Params), i32, C.POINTER(Stats),
                                C.POINTER(C.c_double), vp]
     L.rhz_stage_winding_voxels_noalloc.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, vp, vp]
+    L.rhz_sdf_log_line.argtypes = [C.POINTER(Stats), i32, C.c_char_p, i64]
     L.rhz_sdf_accuracy.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, C.c_uint64, vp, C.POINTER(Accuracy)]
     L.rhz_tuning_set.argtypes = [C.c_char_p, i64]
     L.rhz_tuning_get.argtypes = [C.c_char_p, C.POINTER(C.c_int64)]

@@ -1644,6 +1644,22 @@ void rhz_sdf_params_default(rhz_sdf_params* p) {
 
 const char* rhz_last_error(void) { return g_err.c_str(); }
 
+int rhz_sdf_log_line(const rhz_sdf_stats* st, int32_t band_voxels, char* buf, int64_t cap) {
+    if (!st || !buf || cap < 1) return set_err(RHZ_ERR_ARG, "null stats or buffer");
+    const double total = st->ms_host_total;
+    const double to_parity = st->ms_splat + st->ms_parity;
+    const int n = std::snprintf(
+        buf, (size_t)cap,
+        "Fast SDF path OK: %.0f ms total (splat+parity %.0f ms | eikonal %d rounds %.0f ms | verify+sign+copy %.0f "
+        "ms) | %dx%dx%d voxels, %lld tris, band %d vox, fixup %lld | verify %d probes, %d sites, %lld voxels "
+        "patched in %d rounds | far residual %lld, extra rounds %d, safe-clamped %lld",
+        total, to_parity, st->eikonal_rounds, st->ms_eikonal, total - to_parity - st->ms_eikonal, st->dims[0],
+        st->dims[1], st->dims[2], (long long)st->n_triangles, band_voxels, (long long)st->fixup_voxels, st->probes,
+        st->sites, (long long)st->patched_voxels, st->patch_rounds, (long long)st->far_residual, st->extra_rounds,
+        (long long)st->safe_clamped);
+    return n < 0 ? set_err(RHZ_ERR_ARG, "formatting failed") : RHZ_OK;
+}
+
 int rhz_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) {

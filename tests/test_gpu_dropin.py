@@ -99,3 +99,22 @@ def test_query_distances_empty_inputs(rhz, geom):
     assert rhz.query_distances(v, f, np.zeros((0, 3), np.float32)).shape == (0,)
     d = rhz.query_distances(v, f[:0], np.ones((5, 3), np.float32))
     assert np.array_equal(d, np.zeros(5, np.float32))
+
+
+def test_dropin_fast_path_logs_reference_line(rhz, geom, tmp_path):
+    """Above the 32Mi-voxel gate compute_mesh_sdf takes the fast path and
+    logs the reference's summary line on channel CUDA (stderr by default)."""
+    v, f = geom.perturbed_icosphere(seed=1, subdiv=4)
+    mesh = tmp_path / "mesh.bin"
+    out = tmp_path / "out.bin"
+    _write_mesh(mesh, v, f)
+    r = subprocess.run([BIN, "sdf", str(mesh), "0.125", str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    raw = np.fromfile(out, dtype=np.uint8)
+    mn, mx = tuple(raw[:12].view(np.int32)), tuple(raw[12:24].view(np.int32))
+    dims = [int(mx[i] - mn[i] + 1) for i in range(3)]
+    assert np.prod(dims) >= 32 * 2 ** 20
+    line = [l for l in r.stderr.splitlines() if l.startswith("[CUDA] Fast SDF path OK:")]
+    assert len(line) == 1, r.stderr
+    assert f"| {dims[0]}x{dims[1]}x{dims[2]} voxels, {len(f)} tris, band 8 vox," in line[0]
+    print(line[0])

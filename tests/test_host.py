@@ -153,3 +153,18 @@ def test_mug_obj_specimen():
     e = np.sort(np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]), axis=1)
     _, counts = np.unique(e, axis=0, return_counts=True)
     assert np.all(counts == 2)
+
+
+def test_log_line_format(rhz):
+    """rhz_sdf_log_line (host only) fills the reference's summary line
+    (cuda_rasterizer.cu:1145-1157) from a call's counters."""
+    st = rhz.Stats()
+    st.dims[0], st.dims[1], st.dims[2] = 883, 823, 923
+    st.n_triangles, st.fixup_voxels, st.eikonal_rounds = 199730, 923, 8
+    st.ms_host_total, st.ms_splat, st.ms_parity, st.ms_eikonal = 104.4, 6.8, 4.3, 90.1
+    st.probes, st.sites, st.patched_voxels, st.patch_rounds = 3, 9, 1058841, 2
+    st.far_residual, st.extra_rounds, st.safe_clamped = 0, 0, 0
+    assert st.log_line() == (
+        "Fast SDF path OK: 104 ms total (splat+parity 11 ms | eikonal 8 rounds 90 ms | verify+sign+copy 3 ms) | "
+        "883x823x923 voxels, 199730 tris, band 8 vox, fixup 923 | verify 3 probes, 9 sites, 1058841 voxels "
+        "patched in 2 rounds | far residual 0, extra rounds 0, safe-clamped 0")
