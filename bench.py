@@ -192,7 +192,7 @@ def mug_specimen(rhz, torch, params, steps, warmup):
     obj = entry.load_obj()
     geom = entry.load_geometry()
     v, f = obj.load_obj(os.path.join(REPO, "tests", "golden", "mug.obj"))
-    v, f = obj.orient_outward(v, f)
+    f = rhz.orient_mesh(v, f)[0]  # the reference's winding pre-pass (obj_parser.cpp:8-34)
     tris = rhz.build_triangles(v, f)
     h = 0.10
     mn, mx = geom.frame_grid(v, h)

@@ -1,6 +1,6 @@
 // C++ drop-in for the mesh-to-SDF entry points of rhizome::CudaRasterizer
-// (native/src/engine/sdf/cuda_rasterizer.hpp:15-35, :73-86, :255-261, :280-291,
-// :661-665), implemented
+// (native/src/engine/sdf/cuda_rasterizer.hpp:15-35, :73-86, :255-261, :280-307,
+// :661-665) and of MeshWindingPass (engine/io/parsers.hpp:9-25), implemented
 // over the C-ABI of rhizome_sdf.h (link with -lrhizome_sdf).  Header-only:
 // nothing of the C++ standard library crosses the shared-library boundary.
 //
@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "rhizome_sdf.h"
@@ -260,6 +261,47 @@ public:
         return true;
     }
 
+    // cuda_rasterizer.hpp:294-301 / cuda_mesh_topology.cu:707-842: winding
+    // unification, non-orientable detection and outward orientation, in place.
+    static bool orient_mesh_gpu(TriangleMesh& inout_mesh, bool& out_is_orientable, bool& out_had_inconsistent_winding,
+                                bool& out_was_inverted, float& out_signed_volume, float& out_time_ms) {
+        const auto t0 = std::chrono::steady_clock::now();
+        out_is_orientable = true;
+        out_had_inconsistent_winding = false;
+        out_was_inverted = false;
+        out_signed_volume = 0.f;
+        int32_t o = 1, h = 0, w = 0;
+        double vol = 0.0;
+        const int rc = rhz_orient_mesh(inout_mesh.vertices.data(), (int64_t)inout_mesh.vertex_count(),
+                                       inout_mesh.indices.data(), (int64_t)inout_mesh.triangle_count(), &o, &h, &w,
+                                       &vol);
+        out_time_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (rc != RHZ_OK) {
+            std::fprintf(stderr, "[CudaRasterizer] orient_mesh_gpu failed (%d): %s\n", rc, rhz_last_error());
+            return false;
+        }
+        out_is_orientable = o != 0;
+        out_had_inconsistent_winding = h != 0;
+        out_was_inverted = w != 0;
+        out_signed_volume = (float)vol;
+        return true;
+    }
+
+    // cuda_rasterizer.hpp:303-307 / cuda_mesh_topology.cu:653-705.
+    static bool compute_signed_volume_gpu(const TriangleMesh& mesh, float& out_signed_volume_mm3, float& out_time_ms) {
+        const auto t0 = std::chrono::steady_clock::now();
+        double vol = 0.0;
+        const int rc = rhz_signed_volume(mesh.vertices.data(), (int64_t)mesh.vertex_count(), mesh.indices.data(),
+                                         (int64_t)mesh.triangle_count(), &vol);
+        out_time_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        out_signed_volume_mm3 = (float)vol;
+        if (rc != RHZ_OK) {
+            std::fprintf(stderr, "[CudaRasterizer] compute_signed_volume_gpu failed (%d): %s\n", rc, rhz_last_error());
+            return false;
+        }
+        return true;
+    }
+
 private:
     static uint64_t next_version() {
         static std::atomic<uint64_t> v{1};
@@ -269,6 +311,36 @@ private:
     static LogSink& log_sink() {
         static LogSink sink = &stderr_sink;
         return sink;
+    }
+};
+
+// engine/io/parsers.hpp:9-25 and obj_parser.cpp:8-34: the winding pre-pass
+// run on every loaded mesh, with the reference's messages.
+struct WindingPrepassResult {
+    bool is_orientable = true;
+    bool had_inconsistent_winding = false;
+    bool was_inverted = false;
+    float signed_volume = 0.0f;
+    std::string message;
+};
+
+class MeshWindingPass {
+public:
+    static WindingPrepassResult orient_mesh(TriangleMesh& mesh) {
+        WindingPrepassResult r;
+        if (mesh.triangle_count() == 0) return r;
+        float ms = 0.f;
+        CudaRasterizer::orient_mesh_gpu(mesh, r.is_orientable, r.had_inconsistent_winding, r.was_inverted,
+                                        r.signed_volume, ms);
+        if (!r.is_orientable)
+            r.message = "Mesh contains non-orientable topology (e.g. Mobius band).";
+        else if (r.was_inverted)
+            r.message = "Mesh normals were inverted; corrected to outward orientation.";
+        else if (r.had_inconsistent_winding)
+            r.message = "Inconsistent face winding detected and unified.";
+        else
+            r.message = "Mesh orientation consistent and outward-facing.";
+        return r;
     }
 };
 

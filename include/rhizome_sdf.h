@@ -266,6 +266,26 @@ int rhz_frame_grid(const float* verts, int64_t n_verts, float h,
 int rhz_winding_numbers(const float* verts, int64_t n_verts, const uint32_t* idx,
                         int64_t n_tris, const float* pts, int64_t n_pts, float* out_w);
 
+/* Winding unification and outward orientation of an indexed mesh, in place
+ * on the host index buffer.  An undirected edge shared by exactly two
+ * triangles links them; every connected component is unified to the winding
+ * of its lowest-numbered triangle (applied when any link was traversed the
+ * same way by both triangles: *had_inconsistent = 1); *is_orientable = 0 when
+ * a component's links cannot all be satisfied (a Moebius band); then, if the
+ * signed volume is below -1e-5, every triangle is reversed
+ * (*was_inverted = 1).  *signed_volume receives the final volume (binary64,
+ * fixed summation order).  Deterministic.  Out-of-range indices return
+ * RHZ_ERR_ARG.  Replaces CudaRasterizer::orient_mesh_gpu
+ * (cuda_rasterizer.hpp:294-301, engine/mesh/cuda_mesh_topology.cu:303-454,
+ * 707-842), the pre-pass of MeshWindingPass::orient_mesh
+ * (engine/io/obj_parser.cpp:8-34).  Host buffers; any output may be NULL. */
+int rhz_orient_mesh(const float* verts, int64_t n_verts, uint32_t* idx, int64_t n_tris, int32_t* is_orientable,
+                    int32_t* had_inconsistent, int32_t* was_inverted, double* signed_volume);
+/* Signed volume sum(v0 . (v1 x v2)) / 6 (binary64, fixed order).  Replaces
+ * CudaRasterizer::compute_signed_volume_gpu (cuda_rasterizer.hpp:303-306,
+ * engine/mesh/cuda_mesh_topology.cu:653-705).  Host buffers. */
+int rhz_signed_volume(const float* verts, int64_t n_verts, const uint32_t* idx, int64_t n_tris, double* volume);
+
 /* Unsigned distance from each query point to an indexed mesh: the square root
  * of the smallest squared point-triangle distance over every triangle
  * (degenerate ones included).  No triangles: every distance is 0.  Replaces

@@ -155,6 +155,10 @@ def lib():
         L.orc_interior_count.argtypes = [_f32p, C.c_int64]
         L.orc_interior_count.restype = C.c_int64
         L.orc_eikonal_redblack.argtypes = [_f32p, _i32p, C.c_float, C.c_int]
+        L.orc_orient_mesh.argtypes = [_f32p, C.c_int64, _u32p, C.c_int64, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.orc_signed_volume.argtypes = [_f32p, _u32p, C.c_int64]
+        L.orc_signed_volume.restype = C.c_double
         L.orc_hash_01.argtypes = [C.c_uint32]
         L.orc_hash_01.restype = C.c_float
         L.orc_gwn_test_points.argtypes = [C.c_int, C.c_uint32, _f32p]
@@ -329,3 +333,19 @@ def gwn_test_points(mode, seed):
     out = np.zeros(30000, dtype=np.float32)
     lib().orc_gwn_test_points(mode, seed, out)
     return out.reshape(10000, 3)
+
+
+def orient_mesh(verts, idx):
+    """(oriented indices (n, 3) uint32, orientable, had_inconsistent,
+    was_inverted, signed volume)."""
+    v = _f32(verts).reshape(-1)
+    i = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1).copy()
+    o, h, w, vol = C.c_int(0), C.c_int(0), C.c_int(0), C.c_double(0)
+    lib().orc_orient_mesh(v, len(v) // 3, i, len(i) // 3, C.byref(o), C.byref(h), C.byref(w), C.byref(vol))
+    return i.reshape(-1, 3), bool(o.value), bool(h.value), bool(w.value), vol.value
+
+
+def signed_volume(verts, idx):
+    v = _f32(verts).reshape(-1)
+    i = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    return lib().orc_signed_volume(v, i, len(i) // 3)

@@ -1443,6 +1443,130 @@ void orc_eikonal_redblack(float* g, const int32_t dims[3], float h, int iters) {
 }
 
 /* dev_hash_01; math_utils.cuh:219-226. */
+/* ------------------------------------------------------------------ */
+/* mesh orientation (cuda_mesh_topology.cu:253-454, 653-842)             */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    uint64_t key;
+    int64_t tri;
+    int dir;
+    int edge;
+} orc_edge;
+
+static int cmp_edge(const void* a, const void* b) {
+    const orc_edge* x = (const orc_edge*)a;
+    const orc_edge* y = (const orc_edge*)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    if (x->tri != y->tri) return x->tri < y->tri ? -1 : 1;
+    return x->edge - y->edge;
+}
+
+double orc_signed_volume(const float* v, const uint32_t* idx, int64_t nt) {
+    double s = 0.0;
+    for (int64_t t = 0; t < nt; ++t) {
+        const float* a = v + 3 * (size_t)idx[3 * t];
+        const float* b = v + 3 * (size_t)idx[3 * t + 1];
+        const float* c = v + 3 * (size_t)idx[3 * t + 2];
+        s += (double)a[0] * ((double)b[1] * c[2] - (double)b[2] * c[1]) +
+             (double)a[1] * ((double)b[2] * c[0] - (double)b[0] * c[2]) +
+             (double)a[2] * ((double)b[0] * c[1] - (double)b[1] * c[0]);
+    }
+    return s / 6.0;
+}
+
+/* Links between the two triangles of every edge shared by exactly two
+ * triangles (found by sorting the edges by their undirected key), a
+ * breadth-first walk of each component from its lowest-numbered triangle
+ * assigning flips relative to it, a link its walk contradicts marking the
+ * mesh non-orientable; flips applied when any link was traversed the same
+ * way by both triangles; the whole mesh reversed when its signed volume is
+ * below -1e-5. */
+void orc_orient_mesh(const float* v, int64_t nv, uint32_t* idx, int64_t nt, int* is_orientable,
+                     int* had_inconsistent, int* was_inverted, double* signed_volume) {
+    (void)nv;
+    *is_orientable = 1;
+    *had_inconsistent = 0;
+    *was_inverted = 0;
+    *signed_volume = 0.0;
+    if (nt <= 0) return;
+    orc_edge* e = (orc_edge*)malloc(sizeof(orc_edge) * 3 * (size_t)nt);
+    int64_t ne = 0;
+    for (int64_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k) {
+            uint32_t a = idx[3 * t + k], b = idx[3 * t + (k + 1) % 3];
+            if (a == b) continue;
+            uint32_t mn = a < b ? a : b, mx = a < b ? b : a;
+            e[ne].key = ((uint64_t)mn << 32) | mx;
+            e[ne].tri = t;
+            e[ne].dir = a < b ? 0 : 1;
+            e[ne].edge = k;
+            ++ne;
+        }
+    qsort(e, (size_t)ne, sizeof(orc_edge), cmp_edge);
+    int64_t* nb = (int64_t*)malloc(sizeof(int64_t) * 3 * (size_t)nt);
+    uint8_t* same = (uint8_t*)calloc(3 * (size_t)nt, 1);
+    for (int64_t i = 0; i < 3 * nt; ++i) nb[i] = -1;
+    for (int64_t i = 0; i < ne;) {
+        int64_t j = i;
+        while (j < ne && e[j].key == e[i].key) ++j;
+        if (j - i == 2 && e[i].tri != e[i + 1].tri) {
+            int sm = e[i].dir == e[i + 1].dir;
+            nb[3 * e[i].tri + e[i].edge] = e[i + 1].tri;
+            nb[3 * e[i + 1].tri + e[i + 1].edge] = e[i].tri;
+            same[3 * e[i].tri + e[i].edge] = (uint8_t)sm;
+            same[3 * e[i + 1].tri + e[i + 1].edge] = (uint8_t)sm;
+            if (sm) *had_inconsistent = 1;
+        }
+        i = j;
+    }
+    free(e);
+    int8_t* flip = (int8_t*)malloc((size_t)nt);
+    int64_t* queue = (int64_t*)malloc(sizeof(int64_t) * (size_t)nt);
+    for (int64_t t = 0; t < nt; ++t) flip[t] = -1;
+    for (int64_t r = 0; r < nt; ++r) {
+        if (flip[r] >= 0) continue;
+        int64_t head = 0, tail = 0;
+        flip[r] = 0;
+        queue[tail++] = r;
+        while (head < tail) {
+            int64_t t = queue[head++];
+            for (int k = 0; k < 3; ++k) {
+                int64_t n = nb[3 * t + k];
+                if (n < 0) continue;
+                int want = flip[t] ^ same[3 * t + k];
+                if (flip[n] < 0) {
+                    flip[n] = (int8_t)want;
+                    queue[tail++] = n;
+                } else if (flip[n] != want) {
+                    *is_orientable = 0;
+                }
+            }
+        }
+    }
+    if (*had_inconsistent)
+        for (int64_t t = 0; t < nt; ++t)
+            if (flip[t]) {
+                uint32_t a = idx[3 * t + 1];
+                idx[3 * t + 1] = idx[3 * t + 2];
+                idx[3 * t + 2] = a;
+            }
+    double vol = orc_signed_volume(v, idx, nt);
+    if (vol < -1e-5) {
+        for (int64_t t = 0; t < nt; ++t) {
+            uint32_t a = idx[3 * t + 1];
+            idx[3 * t + 1] = idx[3 * t + 2];
+            idx[3 * t + 2] = a;
+        }
+        vol = -vol;
+        *was_inverted = 1;
+    }
+    *signed_volume = vol;
+    free(nb);
+    free(same);
+    free(flip);
+    free(queue);
+}
+
 float orc_hash_01(uint32_t x) {
     x ^= x >> 16;
     x *= 0x7feb352du;

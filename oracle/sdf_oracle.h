@@ -48,6 +48,16 @@
  *                                    sphere: closed 2-manifold, genus 0,
  *                                    enclosed volume; plane: exact positions
  *   orc_interior_count               direct count
+ *   orc_orient_mesh / signed_volume  closed forms: seeded flipped faces come
+ *                                    back exactly, an inside-out sphere is
+ *                                    inverted, a Moebius band is
+ *                                    non-orientable, a nested inside-out
+ *                                    shell keeps its winding, unit cube and
+ *                                    tetrahedron volumes; a planted error in
+ *                                    the link direction fails them
+ *   orc_eikonal_redblack             linear field from a planar seed, cells
+ *                                    with no interior neighbours untouched,
+ *                                    signs kept, sweep order changes results
  *   orc_verify_indices               closed form (double arithmetic)
  *   orc_patch_box_*                  closed form
  *   orc_splat_band                   equals the brute-force min over all
@@ -230,6 +240,12 @@ int orc_mc_table(int c, int* edges);
 void orc_mc_count(const float* g, const int32_t dims[3], float h, float iso, int64_t* nv, int64_t* nt);
 void orc_mc_extract(const float* g, const int32_t dims[3], float h, const float origin[3], float iso,
                     float* verts, uint32_t* idx);
+/* Winding unification and outward orientation, in place
+ * (CudaRasterizer::orient_mesh_gpu, cuda_mesh_topology.cu:303-454, 707-842). */
+void orc_orient_mesh(const float* v, int64_t nv, uint32_t* idx, int64_t nt, int* is_orientable,
+                     int* had_inconsistent, int* was_inverted, double* signed_volume);
+/* sum(v0 . (v1 x v2)) / 6 in binary64, triangle order (:653-705). */
+double orc_signed_volume(const float* v, const uint32_t* idx, int64_t nt);
 /* Voxels with value <= 0 (ui/viewport/cuda_viewport_gl.cu:84-110). */
 int64_t orc_interior_count(const float* g, int64_t n);
 /* Standalone red-black eikonal (cuda_rasterizer.cu:1536-1629), 8 iterations. */

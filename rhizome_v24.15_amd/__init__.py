@@ -230,6 +230,9 @@ def lib():
                                C.POINTER(C.c_double), vp]
     L.rhz_stage_winding_voxels_noalloc.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, vp, vp]
     L.rhz_sdf_log_line.argtypes = [C.POINTER(Stats), i32, C.c_char_p, i64]
+    L.rhz_orient_mesh.argtypes = [vp, i64, vp, i64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    L.rhz_signed_volume.argtypes = [vp, i64, vp, i64, C.POINTER(C.c_double)]
     L.rhz_sdf_accuracy.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, C.c_uint64, vp, C.POINTER(Accuracy)]
     L.rhz_tuning_set.argtypes = [C.c_char_p, i64]
     L.rhz_tuning_get.argtypes = [C.c_char_p, C.POINTER(C.c_int64)]
@@ -417,6 +420,27 @@ def winding_numbers(verts, idx, pts):
     check(lib().rhz_winding_numbers(v.ctypes.data, len(v) // 3, i.ctypes.data, len(i) // 3, p.ctypes.data,
                                     len(out), out.ctypes.data), "rhz_winding_numbers")
     return out
+
+
+def orient_mesh(verts, idx):
+    """Winding unification and outward orientation (rhz_orient_mesh).
+    Returns (indices (n, 3) uint32, orientable, had_inconsistent,
+    was_inverted, signed volume)."""
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
+    i = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1).copy()
+    o, h, w, vol = C.c_int32(0), C.c_int32(0), C.c_int32(0), C.c_double(0)
+    check(lib().rhz_orient_mesh(v.ctypes.data, len(v) // 3, i.ctypes.data, len(i) // 3, C.byref(o), C.byref(h),
+                                C.byref(w), C.byref(vol)), "rhz_orient_mesh")
+    return i.reshape(-1, 3), bool(o.value), bool(h.value), bool(w.value), vol.value
+
+
+def signed_volume(verts, idx):
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
+    i = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    vol = C.c_double(0)
+    check(lib().rhz_signed_volume(v.ctypes.data, len(v) // 3, i.ctypes.data, len(i) // 3, C.byref(vol)),
+          "rhz_signed_volume")
+    return vol.value
 
 
 def query_distances(verts, idx, pts):

@@ -29,6 +29,7 @@ SOURCES = [
     "k_mlp.hip",
     "k_cluster.hip",
     "k_mc.hip",
+    "k_topology.hip",
     "noalloc.hip",
     "sdf_pipeline.cpp",
 ]

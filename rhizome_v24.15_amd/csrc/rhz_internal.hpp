@@ -256,6 +256,36 @@ hipError_t launch_mlp_grad(const float* P, const float* X, const float* Y, int64
 hipError_t launch_adam(float* P, const float* G, float* M, float* V, int n, float lr, float b1, float b2, float eps,
                        int step, float scale, float wd, hipStream_t s);
 
+// ---- k_topology.hip -------------------------------------------------------
+// Open-addressing table of undirected edges (size a power of two, at least
+// twice the edge count: edge_table_size(nt)); per key, the number of
+// triangles on the edge and the min / max of (triangle << 1 | direction).
+struct EdgeTable {
+    unsigned long long* key;
+    uint32_t* count;
+    unsigned long long* lo;
+    unsigned long long* hi;
+    int64_t size;
+};
+int64_t edge_table_size(int64_t nt);
+// Edge table of idx, then the links: nb[3 t + e] = the other triangle on edge
+// e of t when exactly two share it (else -1), same[3 t + e] = 1 when both
+// traverse it the same way (then *inconsistent = 1).  slot: 3 nt int64.
+hipError_t launch_edge_links(const uint32_t* idx, int64_t nt, const EdgeTable& E, int64_t* slot, int32_t* nb,
+                             uint8_t* same, int* inconsistent, hipStream_t s);
+hipError_t launch_label_init(int64_t nt, int32_t* parent, uint8_t* parity, hipStream_t s);
+// One hook (p0, q0 -> p1, q1) and one jump (p1, q1 -> p0, q0); *changed = 1
+// if a parent moved.
+hipError_t launch_label_step(int64_t nt, const int32_t* nb, const uint8_t* same, int32_t* p0, uint8_t* q0,
+                             int32_t* p1, uint8_t* q1, int* changed, hipStream_t s);
+hipError_t launch_orient_check(int64_t nt, const int32_t* nb, const uint8_t* same, const uint8_t* parity,
+                               int* non_orientable, hipStream_t s);
+// Reverses triangles with flip[t] != 0 (all when flip is null).
+hipError_t launch_flip(uint32_t* idx, int64_t nt, const uint8_t* flip, hipStream_t s);
+// volume_blocks(nt) binary64 partial sums of v0 . (v1 x v2).
+int volume_blocks(int64_t nt);
+hipError_t launch_volume_partials(const float* v, const uint32_t* idx, int64_t nt, double* part, hipStream_t s);
+
 // ---- k_mesh.hip -----------------------------------------------------------
 hipError_t launch_tri_valid(const float* verts, int64_t nv, const uint32_t* idx, int64_t nt,
                             uint32_t* valid, hipStream_t s);

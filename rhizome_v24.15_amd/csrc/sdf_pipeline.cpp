@@ -125,6 +125,18 @@ enum Slot {
     S_MC_TRIS,
     S_DIST3,       // second FSM working buffer (rows padded to 128 B)
     S_ACC,         // accuracy report: field values at the samples
+    S_TP_KEY,      // orientation: edge table keys, counts, min / max tags
+    S_TP_CNT,
+    S_TP_LO,
+    S_TP_HI,
+    S_TP_SLOT,     // table slot of every triangle edge
+    S_TP_NB,       // linked triangle of every edge, and whether it agrees
+    S_TP_SAME,
+    S_TP_P0,       // component labels (double-buffered) and parities
+    S_TP_Q0,
+    S_TP_P1,
+    S_TP_Q1,
+    S_TP_VOL,      // signed-volume partial sums
     S_COUNT
 };
 
@@ -2021,6 +2033,118 @@ int rhz_winding_numbers(const float* verts, int64_t n_verts, const uint32_t* idx
     HIP_TRY(hipMemcpy(dp, pts, sizeof(float) * 3 * (size_t)n_pts, hipMemcpyHostToDevice));
     HIP_TRY(launch_winding_points(dv, di, n_tris, dp, n_pts, dw, nullptr));
     HIP_TRY(hipMemcpy(out_w, dw, sizeof(float) * (size_t)n_pts, hipMemcpyDeviceToHost));
+    return RHZ_OK;
+}
+
+// Stages an indexed mesh in the workspace (vertices in S_MISC, indices in
+// S_MISC2); rejects out-of-range indices.
+static int stage_mesh(Workspace* w, const float* verts, int64_t n_verts, const uint32_t* idx, int64_t n_tris,
+                      float** dv, uint32_t** di) {
+    for (int64_t t = 0; t < 3 * n_tris; ++t)
+        if ((int64_t)idx[t] >= n_verts) return set_err(RHZ_ERR_ARG, "index out of range");
+    *dv = (float*)ws_get(w, S_MISC, sizeof(float) * 3 * (size_t)std::max<int64_t>(n_verts, 1));
+    *di = (uint32_t*)ws_get(w, S_MISC2, sizeof(uint32_t) * 3 * (size_t)n_tris);
+    if (!*dv || !*di) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+    HIP_TRY(hipMemcpy(*dv, verts, sizeof(float) * 3 * (size_t)n_verts, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(*di, idx, sizeof(uint32_t) * 3 * (size_t)n_tris, hipMemcpyHostToDevice));
+    return RHZ_OK;
+}
+
+// Signed volume of a staged mesh: fixed-order sum of the per-block partials.
+static int staged_volume(Workspace* w, const float* dv, const uint32_t* di, int64_t n_tris, double* vol) {
+    const int nb = volume_blocks(n_tris);
+    double* part = (double*)ws_get(w, S_TP_VOL, sizeof(double) * (size_t)nb);
+    if (!part) return set_err(RHZ_ERR_NOMEM, "allocation failed");
+    HIP_TRY(launch_volume_partials(dv, di, n_tris, part, nullptr));
+    std::vector<double> h((size_t)nb);
+    HIP_TRY(hipMemcpy(h.data(), part, sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost));
+    double sum = 0.0;
+    for (double x : h) sum += x;
+    *vol = sum / 6.0;
+    return RHZ_OK;
+}
+
+int rhz_signed_volume(const float* verts, int64_t n_verts, const uint32_t* idx, int64_t n_tris, double* volume) {
+    int rc = need_device();
+    if (rc) return rc;
+    if (!volume || n_tris < 0 || n_verts < 0 || (n_tris > 0 && (!idx || !verts)))
+        return set_err(RHZ_ERR_ARG, "null buffer");
+    *volume = 0.0;
+    if (n_tris == 0 || n_verts == 0) return RHZ_OK;  // cuda_mesh_topology.cu:660-664
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    float* dv = nullptr;
+    uint32_t* di = nullptr;
+    rc = stage_mesh(w, verts, n_verts, idx, n_tris, &dv, &di);
+    if (rc) return rc;
+    return staged_volume(w, dv, di, n_tris, volume);
+}
+
+int rhz_orient_mesh(const float* verts, int64_t n_verts, uint32_t* idx, int64_t n_tris, int32_t* is_orientable,
+                    int32_t* had_inconsistent, int32_t* was_inverted, double* signed_volume) {
+    int rc = need_device();
+    if (rc) return rc;
+    if (n_tris < 0 || n_verts < 0 || (n_tris > 0 && (!idx || !verts))) return set_err(RHZ_ERR_ARG, "null buffer");
+    if (n_tris >= ((int64_t)1 << 31)) return set_err(RHZ_ERR_ARG, "more than 2^31 - 1 triangles");
+    int32_t orientable = 1, inconsistent = 0, inverted = 0;
+    double vol = 0.0;
+    if (n_tris > 0 && n_verts > 0) {  // an empty mesh is oriented (cuda_mesh_topology.cu:722-725)
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        Workspace* w = workspace();
+        if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+        float* dv = nullptr;
+        uint32_t* di = nullptr;
+        rc = stage_mesh(w, verts, n_verts, idx, n_tris, &dv, &di);
+        if (rc) return rc;
+        const int64_t ts = edge_table_size(n_tris);
+        EdgeTable E{(unsigned long long*)ws_get(w, S_TP_KEY, sizeof(unsigned long long) * (size_t)ts),
+                    (uint32_t*)ws_get(w, S_TP_CNT, sizeof(uint32_t) * (size_t)ts),
+                    (unsigned long long*)ws_get(w, S_TP_LO, sizeof(unsigned long long) * (size_t)ts),
+                    (unsigned long long*)ws_get(w, S_TP_HI, sizeof(unsigned long long) * (size_t)ts), ts};
+        int64_t* slot = (int64_t*)ws_get(w, S_TP_SLOT, sizeof(int64_t) * 3 * (size_t)n_tris);
+        int32_t* nb = (int32_t*)ws_get(w, S_TP_NB, sizeof(int32_t) * 3 * (size_t)n_tris);
+        uint8_t* same = (uint8_t*)ws_get(w, S_TP_SAME, 3 * (size_t)n_tris);
+        int32_t* p0 = (int32_t*)ws_get(w, S_TP_P0, sizeof(int32_t) * (size_t)n_tris);
+        int32_t* p1 = (int32_t*)ws_get(w, S_TP_P1, sizeof(int32_t) * (size_t)n_tris);
+        uint8_t* q0 = (uint8_t*)ws_get(w, S_TP_Q0, (size_t)n_tris);
+        uint8_t* q1 = (uint8_t*)ws_get(w, S_TP_Q1, (size_t)n_tris);
+        if (!E.key || !E.count || !E.lo || !E.hi || !slot || !nb || !same || !p0 || !p1 || !q0 || !q1)
+            return set_err(RHZ_ERR_NOMEM, "allocation failed");
+        int* flags = flag_ptr(w, 0);  // [0] inconsistent, [1] changed, [2] non-orientable
+        HIP_TRY(launch_zero_i32(flags, 3, nullptr));
+        HIP_TRY(launch_edge_links(di, n_tris, E, slot, nb, same, flags + 0, nullptr));
+        HIP_TRY(launch_label_init(n_tris, p0, q0, nullptr));
+        // hook + jump steps until no parent moves (checked every 8 steps);
+        // the labels settle within diameter + 1 steps
+        constexpr int kBatch = 8;
+        for (int64_t done = 0;; done += kBatch) {
+            if (done > n_tris + kBatch) return set_err(RHZ_ERR_HIP, "component labels did not settle");
+            HIP_TRY(launch_zero_i32(flags + 1, 1, nullptr));
+            for (int k = 0; k < kBatch; ++k) HIP_TRY(launch_label_step(n_tris, nb, same, p0, q0, p1, q1, flags + 1, nullptr));
+            int changed = 1;
+            HIP_TRY(hipMemcpy(&changed, flags + 1, sizeof(int), hipMemcpyDeviceToHost));
+            if (!changed) break;
+        }
+        HIP_TRY(launch_orient_check(n_tris, nb, same, q0, flags + 2, nullptr));
+        int h[3] = {0, 0, 0};
+        HIP_TRY(hipMemcpy(h, flags, sizeof(h), hipMemcpyDeviceToHost));
+        inconsistent = h[0] != 0;
+        orientable = h[2] == 0;
+        if (inconsistent) HIP_TRY(launch_flip(di, n_tris, q0, nullptr));
+        rc = staged_volume(w, dv, di, n_tris, &vol);
+        if (rc) return rc;
+        if (vol < -1e-5) {  // cuda_mesh_topology.cu:806-810
+            HIP_TRY(launch_flip(di, n_tris, nullptr, nullptr));
+            vol = -vol;
+            inverted = 1;
+        }
+        HIP_TRY(hipMemcpy(idx, di, sizeof(uint32_t) * 3 * (size_t)n_tris, hipMemcpyDeviceToHost));
+    }
+    if (is_orientable) *is_orientable = orientable;
+    if (had_inconsistent) *had_inconsistent = inconsistent;
+    if (was_inverted) *was_inverted = inverted;
+    if (signed_volume) *signed_volume = vol;
     return RHZ_OK;
 }
 

@@ -38,6 +38,7 @@ __all__ = [
     "two_cubes",
     "bake_batch",
     "uv_sphere",
+    "moebius",
     "merge",
     "signed_volume",
     "orient_outward",
@@ -334,6 +335,28 @@ def two_cubes():
          (8, 10, 9), (8, 11, 10), (12, 13, 14), (12, 14, 15), (8, 9, 13), (8, 13, 12),
          (9, 10, 14), (9, 14, 13), (10, 11, 15), (10, 15, 14), (11, 8, 12), (11, 12, 15)]
     return _as_mesh(v, f)
+
+
+def moebius(R=20.0, half_width=4.0, n=96):
+    """A Moebius band: n quads around a circle of radius R whose cross
+    direction turns by half a turn; the last quad joins the first with top
+    and bottom exchanged, so no consistent winding exists."""
+    v = []
+    for i in range(n):
+        th = 2 * math.pi * i / n
+        r = np.array([math.cos(th), math.sin(th), 0.0])
+        d = math.cos(th / 2) * r + math.sin(th / 2) * np.array([0.0, 0.0, 1.0])
+        c = R * r
+        v += [c - half_width * d, c + half_width * d]  # bottom 2i, top 2i + 1
+    f = []
+    for i in range(n):
+        b0, t0 = 2 * i, 2 * i + 1
+        if i + 1 < n:
+            b1, t1 = 2 * i + 2, 2 * i + 3
+        else:  # half a turn later the cross direction is reversed
+            b1, t1 = 1, 0
+        f += [(b0, b1, t1), (b0, t1, t0)]
+    return _as_mesh(np.array(v), f)
 
 
 def uv_sphere(radius=15.0, stacks=48, slices=48):

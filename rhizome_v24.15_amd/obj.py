@@ -2,9 +2,9 @@
 
 Reads ``v x y z`` and ``f`` records (1-based or negative indices, ``a/b/c``
 forms, polygons fanned into triangles), as the reference's ObjParser does
-(native/src/engine/io/obj_parser.cpp:36-120), and applies the outward
-orientation check of its winding pre-pass (obj_parser.cpp:8-34): a closed
-mesh with negative signed volume has every face flipped.
+(native/src/engine/io/obj_parser.cpp:36-120).  The reference's winding
+pre-pass after loading (obj_parser.cpp:8-34) is the library's
+``orient_mesh`` (rhz_orient_mesh).
 """
 from __future__ import annotations
 
@@ -46,15 +46,3 @@ def save_obj(path: str, verts, faces) -> None:
         for a, b, c in (f + 1).tolist():
             fh.write(f"f {a} {b} {c}\n")
 
-
-def signed_volume(v, f) -> float:
-    v = np.asarray(v, dtype=np.float64)
-    a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
-    return float(np.einsum("ij,ij->i", a, np.cross(b, c)).sum() / 6.0)
-
-
-def orient_outward(v, f):
-    """Flips every face of a mesh whose signed volume is negative."""
-    if signed_volume(v, f) < 0:
-        return v, f[:, ::-1].copy()
-    return v, f

@@ -7,6 +7,7 @@
 //   dropin_main sdf <mesh.bin> <h> <out.bin>
 //     mesh.bin: int64 nv, int64 nt, float xyz[3 nv], uint32 idx[3 nt]
 //     out.bin:  int32 min[3], int32 max[3], float field[V] (build_from_mesh)
+//   dropin_main orient <mesh.bin> <out.bin>   (MeshWindingPass::orient_mesh; out: uint32 idx[3 nt])
 //   dropin_main wind <mesh.bin> <pts.bin> <out.bin>
 //     pts.bin: int64 np, float xyz[3 np]; out.bin: float w[np]
 // Exit status: 0 ok, 1 usage or I/O error, 2 the library call failed.
@@ -121,6 +122,19 @@ int main(int argc, char** argv) {
         std::fclose(o);
         return 0;
     }
-    std::fprintf(stderr, "usage: dropin_main info | sdf mesh.bin h out.bin | wind|dist mesh.bin pts.bin out.bin | mc|vol grid.bin out.bin\n");
+    if (argc == 4 && !std::strcmp(argv[1], "orient")) {
+        // the winding pre-pass of the OBJ loader: out.bin = the oriented indices
+        TriangleMesh m;
+        if (!read_mesh(argv[2], m)) return 1;
+        const WindingPrepassResult r = MeshWindingPass::orient_mesh(m);
+        FILE* o = std::fopen(argv[3], "wb");
+        if (!o) return 1;
+        std::fwrite(m.indices.data(), 4, m.indices.size(), o);
+        std::fclose(o);
+        std::printf("orientable %d inconsistent %d inverted %d volume %.6f\n%s\n", r.is_orientable ? 1 : 0,
+                    r.had_inconsistent_winding ? 1 : 0, r.was_inverted ? 1 : 0, r.signed_volume, r.message.c_str());
+        return 0;
+    }
+    std::fprintf(stderr, "usage: dropin_main info | sdf mesh.bin h out.bin | wind|dist mesh.bin pts.bin out.bin | mc|vol grid.bin out.bin | orient mesh.bin out.bin\n");
     return 1;
 }

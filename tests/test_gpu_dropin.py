@@ -118,3 +118,20 @@ def test_dropin_fast_path_logs_reference_line(rhz, geom, tmp_path):
     assert len(line) == 1, r.stderr
     assert f"| {dims[0]}x{dims[1]}x{dims[2]} voxels, {len(f)} tris, band 8 vox," in line[0]
     print(line[0])
+
+
+def test_dropin_winding_prepass_matches_binding(rhz, geom, tmp_path):
+    """MeshWindingPass::orient_mesh (the OBJ loader's pre-pass) through the
+    C++ drop-in: the binding's indices and flags, the reference's message."""
+    v, f = geom.icosphere(3)
+    v = (v * 20.0).astype(np.float32)
+    f = np.array(f, dtype=np.uint32)
+    f[5:40] = f[5:40][:, [0, 2, 1]]
+    mesh = tmp_path / "mesh.bin"
+    out = tmp_path / "out.bin"
+    _write_mesh(mesh, v, f)
+    lines = _run("orient", str(mesh), str(out)).splitlines()
+    ref = rhz.orient_mesh(v, f)
+    assert np.array_equal(np.fromfile(out, dtype=np.uint32).reshape(-1, 3), ref[0])
+    assert lines[0].startswith("orientable 1 inconsistent 1 inverted 0 volume")
+    assert lines[1] == "Inconsistent face winding detected and unified."

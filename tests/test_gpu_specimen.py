@@ -28,7 +28,8 @@ def mug(rhz):
 
     obj = entry.load_obj()
     v, f = obj.load_obj(os.path.join(HERE, "golden", "mug.obj"))
-    v, f = obj.orient_outward(v, f)
+    f, orientable, _, _, _ = rhz.orient_mesh(v, f)  # the reference's pre-pass (obj_parser.cpp:8-34)
+    assert orientable
     return v, f, rhz.build_triangles(v, f)
 
 

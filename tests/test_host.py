@@ -143,7 +143,7 @@ def test_mug_obj_specimen():
     spec.loader.exec_module(obj)
     v, f = obj.load_obj(os.path.join(REPO, "tests", "golden", "mug.obj"))
     assert v.shape == (1553, 3) and f.shape == (3106, 3)
-    vol = obj.signed_volume(v, f)
+    vol = entry.load_geometry().signed_volume(v, f)
     assert abs(abs(vol) - 75913) < 1.0
     geom = entry.load_geometry()
     mn, mx = geom.frame_grid(v, 0.1)
