@@ -255,10 +255,31 @@ def batched_bake(rhz, torch, params, n_meshes=64, log=None, fixed=None):
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3
     lat = np.array(ms)
+    # the other bake mode at the same lane count, for comparison: batched
+    # rounds finish every item with the batch, lanes finish each on its own
+    mode = rhz.tuning_get("bake_batched")
+    rhz.tuning_set("bake_batched", 1 - mode)
+    for _ in range(2):
+        rhz.bake(items, params, best["n_lanes"])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rc2, _, ms2 = rhz.bake(items, params, best["n_lanes"])
+    torch.cuda.synchronize()
+    wall2 = (time.perf_counter() - t1) * 1e3
+    rhz.tuning_set("bake_batched", mode)
+    lat2 = np.array(ms2)
+    names = {1: "batched rounds (every lattice's eikonal rounds in shared launches, one captured graph)",
+             0: "lanes (each item's whole build on one of n_lanes streams)"}
     return {"workload": f"{n_meshes} seeded vessels, {min(tris_n)}-{max(tris_n)} tris, longest side 64-384 "
                         f"at 0.25-0.5 mm, {total} voxels", "status": rc,
             "value": total / (wall * 1e-3) / 1e9, "unit": "Gvoxel/s", "batch_ms": wall,
             "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
+            "mode": names[mode],
+            "other_mode": {"mode": names[1 - mode], "status": rc2, "value": total / (wall2 * 1e-3) / 1e9,
+                           "batch_ms": wall2, "latency_ms_p50": float(np.percentile(lat2, 50)),
+                           "latency_ms_p99": float(np.percentile(lat2, 99))},
+            "latency_note": "per-item latency: from the item's start on its lane to its field (lanes), or the "
+                            "whole batch (batched rounds: every field is complete when the batch is)",
             "tuning": best, "autotune_table": table}
 
 
@@ -343,7 +364,7 @@ def main():
                     help="also build the whole lattice with the plain exact-winding method and count sign "
                          "mismatches (minutes at 0.10 mm)")
     ap.add_argument("--bake-only", action="store_true",
-                    help="only the batched bake, at 8 lanes with the one-line walker (no autotuning): two "
+                    help="only the batched bake, at 8 lanes with batched rounds (no autotuning): two "
                          "warm-up bakes and one timed bake; prints its JSON record")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 2 GiB device buffer 3 times (reference kernel for rocprofv3 --pmc "
@@ -372,8 +393,8 @@ def main():
     method = {"fast": rhz.METHOD_FAST, "learned": rhz.METHOD_FAST_LEARNED, "plain": rhz.METHOD_PLAIN_EXACT,
               "legacy": rhz.METHOD_LEGACY}[args.method]
     if args.bake_only:
-        fixed = {"n_lanes": 8, "sweep_shape_x": 1, "sweep_shape_y": 1, "sweep_shape_z": 1, "sweep_step": 8,
-                 "fsm_batch_rounds_max_voxels": 16 << 20, "fsm_graph": 1}
+        fixed = {"n_lanes": 8, "sweep_shape_x": 14, "sweep_shape_y": 22, "sweep_shape_z": 22, "sweep_step": 2,
+                 "fsm_batch_rounds_max_voxels": 16 << 20, "fsm_graph": 1, "bake_batched": 1}
         rec = batched_bake(rhz, torch, rhz.default_params(method=method), log=log, fixed=fixed)
         print(json.dumps(rec))
         return
@@ -494,8 +515,8 @@ def main():
     if not args.no_batch:
         try:
             saved = {k: rhz.tuning_get(k) for k in ("sweep_shape_x", "sweep_shape_y", "sweep_shape_z", "sweep_step",
-                                                    "sweep_rows",
-                                                    "fsm_batch_rounds_max_voxels", "fsm_graph")}
+                                                    "sweep_rows", "fsm_batch_rounds_max_voxels", "fsm_graph",
+                                                    "bake_batched")}
             batch = batched_bake(rhz, torch, params, log=log)
             for k, v in saved.items():
                 rhz.tuning_set(k, v)

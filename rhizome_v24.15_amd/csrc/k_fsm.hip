@@ -517,6 +517,43 @@ __global__ __launch_bounds__(256) void sweep_tile_kernel(const float* __restrict
     if (__ballot(chg) != 0 && threadIdx.x == 0) atomicOr(changed, 1);
 }
 
+// The job (lattice) that logical block lb belongs to: starts[j] <= lb <
+// starts[j + 1] (starts[0] = 0, starts[njobs] = the grid size).
+__device__ __forceinline__ int batch_job(const unsigned* __restrict__ starts, int njobs, unsigned lb) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= lb) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One directional sweep of many lattices in one launch: the blocks of job j
+// are the blocks sweep_tile_kernel would launch for it alone, so every
+// lattice gets exactly its own sweep.  Round r of job j runs only if round
+// r - 1 lowered a voxel of that lattice (flags[r - 1]); a round after one
+// without change would leave the field as it is.  Changes go to flags[r].
+template <int SF, int SW, int SC, int DIR, int STEP, int NF, int NC>
+__global__ __launch_bounds__(256) void sweep_tile_batch_kernel(const SweepJob* __restrict__ jobs,
+                                                               const unsigned* __restrict__ starts, int njobs,
+                                                               unsigned nblocks, int r) {
+    const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
+    const int j = __builtin_amdgcn_readfirstlane(batch_job(starts, njobs, lb));
+    const SweepJob J = jobs[j];
+    if (r > 0 && J.flags[r - 1] == 0) return;
+    const unsigned local = lb - starts[j];
+    const int bf = (int)(local % (unsigned)J.nbf);
+    const int c0 = __builtin_amdgcn_readfirstlane(((int)(local / (unsigned)J.nbf) * 4 + (int)threadIdx.y) * NC);
+    if (c0 >= J.nc) return;
+    const SweepConst K = sweep_const(J.h);
+    const bool face = c0 == 0 || c0 + NC >= J.nc || bf == 0 || (bf + 1) * NF * 64 >= J.nf;
+    const bool chg =
+        face ? tile_walk<SF, SW, SC, DIR, STEP, NF, NC, true>(J.src, J.dst, J.nf, J.nw, J.nc, J.ws, J.cs, bf, c0, K)
+             : tile_walk<SF, SW, SC, DIR, STEP, NF, NC, false>(J.src, J.dst, J.nf, J.nw, J.nc, J.ws, J.cs, bf, c0, K);
+    if (__ballot(chg) != 0 && threadIdx.x == 0) atomicOr(J.flags + r, 1);
+}
+
 template <int SF, int SW, int SC, int DIR, int STEP, int ROWS, bool FMAJ>
 __global__ __launch_bounds__(64 * ROWS) void sweep_lane_kernel(const float* __restrict__ src,
                                                                float* __restrict__ dst, int nf, int nw, int nc,
@@ -606,13 +643,11 @@ __global__ __launch_bounds__(64 * ROWS) void sweep_lane_kernel(const float* __re
 // keeps the transpose at copy speed.
 constexpr int kTrTiles = 2;
 
-__global__ __launch_bounds__(256) void transpose_planes_kernel(const float* __restrict__ in, float* __restrict__ out,
-                                                               int rows, int cols, int64_t in_pitch, int64_t in_plane,
-                                                               int64_t out_pitch, int64_t out_plane, int tiles_r,
-                                                               int tiles_c, unsigned nblocks) {
-    __shared__ float tile[2][64][65];
+__device__ __forceinline__ void transpose_group(const float* __restrict__ in, float* __restrict__ out, int rows,
+                                                int cols, int64_t in_pitch, int64_t in_plane, int64_t out_pitch,
+                                                int64_t out_plane, int tiles_r, int tiles_c, unsigned lb,
+                                                float (&tile)[2][64][65]) {
     const int groups_c = (tiles_c + kTrTiles - 1) / kTrTiles;
-    const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
     const int per_plane = tiles_r * groups_c;
     const int z = (int)(lb / (unsigned)per_plane);
     const int t = (int)(lb % (unsigned)per_plane);
@@ -645,6 +680,30 @@ __global__ __launch_bounds__(256) void transpose_planes_kernel(const float* __re
             if (r < rows && c < cols) outz[(int64_t)c * out_pitch + r] = tile[buf][tx][ty + 4 * k];
         }
     }
+}
+
+__global__ __launch_bounds__(256) void transpose_planes_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                               int rows, int cols, int64_t in_pitch, int64_t in_plane,
+                                                               int64_t out_pitch, int64_t out_plane, int tiles_r,
+                                                               int tiles_c, unsigned nblocks) {
+    __shared__ float tile[2][64][65];
+    transpose_group(in, out, rows, cols, in_pitch, in_plane, out_pitch, out_plane, tiles_r, tiles_c,
+                    xcd_logical_block(blockIdx.x, nblocks), tile);
+}
+
+// The plane transposes of many lattices in one launch (the blocks of job j
+// are those of transpose_planes_kernel for it alone), gated like the batched
+// sweeps: round r of a lattice runs only if its round r - 1 changed a voxel.
+__global__ __launch_bounds__(256) void transpose_batch_kernel(const TransJob* __restrict__ jobs,
+                                                              const unsigned* __restrict__ starts, int njobs,
+                                                              unsigned nblocks, int r) {
+    __shared__ float tile[2][64][65];
+    const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
+    const int j = __builtin_amdgcn_readfirstlane(batch_job(starts, njobs, lb));
+    const TransJob J = jobs[j];
+    if (r > 0 && J.flags[r - 1] == 0) return;
+    transpose_group(J.in, J.out, J.rows, J.cols, J.in_pitch, J.in_plane, J.out_pitch, J.out_plane, J.tiles_r,
+                    J.tiles_c, lb - starts[j], tile);
 }
 
 // ---- standalone red-black eikonal (cuda_rasterizer.cu:1536-1578) ---------------
@@ -854,6 +913,82 @@ void fsm_layouts(const Grid& g, FieldLayout* xy, FieldLayout* yx) {
     const bool ok = lane_sweeps_ok(g) && px * dy < (int64_t(1) << 27) && py * dx < (int64_t(1) << 27);
     *xy = ok ? FieldLayout{px, px * dy} : dense_xy(g);
     *yx = ok ? FieldLayout{py, py * dx} : dense_yx(g);
+}
+
+// Tile shape (NF x NC) of the batched sweeps per orientation: the tuning's
+// tile shape for that walk axis, or, for the one-line walker, the default
+// tiles (1 x 4 for x on the y-fastest copy, 2 x 1 for y and z).
+static void batch_shape(int orient, int* nf_tile, int* nc_tile) {
+    const int walk = orient == 0 ? 0 : orient;  // walk axis: x, y, z
+    int shape = sweep_tuning().shape[walk];
+    if (shape != 21 && shape != 22 && shape != 12 && shape != 14) shape = orient == 0 ? 14 : 21;
+    *nf_tile = shape / 10;
+    *nc_tile = shape % 10;
+}
+
+int64_t sweep_batch_blocks(int orient, int nf, int nc, int* nbf) {
+    int NF = 2, NC = 1;
+    batch_shape(orient, &NF, &NC);
+    *nbf = (nf + 64 * NF - 1) / (64 * NF);
+    const int ngroups = (nc + NC - 1) / NC;
+    return (int64_t)*nbf * ((ngroups + 3) / 4);
+}
+
+namespace {
+template <int SF, int SW, int SC, int DIR, int STEP>
+hipError_t launch_sweep_batch_s(int NF, int NC, const SweepJob* jobs, const unsigned* starts, int njobs,
+                                unsigned nblocks, int r, hipStream_t s) {
+#define RHZ_BTILE(A, B)                                                                                  \
+    if (NF == A && NC == B) {                                                                            \
+        sweep_tile_batch_kernel<SF, SW, SC, DIR, STEP, A, B><<<nblocks, dim3(64, 4), 0, s>>>(jobs, starts, njobs, \
+                                                                                           nblocks, r);   \
+        return hipGetLastError();                                                                        \
+    }
+    RHZ_BTILE(2, 1)
+    RHZ_BTILE(2, 2)
+    RHZ_BTILE(1, 2)
+    RHZ_BTILE(1, 4)
+#undef RHZ_BTILE
+    return hipErrorInvalidValue;
+}
+
+template <int SF, int SW, int SC, int DIR>
+hipError_t launch_sweep_batch_d(int NF, int NC, const SweepJob* jobs, const unsigned* starts, int njobs,
+                                unsigned nblocks, int r, hipStream_t s) {
+    const int step = sweep_tuning().step;
+    if (step == 4) return launch_sweep_batch_s<SF, SW, SC, DIR, 4>(NF, NC, jobs, starts, njobs, nblocks, r, s);
+    if (step == 8) return launch_sweep_batch_s<SF, SW, SC, DIR, 8>(NF, NC, jobs, starts, njobs, nblocks, r, s);
+    return launch_sweep_batch_s<SF, SW, SC, DIR, 2>(NF, NC, jobs, starts, njobs, nblocks, r, s);
+}
+}  // namespace
+
+hipError_t launch_sweep_batch(int orient, int dir, const SweepJob* jobs, const unsigned* starts, int njobs,
+                              unsigned nblocks, int r, hipStream_t s) {
+    if (njobs <= 0 || nblocks == 0) return hipSuccess;
+    int NF = 2, NC = 1;
+    batch_shape(orient, &NF, &NC);
+    if (orient == 0)  // y-fastest copy: lanes y (slot 1), walk x (slot 0), cross z (slot 2)
+        return dir > 0 ? launch_sweep_batch_d<1, 0, 2, 1>(NF, NC, jobs, starts, njobs, nblocks, r, s)
+                       : launch_sweep_batch_d<1, 0, 2, -1>(NF, NC, jobs, starts, njobs, nblocks, r, s);
+    if (orient == 1)  // lanes x, walk y, cross z
+        return dir > 0 ? launch_sweep_batch_d<0, 1, 2, 1>(NF, NC, jobs, starts, njobs, nblocks, r, s)
+                       : launch_sweep_batch_d<0, 1, 2, -1>(NF, NC, jobs, starts, njobs, nblocks, r, s);
+    // lanes x, walk z, cross y
+    return dir > 0 ? launch_sweep_batch_d<0, 2, 1, 1>(NF, NC, jobs, starts, njobs, nblocks, r, s)
+                   : launch_sweep_batch_d<0, 2, 1, -1>(NF, NC, jobs, starts, njobs, nblocks, r, s);
+}
+
+int64_t transpose_batch_blocks(int rows, int cols, int planes, int* tiles_r, int* tiles_c) {
+    *tiles_r = (rows + 63) / 64;
+    *tiles_c = (cols + 63) / 64;
+    return (int64_t)*tiles_r * ((*tiles_c + kTrTiles - 1) / kTrTiles) * planes;
+}
+
+hipError_t launch_transpose_batch(const TransJob* jobs, const unsigned* starts, int njobs, unsigned nblocks, int r,
+                                  hipStream_t s) {
+    if (njobs <= 0 || nblocks == 0) return hipSuccess;
+    transpose_batch_kernel<<<nblocks, 256, 0, s>>>(jobs, starts, njobs, nblocks, r);
+    return hipGetLastError();
 }
 
 hipError_t launch_redblack(float* g, const int32_t dims[3], float h, int parity, hipStream_t s) {

@@ -129,6 +129,37 @@ hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir,
                            const FieldLayout& L);
 // The padded layouts of the FSM's working copies (x-fastest and y-fastest).
 void fsm_layouts(const Grid& g, FieldLayout* xy, FieldLayout* yx);
+
+// ---- batched rounds (many lattices per launch) ----------------------------
+// One lattice's part of a batched directional sweep: the lane sweep's
+// arguments (fast axis nf cells, walk nw steps of stride ws, nc cross lines
+// of stride cs) and the lattice's per-round change flags (flags[r] = round r
+// lowered a voxel; round r > 0 runs only if flags[r - 1] != 0).
+struct SweepJob {
+    const float* src;
+    float* dst;
+    int64_t ws, cs;
+    int nf, nw, nc, nbf;
+    float h;
+    int* flags;
+};
+// One lattice's part of a batched plane transpose (gated like the sweeps).
+struct TransJob {
+    const float* in;
+    float* out;
+    int64_t in_pitch, in_plane, out_pitch, out_plane;
+    int rows, cols, tiles_r, tiles_c;
+    int* flags;
+};
+// Sweep orientations: 0 = x-sweep on the y-fastest copy, 1 = y-sweep, 2 =
+// z-sweep (x-fastest copy).  Blocks of one job, and its fast-axis block
+// count (the job's nbf), for the batched tile shape of that orientation.
+int64_t sweep_batch_blocks(int orient, int nf, int nc, int* nbf);
+hipError_t launch_sweep_batch(int orient, int dir, const SweepJob* jobs, const unsigned* starts, int njobs,
+                              unsigned nblocks, int r, hipStream_t s);
+int64_t transpose_batch_blocks(int rows, int cols, int planes, int* tiles_r, int* tiles_c);
+hipError_t launch_transpose_batch(const TransJob* jobs, const unsigned* starts, int njobs, unsigned nblocks, int r,
+                                  hipStream_t s);
 // ---- k_mc.hip -------------------------------------------------------------
 // Marching cubes in three passes: classify (in: inside byte per voxel; info: u8 per voxel; blk_v / blk_t:
 // u32 per block of mc_blocks(dims)), prefix sums of the block totals into

@@ -200,6 +200,14 @@ struct Lane {
 std::mutex g_lane_mu;
 Lane g_lanes[64][kMaxLanes];
 
+// Device memory of the batched rounds, cached per device (grow-only).
+struct BakeWork {
+    Workspace ws;            // slots below, plus counters
+    hipStream_t stream = nullptr;
+    bool ready = false;
+};
+BakeWork g_bake[64];
+
 Lane* bake_lane(int lane) {
     int d = current_device();
     if (d < 0 || d >= 64 || lane < 0 || lane >= kMaxLanes) return nullptr;
@@ -1370,6 +1378,9 @@ int learned_sign_stage(Ctx& c, uint8_t* sign, const CsrBuffers& b, unsigned* bit
 // eikonal rounds run (tuning key "verify_overlap").
 constexpr int64_t kOverlapMinVoxels = 1ll << 24;
 std::atomic<int> g_verify_overlap{1};
+// 1: rhz_sdf_bake runs the eikonal rounds of all its lattices as batched
+// launches (tuning key "bake_batched"); 0: every item's whole build on a lane.
+std::atomic<int> g_bake_batched{1};
 
 // Parity signs over the candidate-list CSR, then the exact fixup of the
 // deferred columns.
@@ -1384,6 +1395,50 @@ int learned_scan(Ctx& c, uint8_t* sign, CsrBuffers* b) {
         if (rc) return rc;
         c.st->fixup_voxels = fixed;
     }
+    return RHZ_OK;
+}
+
+// The two halves of a FAST_LEARNED build that the batched bake runs apart
+// from the eikonal rounds.  learned_band: fill and band splat into the field
+// (all the rounds need).  learned_signs: candidate compaction from the band,
+// signed scan, fixup and the sign stage, into `sign` (all the clamp and the
+// apply need; nothing here writes the field).
+int learned_band(Ctx& c) {
+    if (!int_range_ok(c.g, c.nt) || c.nt <= 0) {
+        c.st->decline_reason = 1;
+        return RHZ_DECLINED;
+    }
+    HIP_TRY(launch_fill(c.d, c.g.n(), kFar, c.s));
+    HIP_TRY(launch_splat(c.tris, c.nt, c.g, (float)c.p->band_voxels * c.g.h, c.d, c.s));
+    return RHZ_OK;
+}
+
+int learned_signs(Ctx& c, uint8_t* sign) {
+    const int64_t V = c.g.n();
+    unsigned* bits = (unsigned*)ws_get(c.w, S_BITS, sizeof(unsigned) * (size_t)((V + 31) / 32));
+    if (!bits) {
+        c.st->decline_reason = 4;
+        return RHZ_DECLINED;
+    }
+    int rc = zero_counters(c);
+    if (rc) return rc;
+    Timer tm(true);
+    tm.mark(0, c.s);
+    int64_t* cand = nullptr;
+    int64_t n_cand = 0;
+    rc = learned_candidate_list(c, &cand, &n_cand);
+    if (rc) return rc;
+    tm.mark(1, c.s);
+    CsrBuffers b;
+    rc = learned_scan(c, sign, &b);
+    if (rc) return rc;
+    tm.mark(2, c.s);
+    rc = learned_sign_stage(c, sign, b, bits, cand, n_cand, tm, 3, 4);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c.s));
+    c.st->ms_parity = tm.ms(1, 2);
+    c.st->ms_proposer = tm.ms(0, 1) + tm.ms(2, 3);
+    c.st->ms_verify = tm.ms(3, 4);
     return RHZ_OK;
 }
 
@@ -1711,6 +1766,11 @@ int rhz_tuning_set(const char* key, int64_t value) {
         g_verify_overlap.store((int)value, std::memory_order_relaxed);
         return RHZ_OK;
     }
+    if (!std::strcmp(key, "bake_batched")) {
+        if (value != 0 && value != 1) return set_err(RHZ_ERR_ARG, "bake_batched must be 0 or 1");
+        g_bake_batched.store((int)value, std::memory_order_relaxed);
+        return RHZ_OK;
+    }
     static const char* kShapeKeys[3] = {"sweep_shape_x", "sweep_shape_y", "sweep_shape_z"};
     for (int a = 0; a < 3; ++a)
         if (!std::strcmp(key, kShapeKeys[a])) {
@@ -1754,6 +1814,10 @@ int rhz_tuning_get(const char* key, int64_t* value) {
         *value = g_verify_overlap.load(std::memory_order_relaxed);
         return RHZ_OK;
     }
+    if (!std::strcmp(key, "bake_batched")) {
+        *value = g_bake_batched.load(std::memory_order_relaxed);
+        return RHZ_OK;
+    }
     return set_err(RHZ_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -1786,12 +1850,14 @@ void rhz_release_workspace(void) {
         Workspace* w = workspace();
         if (w) ws_free_slots(w);
     }
-    // the bake lanes' workspaces (their counters and streams stay)
+    // the bake lanes' workspaces and the batched rounds' buffers (their
+    // counters and streams stay)
     std::lock_guard<std::mutex> lk(g_lane_mu);
     const int d = current_device();
     if (d < 0 || d >= 64) return;
     for (int l = 0; l < kMaxLanes; ++l)
         if (g_lanes[d][l].ready) ws_free_slots(&g_lanes[d][l].ws);
+    if (g_bake[d].ready) ws_free_slots(&g_bake[d].ws);
 }
 
 // The body of rhz_sdf_compute_device; the caller holds g_ws_mu.
@@ -1827,6 +1893,322 @@ int rhz_sdf_compute_device(const rhz_triangle* d_tris, int64_t n_tris, float h, 
     rhz_sdf_stats local;
     std::lock_guard<std::mutex> lk(g_ws_mu);
     return compute_device_locked(d_tris, n_tris, g, params, d_out, (hipStream_t)stream, stats ? stats : &local, t0);
+}
+
+// ---- batched bake: the eikonal rounds of every lattice in shared launches ----
+namespace rhz {
+namespace {
+
+enum BakeSlot { B_SIGN = 0, B_FSM, B_FLAGS, B_FAR, B_JOBS, B_STARTS };
+constexpr int kBakeRoundStride = 32;  // flag slots per item (>= max_rounds + 1)
+
+BakeWork* bake_work() {
+    const int d = current_device();
+    if (d < 0 || d >= 64) return nullptr;
+    BakeWork& b = g_bake[d];
+    if (!b.ready) {
+        if (hipMalloc(&b.ws.d_cnt, 512) != hipSuccess) return nullptr;
+        if (hipHostMalloc((void**)&b.ws.h_cnt, 512, hipHostMallocDefault) != hipSuccess) return nullptr;
+        if (hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        b.ws.init = true;
+        b.ready = true;
+    }
+    ws_reap(&b.ws);
+    return &b;
+}
+
+struct BakeItemState {
+    Grid g;
+    FieldLayout lx, ly;
+    int64_t cells = 0;        // floats per working copy
+    float* a = nullptr;       // working copies (the result ends in a, layout lx)
+    float* b = nullptr;
+    uint8_t* sign = nullptr;  // final signs (dense)
+    int* flags = nullptr;     // per-round change flags
+    int rounds = 0, extra = 0;
+    int64_t far = 0;
+};
+
+// The batched rounds apply when every lattice takes the lane sweeps and the
+// build is the learned method with the reference round policy.
+bool bake_batchable(const rhz_bake_item* items, int n, const std::vector<Grid>& grids, const rhz_sdf_params* p) {
+    if (p->method != RHZ_METHOD_FAST_LEARNED || p->fsm_policy != RHZ_FSM_REFERENCE) return false;
+    if (p->max_rounds < 1 || p->max_rounds >= kBakeRoundStride) return false;
+    for (int i = 0; i < n; ++i) {
+        FieldLayout lx, ly;
+        fsm_layouts(grids[(size_t)i], &lx, &ly);
+        if (!lane_sweeps_ok(grids[(size_t)i]) || !int_range_ok(grids[(size_t)i], items[i].n_tris)) return false;
+    }
+    return true;
+}
+
+// One lattice's round outside the batch (audit-gated extension rounds):
+// sweep_round_p's launches on the item's working copies, in place in a.
+int item_round(BakeItemState& it, int* flag, hipStream_t s) {
+    const Grid& g = it.g;
+    HIP_TRY(launch_transpose_xy(it.a, it.b, g, true, s, it.lx, it.ly));
+    HIP_TRY(launch_sweep_yx(it.b, it.a, g, +1, flag, s, it.ly));
+    HIP_TRY(launch_sweep_yx(it.a, it.b, g, -1, flag, s, it.ly));
+    HIP_TRY(launch_transpose_xy(it.b, it.a, g, false, s, it.ly, it.lx));
+    HIP_TRY(launch_sweep(it.a, it.b, g, 1, +1, flag, s, it.lx));
+    HIP_TRY(launch_sweep(it.b, it.a, g, 1, -1, flag, s, it.lx));
+    HIP_TRY(launch_sweep(it.a, it.b, g, 2, +1, flag, s, it.lx));
+    HIP_TRY(launch_sweep(it.b, it.a, g, 2, -1, flag, s, it.lx));
+    return RHZ_OK;
+}
+
+}  // namespace
+}  // namespace rhz
+
+// The batched bake (tuning key "bake_batched", default on): phase 1 runs fill
+// and splat of every item on the lanes; then the eikonal rounds of ALL
+// lattices run on the bake stream as one captured graph of batched launches
+// (per round: two transposes and six sweeps, each one launch over every
+// lattice; a lattice whose previous round changed nothing is skipped by its
+// device flag), while the lanes run each item's sign stage (candidates,
+// scan, fixup, features, ranking, exact probes) into per-item sign buffers;
+// last, per item, the audit, extension rounds if any, clamp and the signed
+// write-back.  Fields and counters equal the single builds'.
+static int bake_batched(const rhz_bake_item* items, int32_t n_items, const rhz_sdf_params* params,
+                        const std::vector<Grid>& grids, const std::vector<int>& order, std::vector<Lane*>& lanes,
+                        std::vector<rhz_sdf_stats>& st, std::vector<int>& rcs, std::vector<double>& lat,
+                        hipEvent_t entry, int dev) {
+    const auto t_start = std::chrono::steady_clock::now();
+    BakeWork* bw = bake_work();
+    if (!bw) return set_err(RHZ_ERR_NOMEM, "bake workspace initialisation failed");
+    const hipStream_t ms = bw->stream;
+    HIP_TRY(hipStreamWaitEvent(ms, entry, 0));
+    const int n = n_items;
+    std::vector<BakeItemState> it((size_t)n);
+    size_t sign_bytes = 0, fsm_cells = 0;
+    for (int i = 0; i < n; ++i) {
+        BakeItemState& s = it[(size_t)i];
+        s.g = grids[(size_t)i];
+        fsm_layouts(s.g, &s.lx, &s.ly);
+        s.cells = std::max(s.lx.plane, s.ly.plane) * (int64_t)s.g.dims[2];
+        sign_bytes += (size_t)((s.g.n() + 255) / 256 * 256);
+        fsm_cells += (size_t)((s.cells + 63) / 64 * 64);
+    }
+    uint8_t* signs = (uint8_t*)ws_get(&bw->ws, B_SIGN, sign_bytes);
+    // (a tail after the last copy, as the single builds' slots have one)
+    float* fsm = (float*)ws_get(&bw->ws, B_FSM, sizeof(float) * 2 * fsm_cells + (64 << 10));
+    int* flags = (int*)ws_get(&bw->ws, B_FLAGS, sizeof(int) * (size_t)n * kBakeRoundStride);
+    unsigned long long* far = (unsigned long long*)ws_get(&bw->ws, B_FAR, sizeof(unsigned long long) * (size_t)n);
+    if (!signs || !fsm || !flags || !far) return set_err(RHZ_ERR_NOMEM, "bake buffers allocation failed");
+    {
+        size_t so = 0, fo = 0;
+        for (int i = 0; i < n; ++i) {
+            BakeItemState& s = it[(size_t)i];
+            s.sign = signs + so;
+            s.a = fsm + fo;
+            s.b = fsm + fsm_cells + fo;
+            s.flags = flags + (size_t)i * kBakeRoundStride;
+            so += (size_t)((s.g.n() + 255) / 256 * 256);
+            fo += (size_t)((s.cells + 63) / 64 * 64);
+        }
+    }
+    // job tables: transposes (round 0 from the dense band, later rounds from
+    // a; the way back), sweeps x+/x- (y-fastest), y+/y-, z+/z-
+    enum { T_FIRST0 = 0, T_FIRST, T_BACK, S_XP, S_XM, S_YP, S_YM, S_ZP, S_ZM, kTables };
+    const int nT = 3, nS = 6;
+    std::vector<TransJob> tj((size_t)nT * n);
+    std::vector<SweepJob> sj((size_t)nS * n);
+    std::vector<unsigned> starts((size_t)5 * (n + 1));  // transposes to y-fastest / back; sweeps x, y, z
+    unsigned nblk[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        const BakeItemState& s = it[(size_t)i];
+        const int dx = s.g.dims[0], dy = s.g.dims[1], dz = s.g.dims[2];
+        const FieldLayout dense = dense_xy(s.g);
+        int tr = 0, tc = 0;
+        const int64_t b0 = transpose_batch_blocks(dy, dx, dz, &tr, &tc);
+        tj[(size_t)0 * n + i] = TransJob{items[i].d_out, s.b, dense.pitch, dense.plane, s.ly.pitch, s.ly.plane,
+                                        dy, dx, tr, tc, s.flags};
+        tj[(size_t)1 * n + i] = TransJob{s.a, s.b, s.lx.pitch, s.lx.plane, s.ly.pitch, s.ly.plane, dy, dx, tr, tc,
+                                        s.flags};
+        const int64_t b1 = transpose_batch_blocks(dx, dy, dz, &tr, &tc);
+        tj[(size_t)2 * n + i] = TransJob{s.b, s.a, s.ly.pitch, s.ly.plane, s.lx.pitch, s.lx.plane, dx, dy, tr, tc,
+                                        s.flags};
+        int nbf = 0;
+        const int64_t bx = sweep_batch_blocks(0, dy, dz, &nbf);
+        sj[(size_t)0 * n + i] = SweepJob{s.b, s.a, s.ly.pitch, s.ly.plane, dy, dx, dz, nbf, s.g.h, s.flags};
+        sj[(size_t)1 * n + i] = SweepJob{s.a, s.b, s.ly.pitch, s.ly.plane, dy, dx, dz, nbf, s.g.h, s.flags};
+        const int64_t by = sweep_batch_blocks(1, dx, dz, &nbf);
+        sj[(size_t)2 * n + i] = SweepJob{s.a, s.b, s.lx.pitch, s.lx.plane, dx, dy, dz, nbf, s.g.h, s.flags};
+        sj[(size_t)3 * n + i] = SweepJob{s.b, s.a, s.lx.pitch, s.lx.plane, dx, dy, dz, nbf, s.g.h, s.flags};
+        const int64_t bz = sweep_batch_blocks(2, dx, dy, &nbf);
+        sj[(size_t)4 * n + i] = SweepJob{s.a, s.b, s.lx.plane, s.lx.pitch, dx, dz, dy, nbf, s.g.h, s.flags};
+        sj[(size_t)5 * n + i] = SweepJob{s.b, s.a, s.lx.plane, s.lx.pitch, dx, dz, dy, nbf, s.g.h, s.flags};
+        const int64_t bl[5] = {b0, b1, bx, by, bz};
+        for (int k = 0; k < 5; ++k) {
+            starts[(size_t)k * (n + 1) + i] = nblk[k];
+            if ((int64_t)nblk[k] + bl[k] >= (int64_t)1 << 31) return set_err(RHZ_ERR_ARG, "batch too large");
+            nblk[k] += (unsigned)bl[k];
+        }
+    }
+    for (int k = 0; k < 5; ++k) starts[(size_t)k * (n + 1) + n] = nblk[k];
+    const size_t tj_bytes = sizeof(TransJob) * tj.size(), sj_bytes = sizeof(SweepJob) * sj.size();
+    char* jobs = (char*)ws_get(&bw->ws, B_JOBS, tj_bytes + sj_bytes);
+    unsigned* d_starts = (unsigned*)ws_get(&bw->ws, B_STARTS, sizeof(unsigned) * starts.size());
+    if (!jobs || !d_starts) return set_err(RHZ_ERR_NOMEM, "bake tables allocation failed");
+    HIP_TRY(hipMemcpyAsync(jobs, tj.data(), tj_bytes, hipMemcpyHostToDevice, ms));
+    HIP_TRY(hipMemcpyAsync(jobs + tj_bytes, sj.data(), sj_bytes, hipMemcpyHostToDevice, ms));
+    HIP_TRY(hipMemcpyAsync(d_starts, starts.data(), sizeof(unsigned) * starts.size(), hipMemcpyHostToDevice, ms));
+    const TransJob* d_tj = reinterpret_cast<const TransJob*>(jobs);
+    const SweepJob* d_sj = reinterpret_cast<const SweepJob*>(jobs + tj_bytes);
+    auto st_of = [&](int k) { return d_starts + (size_t)k * (n + 1); };
+
+    // phase 1 (lanes): fill and splat of every item
+    const int n_lanes = (int)lanes.size();
+    std::atomic<int> next{0};
+    std::vector<std::string> errs((size_t)n);
+    auto run_items = [&](int l, bool signs_phase) {
+        (void)hipSetDevice(dev);
+        Lane* L = lanes[(size_t)l];
+        for (;;) {
+            const int k = next.fetch_add(1);
+            if (k >= n) break;
+            const int i = order[(size_t)k];
+            if (rcs[(size_t)i] != RHZ_OK) continue;
+            Ctx c{reinterpret_cast<const float*>(items[i].d_tris), items[i].n_tris, grids[(size_t)i], params,
+                  items[i].d_out, L->stream, &st[(size_t)i], &L->ws};
+            const int r = signs_phase ? learned_signs(c, it[(size_t)i].sign) : learned_band(c);
+            if (r != RHZ_OK) {
+                rcs[(size_t)i] = r;
+                if (r < 0) errs[(size_t)i] = rhz_last_error();
+            }
+        }
+        if (hipStreamSynchronize(L->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            for (int i = 0; i < n; ++i)
+                if (rcs[(size_t)i] == RHZ_OK) rcs[(size_t)i] = set_err(RHZ_ERR_HIP, "bake lane sync failed");
+        }
+    };
+    // every lane (the calling thread is lane 0) takes items until none is left
+    auto run_lanes = [&](bool signs_phase) {
+        next.store(0);
+        std::vector<std::thread> th;
+        for (int l = 1; l < n_lanes; ++l) th.emplace_back(run_items, l, signs_phase);
+        run_items(0, signs_phase);
+        for (auto& t : th) t.join();
+    };
+    run_lanes(false);
+    const auto t_band = std::chrono::steady_clock::now();
+
+    // phase 2: the rounds of every lattice as one captured graph on the bake
+    // stream, while the lanes run the sign stages
+    const int cap = params->max_rounds;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int rc = RHZ_OK;
+    {
+        hipError_t e = hipStreamBeginCapture(ms, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) e = launch_zero_i32(flags, n * kBakeRoundStride, ms);
+        for (int r = 0; r < cap && e == hipSuccess; ++r) {
+            e = launch_transpose_batch(d_tj + (size_t)(r == 0 ? 0 : 1) * n, st_of(0), n, nblk[0], r, ms);
+            if (e == hipSuccess) e = launch_sweep_batch(0, +1, d_sj + (size_t)0 * n, st_of(2), n, nblk[2], r, ms);
+            if (e == hipSuccess) e = launch_sweep_batch(0, -1, d_sj + (size_t)1 * n, st_of(2), n, nblk[2], r, ms);
+            if (e == hipSuccess) e = launch_transpose_batch(d_tj + (size_t)2 * n, st_of(1), n, nblk[1], r, ms);
+            if (e == hipSuccess) e = launch_sweep_batch(1, +1, d_sj + (size_t)2 * n, st_of(3), n, nblk[3], r, ms);
+            if (e == hipSuccess) e = launch_sweep_batch(1, -1, d_sj + (size_t)3 * n, st_of(3), n, nblk[3], r, ms);
+            if (e == hipSuccess) e = launch_sweep_batch(2, +1, d_sj + (size_t)4 * n, st_of(4), n, nblk[4], r, ms);
+            if (e == hipSuccess) e = launch_sweep_batch(2, -1, d_sj + (size_t)5 * n, st_of(4), n, nblk[4], r, ms);
+        }
+        const hipError_t ec = hipStreamEndCapture(ms, &graph);
+        if (e == hipSuccess) e = ec;
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipGraphLaunch(exec, ms);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            rc = set_err(RHZ_ERR_HIP, "batched rounds: %s", hipGetErrorString(e));
+        }
+    }
+    run_lanes(true);  // the sign stages, beside the rounds
+    if (rc == RHZ_OK && hipStreamSynchronize(ms) != hipSuccess) rc = set_err(RHZ_ERR_HIP, "batched rounds failed");
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    const auto t_rounds = std::chrono::steady_clock::now();
+
+    // rounds per lattice, then the audit of every lattice
+    std::vector<int> hflags((size_t)n * kBakeRoundStride);
+    HIP_TRY(hipMemcpy(hflags.data(), flags, sizeof(int) * hflags.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(launch_zero_i32(reinterpret_cast<int*>(far), 2 * n, ms));
+    for (int i = 0; i < n; ++i) {
+        BakeItemState& s = it[(size_t)i];
+        s.rounds = cap;
+        for (int r = 0; r < cap; ++r)
+            if (hflags[(size_t)i * kBakeRoundStride + r] == 0) {
+                s.rounds = r + 1;
+                break;
+            }
+        HIP_TRY(launch_count_far_rows(s.a, s.lx.pitch, s.lx.plane, s.g.dims[0], s.g.dims[1], s.g.dims[2], far + i, ms));
+    }
+    std::vector<unsigned long long> hfar((size_t)n);
+    HIP_TRY(hipMemcpyAsync(hfar.data(), far, sizeof(unsigned long long) * (size_t)n, hipMemcpyDeviceToHost, ms));
+    HIP_TRY(hipStreamSynchronize(ms));
+    // audit-gated extension rounds (rare: a round reaches every voxel that
+    // straight lines connect to the band)
+    for (int i = 0; i < n; ++i) {
+        BakeItemState& s = it[(size_t)i];
+        s.far = (int64_t)hfar[(size_t)i];
+        while (s.far > 0 && s.extra < params->max_extra_rounds) {
+            int* ext_flag = s.flags + kBakeRoundStride - 1;
+            rc = item_round(s, ext_flag, ms);
+            if (rc) return rc;
+            ++s.extra;
+            HIP_TRY(launch_zero_i32(reinterpret_cast<int*>(far + i), 2, ms));
+            HIP_TRY(launch_count_far_rows(s.a, s.lx.pitch, s.lx.plane, s.g.dims[0], s.g.dims[1], s.g.dims[2], far + i,
+                                          ms));
+            HIP_TRY(hipMemcpyAsync(&hfar[(size_t)i], far + i, sizeof(unsigned long long), hipMemcpyDeviceToHost, ms));
+            HIP_TRY(hipStreamSynchronize(ms));
+            s.far = (int64_t)hfar[(size_t)i];
+        }
+    }
+    // clamp and the signed write-back
+    for (int i = 0; i < n; ++i) {
+        if (rcs[(size_t)i] != RHZ_OK) continue;
+        BakeItemState& s = it[(size_t)i];
+        const FieldLayout dense = dense_xy(s.g);
+        const int dx = s.g.dims[0], dy = s.g.dims[1], dz = s.g.dims[2];
+        rhz_sdf_stats& si = st[(size_t)i];
+        si.eikonal_rounds = s.rounds + s.extra;
+        si.extra_rounds = s.extra;
+        si.sweeps = 6 * (s.rounds + s.extra);
+        si.safe_clamped = 0;
+        si.far_residual = 0;
+        if (s.far == 0) {
+            HIP_TRY(launch_copy_apply_rows(s.a, s.lx.pitch, s.lx.plane, items[i].d_out, s.sign, dx, dy, dz, ms));
+        } else {
+            HIP_TRY(launch_copy_rows(s.a, s.lx.pitch, s.lx.plane, items[i].d_out, dense.pitch, dense.plane, dx, dy,
+                                     dz, ms));
+            si.safe_clamped = s.far;
+            HIP_TRY(launch_clamp_far(items[i].d_out, s.sign, s.g.n(), (float)params->band_voxels * s.g.h,
+                                     params->clamp_intended_sign, ms));
+            HIP_TRY(launch_zero_i32(reinterpret_cast<int*>(far + i), 2, ms));
+            HIP_TRY(launch_count_far(items[i].d_out, s.g.n(), far + i, ms));
+            HIP_TRY(hipMemcpyAsync(&hfar[(size_t)i], far + i, sizeof(unsigned long long), hipMemcpyDeviceToHost, ms));
+            HIP_TRY(hipStreamSynchronize(ms));
+            si.far_residual = (int64_t)hfar[(size_t)i];
+            HIP_TRY(launch_apply(items[i].d_out, s.sign, s.g.n(), ms));
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(ms));
+    const auto t_end = std::chrono::steady_clock::now();
+    const double ms_all = std::chrono::duration<double, std::milli>(t_end - t_start).count();
+    const double ms_band = std::chrono::duration<double, std::milli>(t_band - t_start).count();
+    const double ms_rounds = std::chrono::duration<double, std::milli>(t_rounds - t_band).count();
+    for (int i = 0; i < n; ++i) {
+        if (rcs[(size_t)i] < 0 && !errs[(size_t)i].empty()) set_err(rcs[(size_t)i], "%s", errs[(size_t)i].c_str());
+        rhz_sdf_stats& si = st[(size_t)i];
+        si.status = rcs[(size_t)i];
+        si.method_used = RHZ_METHOD_FAST_LEARNED;
+        si.ms_splat = ms_band;
+        si.ms_eikonal = ms_rounds;  // the shared rounds (sign stages beside them)
+        si.ms_total = ms_all;
+        si.ms_host_total = ms_all;
+        lat[(size_t)i] = ms_all;  // every field is complete when the batch is
+    }
+    return RHZ_OK;
 }
 
 int rhz_sdf_bake(const rhz_bake_item* items, int32_t n_items, const rhz_sdf_params* params, int32_t n_lanes,
@@ -1870,6 +2252,24 @@ int rhz_sdf_bake(const rhz_bake_item* items, int32_t n_items, const rhz_sdf_para
     std::vector<rhz_sdf_stats> st((size_t)n_items);
     std::vector<double> lat((size_t)n_items, 0.0);
     std::vector<int> rcs((size_t)n_items, RHZ_OK);
+    if (g_bake_batched.load() && n_items > 0 && bake_batchable(items, n_items, grids, params)) {
+        for (int i = 0; i < n_items; ++i) {
+            std::memset(&st[(size_t)i], 0, sizeof(rhz_sdf_stats));
+            for (int j = 0; j < 3; ++j) st[(size_t)i].dims[j] = grids[(size_t)i].dims[j];
+            st[(size_t)i].n_triangles = items[i].n_tris;
+        }
+        rc = bake_batched(items, n_items, params, grids, order, lanes, st, rcs, lat, entry, dev);
+        for (Lane* L : lanes) ws_reap(&L->ws);
+        if (rc) return rc;
+        int first = RHZ_OK;
+        for (int i = 0; i < n_items; ++i) {
+            if (stats) stats[i] = st[(size_t)i];
+            if (ms_item) ms_item[i] = lat[(size_t)i];
+            if (rcs[(size_t)i] < 0 && first == RHZ_OK) first = rcs[(size_t)i];
+        }
+        if (first < 0) return set_err(first, "a bake item failed (status in its stats)");
+        return RHZ_OK;
+    }
     std::atomic<int> next{0};
     auto worker = [&](int l) {
         (void)hipSetDevice(dev);

@@ -6,7 +6,17 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_bake_matches_single_builds(rhz, geom, torch_gpu):
+@pytest.fixture(params=[1, 0], ids=["batched_rounds", "lanes"])
+def bake_mode(rhz, request):
+    """Both bake modes: the eikonal rounds of all lattices as batched
+    launches (default), or every item's whole build on a lane."""
+    saved = rhz.tuning_get("bake_batched")
+    rhz.tuning_set("bake_batched", request.param)
+    yield request.param
+    rhz.tuning_set("bake_batched", saved)
+
+
+def test_bake_matches_single_builds(rhz, geom, torch_gpu, bake_mode):
     torch = torch_gpu
     meshes = geom.bake_batch(n=8, seed=77, tri_range=(2_000, 20_000), side_range=(32, 96))
     items, singles = [], []
@@ -39,7 +49,7 @@ def test_bake_matches_single_builds(rhz, geom, torch_gpu):
                 assert getattr(st, k) == getattr(rs, k), k
 
 
-def test_bake_after_growth_and_release(rhz, geom, torch_gpu):
+def test_bake_after_growth_and_release(rhz, geom, torch_gpu, bake_mode):
     """Lanes that outgrow their workspace retire the old buffers and free them
     after the call; a batch of larger lattices after a small one, and a bake
     after the workspaces were released, still equal the single builds."""
@@ -118,7 +128,7 @@ def test_fsm_batched_rounds_and_graph_match_round_loop(rhz, torch_gpu, request, 
 
 
 @pytest.mark.timeout(600)
-def test_full_batch_matches_single_builds_and_winding(rhz, geom, torch_gpu):
+def test_full_batch_matches_single_builds_and_winding(rhz, geom, torch_gpu, bake_mode):
     """The full batched-bake workload: 64 seeded vessels (2k-500k triangles,
     longest side 64-384 at 0.25-0.5 mm; lattices of >= 16M voxels run the
     overlapped sign stage inside their lane).  Every field equals its single
