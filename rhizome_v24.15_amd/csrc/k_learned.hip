@@ -12,6 +12,8 @@
 //     the top-ranked ones with a score histogram,
 //   * commit exact decisions, and grow the set around every mismatch found
 //     (26-neighbourhood, each voxel probed at most once: a bitmap).
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
 #include "rhz_device.hpp"
 #include "rhz_internal.hpp"
 
@@ -127,27 +129,53 @@ __device__ __forceinline__ bool segment_may_cross(float px, float py, float pz, 
 //  10 the certificate flag itself  11 distance from the voxel centre to the
 //  nearest crossing of its jittered column ray (voxels)
 // z extent of every triangle (the features kernel's cheap reject).
-__global__ void tri_zrange_kernel(const float* __restrict__ tris, int64_t nt, float2* __restrict__ zr) {
+// The largest z extent goes to *max_ext (atomic max on the bit pattern of a
+// non-negative float).
+__global__ void tri_zrange_kernel(const float* __restrict__ tris, int64_t nt, float2* __restrict__ zr,
+                                  unsigned* __restrict__ max_ext) {
+    float ext = 0.0f;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (int64_t)gridDim.x * blockDim.x) {
         const Tri9 v = load_tri(tris, t);
-        zr[t] = make_float2(fminf(v.az, fminf(v.bz, v.cz)), fmaxf(v.az, fmaxf(v.bz, v.cz)));
+        const float lo = fminf(v.az, fminf(v.bz, v.cz)), hi = fmaxf(v.az, fmaxf(v.bz, v.cz));
+        zr[t] = make_float2(lo, hi);
+        ext = fmaxf(ext, hi - lo);
     }
+    atomicMax(max_ext, __float_as_uint(ext));
+}
+
+// Order-preserving map of a float to an unsigned key (all non-NaN values).
+__device__ __forceinline__ uint32_t zkey_of(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// key[e] = (zkey(zmin of ids[e]) << 32) | ids[e]: sorting a supercell's keys
+// orders its list by lowest z, then triangle id.
+__global__ void zsort_keys_kernel(const int32_t* __restrict__ ids, int64_t n, const float2* __restrict__ zr,
+                                  unsigned long long* __restrict__ key) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int32_t t = ids[e];
+    key[e] = ((unsigned long long)zkey_of(zr[t].x) << 32) | (uint32_t)t;
 }
 
 // Every feature depends only on triangles within 4 h of the voxel (the
 // second-nearest distance is capped at 4 h, ray crossings at 2 h, the rest
 // are nearer), so a triangle whose z extent is farther than 4.004 h from the
-// voxel is skipped before its distance is computed; the features are the same
-// bits as with every triangle of the column list.  The column list is walked
-// kFeatBatch entries at a time, ids and z extents loaded together (the walk
-// is latency-bound); entries are still taken in list order, which the
-// order-dependent features (the first of tied nearest triangles, the first
-// risky one) need.
+// voxel is skipped before its distance is computed.  The column list is
+// sorted by (lowest z, triangle id), so the walk starts at the first entry
+// whose lowest z reaches pz - reach - (largest z extent) and stops after the
+// last whose lowest z is within pz + reach, both widened by a relative margin
+// far above float rounding: every entry left out would fail the exact reach
+// test, so the same triangles pass it as with the whole list, and they are
+// taken in list order (which fixes the first of tied nearest triangles and
+// the order of the normal sums).  The window is walked kFeatBatch entries at
+// a time, ids and z extents loaded together (the walk is latency-bound).
 constexpr int kFeatBatch = 8;  // column-list entries whose loads are issued together
 
 __global__ void features_kernel(const float* __restrict__ tris, const float4* __restrict__ xyab,
                                 const float2* __restrict__ zr, const int64_t* __restrict__ start,
-                                const int32_t* __restrict__ ids,
+                                const unsigned long long* __restrict__ keys, const unsigned* __restrict__ max_ext,
                                 const uint8_t* __restrict__ sign, const int64_t* __restrict__ list, int64_t n,
                                 int mnx, int mny, int mnz, int dx, int dy, float h, int sup, int sdx, float jx,
                                 float jy, float* __restrict__ feats, uint8_t* __restrict__ risk,
@@ -174,12 +202,33 @@ __global__ void features_kernel(const float* __restrict__ tris, const float4* __
     const float half_len = 0.5f * sqrtf(jx * jx + jy * jy) * h;
     const int64_t s = (int64_t)(y / sup) * sdx + (x / sup);
     const float reach = 4.004f * h;
-    const int64_t l0 = start[s], l1 = start[s + 1];
+    const float ext = __uint_as_float(*max_ext);
+    const float margin = 0x1.0p-10f * (fabsf(pz) + reach + ext);
+    const uint32_t klo = zkey_of(pz - reach - ext - margin), khi = zkey_of(pz + reach + margin);
+    int64_t l0 = start[s], l1 = start[s + 1];
+    {
+        // first entry with lowest-z key >= klo
+        int64_t a = l0, b = l1;
+        while (a < b) {
+            const int64_t m = (a + b) >> 1;
+            if ((uint32_t)(keys[m] >> 32) < klo) a = m + 1;
+            else b = m;
+        }
+        l0 = a;
+        // first entry with lowest-z key > khi
+        b = l1;
+        while (a < b) {
+            const int64_t m = (a + b) >> 1;
+            if ((uint32_t)(keys[m] >> 32) <= khi) a = m + 1;
+            else b = m;
+        }
+        l1 = a;
+    }
     for (int64_t eb = l0; eb < l1; eb += kFeatBatch) {
         // every index is clamped into [l0, l1), so the loads need no guard
         int tid[kFeatBatch];
 #pragma unroll
-        for (int j = 0; j < kFeatBatch; ++j) tid[j] = ids[min(eb + j, l1 - 1)];
+        for (int j = 0; j < kFeatBatch; ++j) tid[j] = (int)(uint32_t)keys[min(eb + j, l1 - 1)];
         bool in_reach[kFeatBatch];
 #pragma unroll
         for (int j = 0; j < kFeatBatch; ++j) {
@@ -390,15 +439,39 @@ hipError_t launch_candidates(const float* d, int64_t n, float thr, int64_t* list
     return hipGetLastError();
 }
 
-hipError_t launch_features(const float* tris, int64_t nt, const CsrBuffers& b, float2* zr, const uint8_t* sign,
+size_t zsort_scratch_bytes(int64_t entries, int64_t nsup, const int64_t* start) {
+    size_t bytes = 0;
+    if (rocprim::segmented_radix_sort_keys(nullptr, bytes, (const unsigned long long*)nullptr,
+                                           (unsigned long long*)nullptr, (unsigned)entries, (unsigned)nsup, start,
+                                           start + 1) != hipSuccess)
+        return 0;
+    return bytes + 256;
+}
+
+hipError_t launch_zsort_lists(const float* tris, int64_t nt, const CsrBuffers& b, int64_t entries, int64_t nsup,
+                              float2* zr, unsigned* max_ext, unsigned long long* keys_tmp, unsigned long long* keys,
+                              void* scratch, size_t scratch_bytes, hipStream_t s) {
+    if (nt <= 0 || entries <= 0) return hipSuccess;
+    hipError_t e = hipMemsetAsync(max_ext, 0, sizeof(unsigned), s);
+    if (e != hipSuccess) return e;
+    tri_zrange_kernel<<<(unsigned)std::min<int64_t>((nt + 255) / 256, 2048), 256, 0, s>>>(tris, nt, zr, max_ext);
+    zsort_keys_kernel<<<(unsigned)((entries + 255) / 256), 256, 0, s>>>(b.ids, entries, zr, keys_tmp);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t bytes = scratch_bytes;
+    return rocprim::segmented_radix_sort_keys(scratch, bytes, keys_tmp, keys, (unsigned)entries, (unsigned)nsup,
+                                              b.start, b.start + 1, 0, 64, s);
+}
+
+hipError_t launch_features(const float* tris, int64_t nt, const CsrBuffers& b, const float2* zr,
+                           const unsigned long long* keys, const unsigned* max_ext, const uint8_t* sign,
                            const int64_t* list, int64_t n, const Grid& g, int sup, float jx, float jy, float* feats,
                            uint8_t* risk, unsigned long long* n_risky, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int sdx = (g.dims[0] + sup - 1) / sup;
-    tri_zrange_kernel<<<(unsigned)std::min<int64_t>((nt + 255) / 256, 2048), 256, 0, s>>>(tris, nt, zr);
-    features_kernel<<<(unsigned)((n + 127) / 128), 128, 0, s>>>(tris, b.xyab, zr, b.start, b.ids, sign, list, n, g.mn[0],
-                                                              g.mn[1], g.mn[2], g.dims[0], g.dims[1], g.h, sup, sdx,
-                                                              jx, jy, feats, risk, n_risky);
+    features_kernel<<<(unsigned)((n + 127) / 128), 128, 0, s>>>(tris, b.xyab, zr, b.start, keys, max_ext, sign, list, n,
+                                                              g.mn[0], g.mn[1], g.mn[2], g.dims[0], g.dims[1], g.h, sup,
+                                                              sdx, jx, jy, feats, risk, n_risky);
     return hipGetLastError();
 }
 

@@ -259,8 +259,17 @@ hipError_t launch_scatter_f64(const double* src, const int64_t* pos, int64_t m, 
 
 hipError_t launch_candidates(const float* d, int64_t n, float thr, int64_t* list, int64_t cap,
                              unsigned long long* counter, hipStream_t s);
-// 12 features per candidate (row-major n x 12) from the column candidate lists.
-hipError_t launch_features(const float* tris, int64_t nt, const CsrBuffers& b, float2* zr, const uint8_t* sign,
+// The column candidate lists in (lowest z, triangle id) order for the
+// features: zr = every triangle's z extent, *max_ext = the largest, keys =
+// the sorted (zkey << 32 | id) entries (keys_tmp: the unsorted ones);
+// scratch: zsort_scratch_bytes.
+size_t zsort_scratch_bytes(int64_t entries, int64_t nsup, const int64_t* start);
+hipError_t launch_zsort_lists(const float* tris, int64_t nt, const CsrBuffers& b, int64_t entries, int64_t nsup,
+                              float2* zr, unsigned* max_ext, unsigned long long* keys_tmp, unsigned long long* keys,
+                              void* scratch, size_t scratch_bytes, hipStream_t s);
+// 12 features per candidate (row-major n x 12) from the z-sorted lists.
+hipError_t launch_features(const float* tris, int64_t nt, const CsrBuffers& b, const float2* zr,
+                           const unsigned long long* keys, const unsigned* max_ext, const uint8_t* sign,
                            const int64_t* list, int64_t n, const Grid& g, int sup, float jx, float jy, float* feats,
                            uint8_t* risk, unsigned long long* n_risky, hipStream_t s);
 hipError_t launch_unprobed_risky(const int64_t* list, const uint8_t* risk, int64_t n, const unsigned* bits,

@@ -137,6 +137,10 @@ enum Slot {
     S_TP_P1,
     S_TP_Q1,
     S_TP_VOL,      // signed-volume partial sums
+    S_ZKEY,        // candidate lists sorted by (lowest z, id), unsorted keys, sort scratch, largest z extent
+    S_ZKEY_TMP,
+    S_ZSORT,
+    S_ZEXT,
     S_COUNT
 };
 
@@ -1150,10 +1154,21 @@ int learned_features(Ctx& c, uint8_t* sign, const CsrBuffers& b, const int64_t* 
     float* f = (float*)ws_get(c.w, S_FEAT, sizeof(float) * 12 * (size_t)std::max<int64_t>(n, 1));
     uint8_t* r = (uint8_t*)ws_get(c.w, S_RISK, (size_t)std::max<int64_t>(n, 1));
     float2* zr = (float2*)ws_get(c.w, S_ZRANGE, sizeof(float2) * (size_t)std::max<int64_t>(c.nt, 1));
-    if (!f || !r || !zr) return set_err(RHZ_ERR_NOMEM, "feature buffer allocation failed");
+    const int64_t entries = c.st->csr_entries;
+    const int64_t nsup = num_supercells(c.g, c.p->sup);
+    unsigned long long* keys = (unsigned long long*)ws_get(c.w, S_ZKEY, sizeof(unsigned long long) * (size_t)entries);
+    unsigned long long* keys_tmp =
+        (unsigned long long*)ws_get(c.w, S_ZKEY_TMP, sizeof(unsigned long long) * (size_t)entries);
+    unsigned* max_ext = (unsigned*)ws_get(c.w, S_ZEXT, sizeof(unsigned));
+    const size_t sort_bytes = zsort_scratch_bytes(entries, nsup, b.start);
+    void* sort_scr = sort_bytes ? ws_get(c.w, S_ZSORT, sort_bytes) : nullptr;
+    if (!f || !r || !zr || !keys || !keys_tmp || !max_ext || !sort_scr)
+        return set_err(RHZ_ERR_NOMEM, "feature buffer allocation failed");
+    HIP_TRY(launch_zsort_lists(c.tris, c.nt, b, entries, nsup, zr, max_ext, keys_tmp, keys, sort_scr, sort_bytes,
+                               c.s));
     HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_RISKY), 0, sizeof(unsigned long long), c.s));
-    HIP_TRY(launch_features(c.tris, c.nt, b, zr, sign, list, n, c.g, c.p->sup, c.p->jitter_x, c.p->jitter_y, f, r,
-                            cnt_ptr(c.w, C_RISKY), c.s));
+    HIP_TRY(launch_features(c.tris, c.nt, b, zr, keys, max_ext, sign, list, n, c.g, c.p->sup, c.p->jitter_x,
+                            c.p->jitter_y, f, r, cnt_ptr(c.w, C_RISKY), c.s));
     const int rc = read_counters(c);
     if (rc) return rc;
     *feats = f;
