@@ -350,9 +350,10 @@ def main():
     ap.add_argument("--h", type=float, default=0.10)
     ap.add_argument("--method", default="learned", choices=["fast", "learned", "plain", "legacy"])
     ap.add_argument("--fsm", default="reference", choices=["reference", "converge"])
-    ap.add_argument("--scorer", default="network", choices=["network", "margin", "certificate"],
-                    help="learned method: rank with the proposer network, the planar-margin rule, or "
-                         "nothing (exact probes of the certificate-flagged voxels only)")
+    ap.add_argument("--scorer", default="network", choices=["network", "margin", "certificate", "flag"],
+                    help="learned method: rank with the proposer network, the planar-margin rule, "
+                         "nothing (exact probes of the certificate-flagged voxels only), or the "
+                         "certificate flag alone as the ranking")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the accuracy report, the sign check and the rounds curve")
@@ -410,7 +411,7 @@ def main():
     d_tris = torch.from_numpy(tris).to("cuda")
     d_out = torch.empty(V, dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
-    scorer = {"network": 0, "margin": 1, "certificate": 2}[args.scorer]
+    scorer = {"network": 0, "margin": 1, "certificate": 2, "flag": 3}[args.scorer]
     params = rhz.default_params(method=method, fsm_policy=fsm, scorer=scorer)
 
     def step():

@@ -97,7 +97,9 @@ typedef struct rhz_sdf_params {
                                      voxels; 0 = the jitter reach |(jitter_x, jitter_y)| (+0.1%),
                                      beyond which a scanline sign cannot be wrong */
     int32_t scorer;               /* FAST_LEARNED: 0 proposer network, 1 planar-margin rule,
-                                     2 stratified (no ranking: every k-th candidate) */
+                                     2 blind (no ranking: k candidates chosen by a hash of
+                                     their voxel index), 3 certificate flag alone (flagged
+                                     first, each class in the same hashed order) */
     int32_t learned_certify;      /* FAST_LEARNED: 1 = also probe every candidate whose voxel-to-ray
                                      segment may cross a triangle and that the ranking missed
                                      (0 sign errors on closed meshes); 0 = ranked probes only */

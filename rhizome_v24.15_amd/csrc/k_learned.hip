@@ -333,6 +333,24 @@ __global__ void margin_score_kernel(const float* __restrict__ feats, int64_t n, 
     if (i < n) score[i] = feats[12 * i + 6];
 }
 
+// Rankings without a model: a uniform position in [0, 0.5) drawn from a
+// splitmix64 hash of the candidate's voxel index (not of its list position,
+// which the atomic compaction leaves run-dependent), plus the certificate flag
+// (feature 10) when `feats` is given: flagged candidates then score in
+// [1, 1.5) and rank first, each class in hashed order.  Without `feats` the
+// top k are a blind uniform subset of the candidates.
+__global__ void hash_score_kernel(const float* __restrict__ feats, const int64_t* __restrict__ cand, int64_t n,
+                                  float* __restrict__ score) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long z = 0x9E3779B97F4A7C15ull * (unsigned long long)(cand[i] + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const float u = (float)(z >> 40) * 0x1.0p-24f;  // [0, 1)
+    score[i] = (feats ? feats[12 * i + 10] : 0.0f) + 0.5f * u;
+}
+
 constexpr int kBins = 4096;
 
 __device__ __forceinline__ int score_bin(float s, float lo, float inv_w) {
@@ -485,6 +503,12 @@ hipError_t launch_unprobed_risky(const int64_t* list, const uint8_t* risk, int64
 hipError_t launch_margin_score(const float* feats, int64_t n, float* score, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     margin_score_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(feats, n, score);
+    return hipGetLastError();
+}
+
+hipError_t launch_hash_score(const float* feats, const int64_t* cand, int64_t n, float* score, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hash_score_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(feats, cand, n, score);
     return hipGetLastError();
 }
 

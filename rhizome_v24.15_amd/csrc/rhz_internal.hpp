@@ -275,6 +275,9 @@ hipError_t launch_features(const float* tris, int64_t nt, const CsrBuffers& b, c
 hipError_t launch_unprobed_risky(const int64_t* list, const uint8_t* risk, int64_t n, const unsigned* bits,
                                  int64_t* out, int64_t cap, unsigned long long* counter, hipStream_t s);
 hipError_t launch_margin_score(const float* feats, int64_t n, float* score, hipStream_t s);
+// A uniform position in [0, 0.5) hashed from each candidate's voxel index, plus
+// the certificate flag (feature 10) when `feats` is not null.
+hipError_t launch_hash_score(const float* feats, const int64_t* cand, int64_t n, float* score, hipStream_t s);
 int score_bins();
 hipError_t launch_score_hist(const float* score, int64_t n, float lo, float hi, unsigned* hist, hipStream_t s);
 hipError_t launch_select(const float* score, const int64_t* list, int64_t n, float lo, float hi, int min_bin,

@@ -1186,30 +1186,12 @@ int learned_select(Ctx& c, const int64_t* cand, const float* feats, int64_t n, i
     if (!out) return set_err(RHZ_ERR_NOMEM, "selection buffer allocation failed");
     *sel = out;
     if (n == 0 || (k < 0 && p->scorer == 2)) {
-        // nothing ranked (a stratified scorer has no positives)
+        // nothing ranked (a blind scorer has no positives)
         *n_sel = 0;
         return RHZ_OK;
     }
-    if (k >= n || (p->scorer == 2 && k >= 0)) {
-        // everything, or a stratified subset without ranking (every n/k-th candidate)
-        const int64_t m = (k >= n) ? n : k;
-        if (m == n) {
-            HIP_TRY(hipMemcpyAsync(out, cand, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToDevice, c.s));
-        } else {
-            float* score = (float*)ws_get(c.w, S_SCORE, sizeof(float) * (size_t)n);
-            if (!score) return set_err(RHZ_ERR_NOMEM, "score buffer allocation failed");
-            // stratified: score 1 for every (n/m)-th candidate, 0 otherwise
-            std::vector<float> hs((size_t)n, 0.0f);
-            const double stride = (double)n / (double)m;
-            for (int64_t i = 0; i < m; ++i) hs[(size_t)std::min<int64_t>(n - 1, (int64_t)((i + 0.5) * stride))] = 1.0f;
-            HIP_TRY(hipMemcpyAsync(score, hs.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice, c.s));
-            HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_SEL), 0, sizeof(unsigned long long), c.s));
-            HIP_TRY(launch_select(score, cand, n, 0.0f, 1.0f, score_bins() - 1, out, n, cnt_ptr(c.w, C_SEL), c.s));
-            int rc = read_counters(c);
-            if (rc) return rc;
-            *n_sel = (int64_t)hc(c, C_SEL);
-            return RHZ_OK;
-        }
+    if (k >= n) {
+        HIP_TRY(hipMemcpyAsync(out, cand, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToDevice, c.s));
         *n_sel = n;
         return RHZ_OK;
     }
@@ -1228,6 +1210,12 @@ int learned_select(Ctx& c, const int64_t* cand, const float* feats, int64_t n, i
         HIP_TRY(launch_mlp_forward(dblob, feats, n, score, c.s));
         lo = -40.0f;
         hi = 40.0f;
+    } else if (p->scorer == 2 || p->scorer == 3) {
+        // blind hashed subset (scores in [0, 0.5)), or the certificate flag
+        // alone (flagged in [1, 1.5), the others below)
+        HIP_TRY(launch_hash_score(p->scorer == 3 ? feats : nullptr, cand, n, score, c.s));
+        lo = 0.0f;
+        hi = p->scorer == 3 ? 1.5f : 0.5f;
     } else {
         // margin = s_out * t - d / h lies in [-0.17, 0.063] for candidates
         HIP_TRY(launch_margin_score(feats, n, score, c.s));
@@ -1236,8 +1224,10 @@ int learned_select(Ctx& c, const int64_t* cand, const float* feats, int64_t n, i
     }
     int min_bin = 0;
     if (k < 0) {
-        // classifier mode: every candidate scored positive (bin of score 0)
-        min_bin = (int)std::floor((0.0 - lo) * score_bins() / (hi - lo));
+        // classifier mode: every candidate scored positive (bin of score 0;
+        // for the flag ranking, the flagged ones)
+        const double pos = (p->scorer == 3) ? 1.0 : 0.0;
+        min_bin = (int)std::floor((pos - lo) * score_bins() / (hi - lo));
     } else {
         HIP_TRY(launch_score_hist(score, n, lo, hi, hist, c.s));
         std::vector<unsigned> hh((size_t)score_bins());

@@ -10,7 +10,9 @@ kernel (compensated FP32, FP64 near 1/2) disagrees with the scanline sign.
 Training: weighted binary cross-entropy, Adam, minibatches, all through
 ``rhz_mlp_grad`` / ``rhz_mlp_adam`` (BF16 matrix-core forward and backward).
 The result is written as ``csrc/proposer_weights.inc`` (the built-in
-parameter blob the library compiles in) plus a JSON summary.
+parameter blob the library compiles in) plus a JSON summary; a named variant
+(``--variant noflag --drop-feature 10``: a network that never sees the
+certificate flag) is written as ``proposer_<variant>.npy`` instead.
 
 Usage (needs a GPU):
     python rhizome_v24.15_amd/proposer_train.py --meshes 200 --out DIR
@@ -57,7 +59,7 @@ def mesh_samples(rhz, geom, torch, seed):
     """Features and labels of one seeded random vessel (GPU tensors)."""
     rng = np.random.default_rng(seed)
     target = int(math.exp(rng.uniform(math.log(50_000), math.log(300_000))))
-    v, f = geom.random_vessel(seed, target)
+    v, f = geom.random_vessel(seed, target, tri_bounds=(50_000, 300_000))
     span = float(np.max(v.max(0) - v.min(0)))
     h = float(np.float32(span / rng.uniform(100.0, 300.0)))
     mn, mx = geom.frame_grid(v, h)
@@ -102,12 +104,15 @@ def evaluate(rhz, torch, d_blob, X, Y):
             "false_neg": fn, "precision": tp / max(tp + fp, 1), "recall": tp / max(tp + fn, 1)}
 
 
-def train(rhz, torch, X, Y, Xv, Yv, epochs, batch, lr, pos_weight, seed, log):
+def train(rhz, torch, X, Y, Xv, Yv, epochs, batch, lr, pos_weight, seed, log, drop=()):
     n_param = rhz.Proposer.param_count()
     n_blob = rhz.Proposer.blob_count()
     mean = X.mean(0).cpu().numpy()
     std = X.std(0).cpu().numpy()
     istd = np.where(std > 1e-6, 1.0 / np.maximum(std, 1e-6), 1.0).astype(np.float32)
+    for k in drop:  # a dropped feature normalises to 0 whatever its value: the network never sees it
+        mean[k] = 0.0
+        istd[k] = 0.0
     blob = init_blob(n_blob, n_param, seed, mean.astype(np.float32), istd)
     d_blob = torch.from_numpy(blob).cuda()
     d_m = torch.zeros(n_param, dtype=torch.float32, device="cuda")
@@ -169,7 +174,13 @@ def main():
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--pos-weight", type=float, default=2.0)
     ap.add_argument("--seed", type=int, default=17)
-    ap.add_argument("--out", default=os.path.join(HERE, "csrc"))
+    ap.add_argument("--out", default=HERE,
+                    help="directory of the outputs: csrc/proposer_weights.inc (built-in variant) or "
+                         "proposer_<variant>.npy, and proposer_[<variant>_]training.json")
+    ap.add_argument("--variant", default="",
+                    help="name of a non-built-in variant (its blob goes to proposer_<variant>.npy)")
+    ap.add_argument("--drop-feature", type=int, action="append", default=[],
+                    help="feature index the network must not see (repeatable)")
     args = ap.parse_args()
 
     def log(*a):
@@ -187,15 +198,22 @@ def main():
     Xv, Yv, meta_v = generate(rhz, geom, torch, val_seeds, log)
     log(f"data: {len(X)} train / {len(Xv)} val samples, positive rate {float(Y.mean()):.4f}; "
         f"{time.time() - t0:.1f} s")
-    blob, hist = train(rhz, torch, X, Y, Xv, Yv, args.epochs, args.batch, args.lr, args.pos_weight, args.seed, log)
+    blob, hist = train(rhz, torch, X, Y, Xv, Yv, args.epochs, args.batch, args.lr, args.pos_weight, args.seed, log,
+                       drop=args.drop_feature)
     summary = {"train_meshes": len(meta), "val_meshes": len(meta_v), "train_samples": int(len(X)),
                "val_samples": int(len(Xv)), "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
-               "pos_weight": args.pos_weight, "seed": args.seed, "val": hist[-1], "history": hist,
-               "seconds": time.time() - t0,
+               "pos_weight": args.pos_weight, "seed": args.seed, "dropped_features": args.drop_feature,
+               "val": hist[-1], "history": hist, "seconds": time.time() - t0,
                "train_tris_range": [min(m["tris"] for m in meta), max(m["tris"] for m in meta)]}
     os.makedirs(args.out, exist_ok=True)
-    write_inc(blob, os.path.join(args.out, "proposer_weights.inc"), summary)
-    with open(os.path.join(args.out, "proposer_training.json"), "w") as fh:
+    if args.variant:
+        np.save(os.path.join(args.out, f"proposer_{args.variant}.npy"), blob.astype(np.float32))
+        name = f"proposer_{args.variant}_training.json"
+    else:
+        os.makedirs(os.path.join(args.out, "csrc"), exist_ok=True)
+        write_inc(blob, os.path.join(args.out, "csrc", "proposer_weights.inc"), summary)
+        name = "proposer_training.json"
+    with open(os.path.join(args.out, name), "w") as fh:
         json.dump(summary, fh, indent=1)
     log(json.dumps({k: v for k, v in summary.items() if k != "history"}))
 

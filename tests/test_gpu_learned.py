@@ -228,7 +228,7 @@ def test_certificate_covers_every_scanline_error(rhz, geom, torch_gpu, request, 
 
 
 def test_ranked_probes_alone_can_miss(rhz, small_sphere):
-    """Without the certificate a blind (stratified) probe budget leaves
+    """Without the certificate a blind (hashed subset) probe budget leaves
     scanline errors behind; with it the result is exact again."""
     tris, h, mn, mx = small_sphere
     ref = _plain_signs(rhz, tris, h, mn, mx)
@@ -241,7 +241,7 @@ def test_ranked_probes_alone_can_miss(rhz, small_sphere):
         assert rc == rhz.RHZ_OK
         res[scorer] = int(np.sum((np.signbit(got.reshape(-1)) != np.signbit(ref)) & ~on_surface))
         assert st.fallback_probes == 0
-    # the planar-margin ranking leaves fewer errors than blind stratified probes
+    # the planar-margin ranking leaves fewer errors than blind probes
     assert res[2] > 0
     assert res[1] < res[2], res
 
@@ -250,7 +250,7 @@ def test_network_gain_over_blind_probes(rhz, ico5_128):
     """The trained proposer against the method without it.  Config: the
     perturbed icosphere on 128^3 (a shape family the proposer never saw in
     training), top 0.1% of voxels re-probed exactly, certificate off: the
-    network leaves far fewer scanline errors behind than blind stratified
+    network leaves far fewer scanline errors behind than blind
     probes of the same budget.  With the certificate on (default), every
     scorer ends at 0 errors and the network spends fewer ranked probes than
     the planar-margin rule."""
@@ -267,7 +267,7 @@ def test_network_gain_over_blind_probes(rhz, ico5_128):
         assert rc == rhz.RHZ_OK
         resid[scorer] = int(np.sum((np.signbit(got.reshape(-1)) != np.signbit(ref)) & ~on_surface))
         stats[scorer] = st.ranked_probes
-    print("residual errors at a 0.1% budget: network", resid[0], "margin", resid[1], "stratified", resid[2],
+    print("residual errors at a 0.1% budget: network", resid[0], "margin", resid[1], "blind", resid[2],
           "probes", stats)
     assert resid[2] > 0
     assert resid[0] * 4 <= resid[2]

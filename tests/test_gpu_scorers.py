@@ -1,0 +1,125 @@
+"""What the proposer network itself buys.
+
+Five rankings of the scanline-sign candidates get the same exact-probe budget
+(the top 0.1% of all voxels), with the segment certificate off, so that
+whatever a ranking misses stays wrong:
+
+- ``network``: the built-in trained MLP over all 12 features;
+- ``noflag``: the same architecture retrained without feature 10 (the
+  certificate flag), shipped as ``proposer_noflag.npy``;
+- ``flag``: the certificate flag alone (flagged candidates first);
+- ``margin``: the planar-margin rule;
+- ``blind``: no ranking, k candidates chosen by a hash of their voxel index.
+
+Residual sign errors are counted against the plain exact-winding method.  The
+perturbed icosphere on 128^3 at 0.5 mm and the vessel at 0.5 mm are both
+shapes the network never saw in training (it was trained on random vessels
+of 50k-300k triangles at other seeds)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PKG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "rhizome_v24.15_amd")
+RANKINGS = ("network", "noflag", "flag", "margin", "blind")
+SCORER = {"network": 0, "noflag": 0, "flag": 3, "margin": 1, "blind": 2}
+
+
+def _case(request, geom, rhz, name):
+    if name == "ico5_128":
+        return request.getfixturevalue("ico5_128")
+    v, f = geom.vessel()
+    tris = rhz.build_triangles(v, f)
+    mn, mx = geom.frame_grid(v, 0.5)
+    return tris, 0.5, mn, mx
+
+
+def _reference(rhz, tris, h, mn, mx):
+    rc, ref, _ = rhz.compute_sdf(tris, h, mn, mx, rhz.default_params(method=rhz.METHOD_PLAIN_EXACT))
+    assert rc == rhz.RHZ_OK
+    ref = ref.reshape(-1)
+    return ref, np.abs(ref) <= 1e-5 * h
+
+
+def _residuals(rhz, tris, h, mn, mx, ref, on_surface, **kw):
+    noflag = np.ascontiguousarray(np.load(os.path.join(PKG, "proposer_noflag.npy")), dtype=np.float32)
+    assert noflag.size == rhz.Proposer.blob_count()
+    out = {}
+    for name in RANKINGS:
+        p = rhz.default_params(method=rhz.METHOD_FAST_LEARNED, scorer=SCORER[name], **kw)
+        if name == "noflag":
+            p.mlp_weights = noflag.ctypes.data_as(C.c_void_p)
+        rc, got, st = rhz.compute_sdf(tris, h, mn, mx, p)
+        assert rc == rhz.RHZ_OK
+        wrong = int(np.sum((np.signbit(got.reshape(-1)) != np.signbit(ref)) & ~on_surface))
+        out[name] = (wrong, int(st.ranked_probes), int(st.fallback_probes), int(st.risky_candidates),
+                     int(st.candidates))
+    return out
+
+
+# exact-probe budgets as fractions of all voxels: 0.1% and four tighter ones
+BUDGETS = (1e-3, 3e-4, 1.5e-4, 7.5e-5, 3.5e-5)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("case", ["ico5_128", "vessel_0p5mm"])
+def test_rankings_at_a_fixed_budget(rhz, geom, torch_gpu, request, case):
+    if rhz.Proposer.builtin() is None:
+        pytest.skip("library built without trained proposer weights")
+    tris, h, mn, mx = _case(request, geom, rhz, case)
+    ref, on_surface = _reference(rhz, tris, h, mn, mx)
+    # the scanline errors before any probe (a budget of one voxel)
+    unprobed = _residuals(rhz, tris, h, mn, mx, ref, on_surface, learned_certify=0, reprobe_fraction=1e-12)
+    n_err = unprobed["blind"][0]
+    print(f"{case}: voxels {ref.size} candidates {unprobed['network'][4]} flagged {unprobed['network'][3]} "
+          f"scanline errors {n_err}", flush=True)
+    table = {}
+    for frac in BUDGETS:
+        res = _residuals(rhz, tris, h, mn, mx, ref, on_surface, learned_certify=0, reprobe_fraction=frac)
+        table[frac] = res
+        print(f"{case} budget {frac:g} (~{res['blind'][1]} probes): " +
+              " ".join(f"{n} {res[n][0]}" for n in RANKINGS), flush=True)
+        # the same budget for every ranking (to the width of one score-histogram
+        # bin), none of it spent outside the ranking
+        probes = [res[n][1] for n in RANKINGS]
+        assert max(probes) <= min(probes) * 1.02 + 4, probes
+        assert all(res[n][2] == 0 for n in RANKINGS)
+        assert all(res[n][0] <= n_err + 1 for n in RANKINGS)
+    for frac, res in table.items():
+        # the network over all features ranks at least as well as the flag
+        # alone and as the network without the flag, at every budget
+        assert res["network"][0] <= min(res["flag"][0], res["noflag"][0]), frac
+        assert res["flag"][0] <= res["blind"][0], frac
+        if res["flag"][0] > 0:
+            assert res["network"][0] < res["flag"][0], frac
+    # every ranking is a function of the voxels alone, not of the run-dependent
+    # order of the candidate list: a second run selects the same probes
+    again = _residuals(rhz, tris, h, mn, mx, ref, on_surface, learned_certify=0, reprobe_fraction=BUDGETS[1])
+    assert again == table[BUDGETS[1]]
+    assert table[BUDGETS[0]]["blind"][0] > 0
+    for frac, res in table.items():
+        # errors removed by a budget smaller than the errors: the network at
+        # least twice as many as blind probes
+        if res["blind"][1] < n_err:
+            assert (n_err - res["network"][0]) >= 2 * (n_err - res["blind"][0]), frac
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("case", ["ico5_128", "vessel_0p5mm"])
+def test_classifier_mode_probe_counts(rhz, geom, torch_gpu, request, case):
+    """Certificate on, no budget (every candidate a scorer calls positive,
+    then every flagged candidate it missed): every ranking ends at 0 errors;
+    the counts show how many probes each spends to get there."""
+    if rhz.Proposer.builtin() is None:
+        pytest.skip("library built without trained proposer weights")
+    tris, h, mn, mx = _case(request, geom, rhz, case)
+    ref, on_surface = _reference(rhz, tris, h, mn, mx)
+    res = _residuals(rhz, tris, h, mn, mx, ref, on_surface)
+    for name in RANKINGS:
+        wrong, ranked, fallback, risky, cand = res[name]
+        print(f"{case} {name}: residual {wrong} ranked {ranked} fallback {fallback} total {ranked + fallback}",
+              flush=True)
+        assert wrong == 0, name
