@@ -352,7 +352,12 @@ int rhz_mlp_adam(float* d_blob, const float* d_grads, float* d_m, float* d_v, in
  * fixup): up to cap voxel ids, cap x 12 features and cap flags (1 = the
  * voxel-to-ray segment may cross a triangle) into device buffers (any may be
  * NULL); d_scan (optional, V bytes) receives the scanline signs (1 inside);
- * *n_out = total count (may exceed cap). */
+ * *n_out = total count (may exceed cap).  Features, in order: 0 nearest
+ * distance / h, 1 scan sign (z ray), 2-4 nearest normal, 5 inside by a +x
+ * ray, 6 planar margin, 7 inside by a +y ray (5 and 7: signed crossing
+ * counts, 0.5 for a line whose crossings do not cancel), 8 log1p(triangles
+ * within h), 9 normal spread within 1.5 h, 10 certificate flag, 11 distance
+ * to the nearest crossing of the voxel's z ray / h (capped at 2). */
 int rhz_learned_candidates(const rhz_triangle* d_tris, int64_t n_tris, float h, const int32_t min_c[3],
                            const int32_t max_c[3], const rhz_sdf_params* params, int64_t* d_voxels,
                            float* d_feats, uint8_t* d_risk, uint8_t* d_scan, int64_t cap, int64_t* n_out,

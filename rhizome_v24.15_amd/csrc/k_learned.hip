@@ -123,8 +123,9 @@ __device__ __forceinline__ bool segment_may_cross(float px, float py, float pz, 
 // Features of candidate voxel v (nearest triangles searched in the voxel's
 // scanline supercell list), and risk[i] = 1 when the segment certificate
 // above cannot rule out a scanline sign error:
-//  0 d/h  1 scan sign  2-4 nearest-triangle normal  5 jitter offset along the
-//  normal (voxels)  6 planar margin  7 second-nearest distance/h
+//  0 d/h  1 scan sign (the z ray)  2-4 nearest-triangle normal
+//  5 inside by the signed crossings of a +x ray (axis_parity_kernel)
+//  6 planar margin  7 inside by a +y ray (axis_parity_kernel)
 //  8 log1p(#triangles within 1 h)  9 normal spread within 1.5 h
 //  10 the certificate flag itself  11 distance from the voxel centre to the
 //  nearest crossing of its jittered column ray (voxels)
@@ -159,10 +160,11 @@ __global__ void zsort_keys_kernel(const int32_t* __restrict__ ids, int64_t n, co
     key[e] = ((unsigned long long)zkey_of(zr[t].x) << 32) | (uint32_t)t;
 }
 
-// Every feature depends only on triangles within 4 h of the voxel (the
-// second-nearest distance is capped at 4 h, ray crossings at 2 h, the rest
-// are nearer), so a triangle whose z extent is farther than 4.004 h from the
-// voxel is skipped before its distance is computed.  The column list is
+// Every feature of this kernel depends only on triangles within 4 h of the
+// voxel (ray crossings are capped at 2 h, the rest are nearer; the nearest
+// distance of a candidate is below the candidate radius), so a triangle whose
+// z extent is farther than 4.004 h from the voxel is skipped before its
+// distance is computed.  The column list is
 // sorted by (lowest z, triangle id), so the walk starts at the first entry
 // whose lowest z reaches pz - reach - (largest z extent) and stops after the
 // last whose lowest z is within pz + reach, both widened by a relative margin
@@ -191,7 +193,7 @@ __global__ void features_kernel(const float* __restrict__ tris, const float4* __
     const float rx = ((float)(mnx + x) + jx) * h;
     const float ry = ((float)(mny + y) + jy) * h;
     const float h2 = h * h;
-    float best = 3.0e38f, second = 3.0e38f;
+    float best = 3.0e38f;
     int best_t = -1;
     int near_cnt = 0;
     float sx = 0.f, sy = 0.f, sz = 0.f;
@@ -246,11 +248,8 @@ __global__ void features_kernel(const float* __restrict__ tris, const float4* __
                 risky = segment_may_cross(px, py, pz, rx, ry, t, tn.y, tn.z, tn.w, eps, half_len);
             }
             if (d2 < best) {
-                second = best;
                 best = d2;
                 best_t = ti;
-            } else if (d2 < second) {
-                second = d2;
             }
             if (d2 <= h2) ++near_cnt;
             if (d2 <= 2.25f * h2) {
@@ -295,15 +294,103 @@ __global__ void features_kernel(const float* __restrict__ tris, const float4* __
     f[2] = nx;
     f[3] = ny;
     f[4] = nz;
-    f[5] = t_off;
-    f[6] = s_out * t_off - dist;
-    f[7] = fminf(sqrtf(second) / h, 4.0f);
+    f[6] = s_out * t_off - dist;  // f[5] and f[7]: axis_parity_kernel
     f[8] = log1pf((float)near_cnt);
     f[9] = spread;
     f[10] = risky ? 1.0f : 0.0f;
     f[11] = fminf(zhit / h, 2.0f);
     risk[i] = risky ? 1 : 0;
     wave_append(n_risky, risky);
+}
+
+// Triangles with their coordinates rotated so that axis `axis` (0 = x,
+// 1 = y) becomes the third one: axis 0 maps (x, y, z) to (y, z, x), axis 1 to
+// (z, x, y).  Both maps are cyclic (rotations), so a triangle's facing along
+// the ray axis keeps its sign, and the column lists and the crossing test of
+// the z scan apply unchanged to rays along x or y.  Vertices and normal
+// alike.
+__global__ void rotate_tris_kernel(const float* __restrict__ tris, int64_t nt, int axis, float* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const float4* src = reinterpret_cast<const float4*>(tris + 12 * t);
+    const float4 a = src[0], b = src[1], c = src[2];
+    const float v[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+    float r[12];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float X = v[3 * k], Y = v[3 * k + 1], Z = v[3 * k + 2];
+        r[3 * k] = axis == 0 ? Y : Z;
+        r[3 * k + 1] = axis == 0 ? Z : X;
+        r[3 * k + 2] = axis == 0 ? X : Y;
+    }
+    float4* dst = reinterpret_cast<float4*>(out + 12 * t);
+    dst[0] = make_float4(r[0], r[1], r[2], r[3]);
+    dst[1] = make_float4(r[4], r[5], r[6], r[7]);
+    dst[2] = make_float4(r[8], r[9], r[10], r[11]);
+}
+
+// Features 5 (axis 0, a +x ray) and 7 (axis 1, a +y ray): the signed
+// crossing count of the jittered ray through the candidate's voxel along
+// that axis, with the z scan's crossing test and rule (a crossing counts as
+// below the voxel when its height is < the voxel centre's), over the column
+// list of the rotated triangles.  1 when the crossings below the voxel do not
+// cancel (inside by that ray), 0 when they do, 0.5 when the crossings of the
+// whole line do not cancel (an open line gives no answer).  With the z scan
+// sign (feature 1) these are the three axis parity counts.
+__global__ void axis_parity_kernel(const float* __restrict__ rtris, const float4* __restrict__ xyab,
+                                   const int64_t* __restrict__ start, const int32_t* __restrict__ ids,
+                                   const int64_t* __restrict__ list, int64_t n, int dx, int dy, int axis, int mnu,
+                                   int mnv, int mnw, float h, int sup, int sdu, float jx, float jy,
+                                   float* __restrict__ feats, int slot) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t v = list[i];
+    const int x = (int)(v % dx);
+    const int64_t r = v / dx;
+    const int y = (int)(r % dy);
+    const int z = (int)(r / dy);
+    const int iu = axis == 0 ? y : z, iv = axis == 0 ? z : x, iw = axis == 0 ? x : y;
+    const float ru = ((float)(mnu + iu) + jx) * h;
+    const float rv = ((float)(mnv + iv) + jy) * h;
+    const float pw = lattice(mnw, iw, h);
+    const int64_t s = (int64_t)(iv / sup) * sdu + (iu / sup);
+    const int64_t e0 = start[s], e1 = start[s + 1];
+    int total = 0, below = 0;
+    for (int64_t eb = e0; eb < e1; eb += kFeatBatch) {
+        // every index is clamped into [e0, e1), so the loads need no guard
+        int tid[kFeatBatch];
+#pragma unroll
+        for (int j = 0; j < kFeatBatch; ++j) tid[j] = ids[min(eb + j, e1 - 1)];
+        bool in_box[kFeatBatch];
+#pragma unroll
+        for (int j = 0; j < kFeatBatch; ++j) {
+            const float4 ab = xyab[tid[j]];
+            in_box[j] = (eb + j < e1) && !(ru < ab.x || ru > ab.z || rv < ab.y || rv > ab.w);
+        }
+#pragma unroll
+        for (int j = 0; j < kFeatBatch; ++j) {
+            if (!in_box[j]) continue;
+            const Tri9 t = load_tri(rtris, tid[j]);
+            const float d0 = (t.bx - t.ax) * (rv - t.ay) - (t.by - t.ay) * (ru - t.ax);
+            const float d1 = (t.cx - t.bx) * (rv - t.by) - (t.cy - t.by) * (ru - t.bx);
+            const float d2 = (t.ax - t.cx) * (rv - t.cy) - (t.ay - t.cy) * (ru - t.cx);
+            const bool pos = (d0 > 0.0f && d1 > 0.0f && d2 > 0.0f);
+            const bool neg = (d0 < 0.0f && d1 < 0.0f && d2 < 0.0f);
+            if (!pos && !neg) continue;
+            const float area2 = d0 + d1 + d2;
+            if (fabsf(area2) < 1e-12f) continue;
+            const float wh = (d1 * t.az + d2 * t.bz + d0 * t.cz) / area2;
+            const int sgn = pos ? 1 : -1;
+            total += sgn;
+            if (wh < pw) below += sgn;
+        }
+    }
+    feats[12 * i + slot] = (total != 0) ? 0.5f : ((below != 0) ? 1.0f : 0.0f);
+}
+
+__global__ void fill_feature_kernel(float* __restrict__ feats, int64_t n, int slot, float value) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) feats[12 * i + slot] = value;
 }
 
 // Risky candidates the ranked probes did not cover (the certificate's
@@ -497,6 +584,30 @@ hipError_t launch_unprobed_risky(const int64_t* list, const uint8_t* risk, int64
                                  int64_t* out, int64_t cap, unsigned long long* counter, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     unprobed_risky_kernel<<<grid_for(n), kBlock, 0, s>>>(list, risk, n, bits, out, cap, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_rotate_tris(const float* tris, int64_t nt, int axis, float* out, hipStream_t s) {
+    if (nt <= 0) return hipSuccess;
+    rotate_tris_kernel<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(tris, nt, axis, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_axis_parity(const float* rtris, const CsrBuffers& b, const int64_t* list, int64_t n,
+                              const Grid& g, int axis, int sup, float jx, float jy, float* feats, int slot,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const Grid r = rotated_grid(g, axis);
+    const int sdu = (r.dims[0] + sup - 1) / sup;
+    axis_parity_kernel<<<(unsigned)((n + 127) / 128), 128, 0, s>>>(rtris, b.xyab, b.start, b.ids, list, n,
+                                                                   g.dims[0], g.dims[1], axis, r.mn[0], r.mn[1],
+                                                                   r.mn[2], g.h, sup, sdu, jx, jy, feats, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_feature(float* feats, int64_t n, int slot, float value, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    fill_feature_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(feats, n, slot, value);
     return hipGetLastError();
 }
 

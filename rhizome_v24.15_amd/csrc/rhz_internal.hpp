@@ -15,6 +15,13 @@ struct Grid {
     int64_t n() const { return (int64_t)dims[0] * dims[1] * dims[2]; }
 };
 
+// The lattice seen with axis `axis` (0 = x, 1 = y) rotated to the third place:
+// (x, y, z) -> (y, z, x) for axis 0, (z, x, y) for axis 1.
+inline Grid rotated_grid(const Grid& g, int axis) {
+    const int a = axis == 0 ? 1 : 2, b = axis == 0 ? 2 : 0, c = axis == 0 ? 0 : 1;
+    return Grid{{g.mn[a], g.mn[b], g.mn[c]}, {g.dims[a], g.dims[b], g.dims[c]}, g.h};
+}
+
 // Records msg as the calling thread's rhz_last_error() and returns code.
 int set_error_message(int code, const char* msg);
 
@@ -275,6 +282,16 @@ hipError_t launch_features(const float* tris, int64_t nt, const CsrBuffers& b, c
 hipError_t launch_unprobed_risky(const int64_t* list, const uint8_t* risk, int64_t n, const unsigned* bits,
                                  int64_t* out, int64_t cap, unsigned long long* counter, hipStream_t s);
 hipError_t launch_margin_score(const float* feats, int64_t n, float* score, hipStream_t s);
+// Triangles with axis `axis` (0 = x, 1 = y) rotated to the third coordinate
+// (as rotated_grid), for column lists along that axis.
+hipError_t launch_rotate_tris(const float* tris, int64_t nt, int axis, float* out, hipStream_t s);
+// Features `slot` of the n candidates: inside by the signed crossings of a ray
+// along `axis` (1), outside (0), open line (0.5); `b` holds the column lists of
+// the rotated triangles `rtris` over rotated_grid(g, axis).
+hipError_t launch_axis_parity(const float* rtris, const CsrBuffers& b, const int64_t* list, int64_t n,
+                              const Grid& g, int axis, int sup, float jx, float jy, float* feats, int slot,
+                              hipStream_t s);
+hipError_t launch_fill_feature(float* feats, int64_t n, int slot, float value, hipStream_t s);
 // A uniform position in [0, 0.5) hashed from each candidate's voxel index, plus
 // the certificate flag (feature 10) when `feats` is not null.
 hipError_t launch_hash_score(const float* feats, const int64_t* cand, int64_t n, float* score, hipStream_t s);

@@ -141,6 +141,14 @@ enum Slot {
     S_ZKEY_TMP,
     S_ZSORT,
     S_ZEXT,
+    S_AX_TRIS,     // triangles rotated for x / y rays, and their column lists
+    S_AX_XYAB,
+    S_AX_COVER,
+    S_AX_COUNTS,
+    S_AX_START,
+    S_AX_CURSOR,
+    S_AX_IDS,
+    S_AX_SCAN,
     S_COUNT
 };
 
@@ -290,7 +298,7 @@ void* ws_get(Workspace* w, int slot, size_t bytes) {
 // counters: ull slots 0..15, int slots at byte offset 128 (32 ints)
 enum CounterId {
     C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4, C_SEL = 5, C_GROW = 6, C_RISKY = 7, C_NCL = 8,
-    C_UNDEC = 9, C_MISM2 = 10, C_INSIDE = 11, C_UNDEC2 = 12
+    C_UNDEC = 9, C_MISM2 = 10, C_INSIDE = 11, C_UNDEC2 = 12, C_AXCSR = 13
 };
 enum FlagId { F_UNKNOWN = 0, F_CHANGED = 1, F_ROUND0 = 2 };  // F_ROUND0 + r: round r changed a voxel
 constexpr int kMaxBatchedRounds = 24;
@@ -1149,6 +1157,48 @@ int learned_candidate_list(Ctx& c, int64_t** cand, int64_t* n_cand) {
 }
 
 // Features of the candidates (and their certificate flags).
+// Features 5 and 7 (the x and y axis parity counts): the triangles rotated
+// so that x, then y, is the ray axis, their column lists built as for the z
+// scan, and one crossing count per candidate.  A line set whose lists would
+// exceed csr_max_entries gives every candidate 0.5 (no answer) instead.
+int axis_parity_features(Ctx& c, const int64_t* list, int64_t n, float* f) {
+    const rhz_sdf_params* p = c.p;
+    const int sup = p->sup;
+    const int slots[2] = {5, 7};
+    if (n <= 0) return RHZ_OK;
+    float* rt = (float*)ws_get(c.w, S_AX_TRIS, sizeof(float) * 12 * (size_t)std::max<int64_t>(c.nt, 1));
+    if (!rt) return set_err(RHZ_ERR_NOMEM, "rotated triangle allocation failed");
+    for (int axis = 0; axis < 2; ++axis) {
+        const Grid r = rotated_grid(c.g, axis);
+        const int64_t nsup = num_supercells(r, sup);
+        CsrBuffers b;
+        b.xyab = (float4*)ws_get(c.w, S_AX_XYAB, sizeof(float4) * (size_t)std::max<int64_t>(c.nt, 1));
+        b.cover = (int4*)ws_get(c.w, S_AX_COVER, sizeof(int4) * (size_t)std::max<int64_t>(c.nt, 1));
+        b.counts = (uint32_t*)ws_get(c.w, S_AX_COUNTS, sizeof(uint32_t) * (size_t)nsup);
+        b.start = (int64_t*)ws_get(c.w, S_AX_START, sizeof(int64_t) * (size_t)(nsup + 1));
+        b.cursor = (uint32_t*)ws_get(c.w, S_AX_CURSOR, sizeof(uint32_t) * (size_t)nsup);
+        void* scan = ws_get(c.w, S_AX_SCAN, scan_scratch_bytes(nsup));
+        if (!b.xyab || !b.cover || !b.counts || !b.start || !b.cursor || !scan)
+            return set_err(RHZ_ERR_NOMEM, "axis column-list allocation failed");
+        HIP_TRY(launch_rotate_tris(c.tris, c.nt, axis, rt, c.s));
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_AXCSR), 0, sizeof(unsigned long long), c.s));
+        HIP_TRY(launch_csr_count(rt, c.nt, r, sup, b, cnt_ptr(c.w, C_AXCSR), c.s));
+        const int rc = read_counters(c);
+        if (rc) return rc;
+        const int64_t entries = (int64_t)hc(c, C_AXCSR);
+        if (entries <= 0 || entries > p->csr_max_entries) {
+            HIP_TRY(launch_fill_feature(f, n, slots[axis], entries <= 0 ? 0.0f : 0.5f, c.s));
+            continue;
+        }
+        b.ids = (int32_t*)ws_get(c.w, S_AX_IDS, sizeof(int32_t) * (size_t)entries);
+        if (!b.ids) return set_err(RHZ_ERR_NOMEM, "axis column-list allocation failed");
+        HIP_TRY(launch_scan_u32(b.counts, nsup, b.start, scan, c.s));
+        HIP_TRY(launch_csr_fill(c.nt, r, sup, b, c.s));
+        HIP_TRY(launch_axis_parity(rt, b, list, n, c.g, axis, sup, p->jitter_x, p->jitter_y, f, slots[axis], c.s));
+    }
+    return RHZ_OK;
+}
+
 int learned_features(Ctx& c, uint8_t* sign, const CsrBuffers& b, const int64_t* list, int64_t n, float** feats,
                      uint8_t** risk, int64_t* n_risky) {
     float* f = (float*)ws_get(c.w, S_FEAT, sizeof(float) * 12 * (size_t)std::max<int64_t>(n, 1));
@@ -1169,7 +1219,9 @@ int learned_features(Ctx& c, uint8_t* sign, const CsrBuffers& b, const int64_t* 
     HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_RISKY), 0, sizeof(unsigned long long), c.s));
     HIP_TRY(launch_features(c.tris, c.nt, b, zr, keys, max_ext, sign, list, n, c.g, c.p->sup, c.p->jitter_x,
                             c.p->jitter_y, f, r, cnt_ptr(c.w, C_RISKY), c.s));
-    const int rc = read_counters(c);
+    int rc = axis_parity_features(c, list, n, f);
+    if (rc) return rc;
+    rc = read_counters(c);
     if (rc) return rc;
     *feats = f;
     *risk = r;

@@ -89,12 +89,14 @@ def test_rankings_at_a_fixed_budget(rhz, geom, torch_gpu, request, case):
         assert all(res[n][2] == 0 for n in RANKINGS)
         assert all(res[n][0] <= n_err + 1 for n in RANKINGS)
     for frac, res in table.items():
-        # the network over all features ranks at least as well as the flag
-        # alone and as the network without the flag, at every budget
-        assert res["network"][0] <= min(res["flag"][0], res["noflag"][0]), frac
-        assert res["flag"][0] <= res["blind"][0], frac
+        # both networks rank better than the flag alone wherever the flag
+        # leaves an error; the network that sees the flag is not worse than the
+        # one that does not (to 0.1% of the errors); the flag beats blind probes
+        assert res["network"][0] <= res["flag"][0] and res["noflag"][0] <= res["flag"][0], frac
         if res["flag"][0] > 0:
-            assert res["network"][0] < res["flag"][0], frac
+            assert res["network"][0] < res["flag"][0] and res["noflag"][0] < res["flag"][0], frac
+        assert res["network"][0] <= res["noflag"][0] + max(2, n_err // 1000), frac
+        assert res["flag"][0] <= res["blind"][0], frac
     # every ranking is a function of the voxels alone, not of the run-dependent
     # order of the candidate list: a second run selects the same probes
     again = _residuals(rhz, tris, h, mn, mx, ref, on_surface, learned_certify=0, reprobe_fraction=BUDGETS[1])
