@@ -222,7 +222,7 @@ def mug_specimen(rhz, torch, params, steps, warmup):
 def batched_bake(rhz, torch, params, n_meshes=64, log=None, fixed=None):
     """The batched bake: 64 seeded vessels (2k-500k triangles, log-uniform),
     longest lattice side 64-384 at 0.25-0.5 mm, built by rhz_sdf_bake with the
-    lane count and sweep shape the autotuner picks on a 16-mesh sample (or
+    lane count and sweep shape the autotuner picks on the whole batch (or
     the configuration `fixed`, without tuning)."""
     import importlib
 
@@ -241,7 +241,8 @@ def batched_bake(rhz, torch, params, n_meshes=64, log=None, fixed=None):
         items.append((torch.from_numpy(tris).cuda(), len(tris), h, mn, mx,
                       torch.empty(V, dtype=torch.float32, device="cuda")))
     if fixed is None:
-        best, table = tune.autotune_bake(rhz, torch, items, params, log=log)
+        # the whole batch: a 16-item sample picked lane counts that the full batch did not favour
+        best, table = tune.autotune_bake(rhz, torch, items, params, sample=len(items), log=log)
     else:
         tune.apply_config(rhz, fixed)
         best, table = dict(fixed), []
@@ -354,6 +355,9 @@ def main():
                     help="learned method: rank with the proposer network, the planar-margin rule, "
                          "nothing (exact probes of the certificate-flagged voxels only), or the "
                          "certificate flag alone as the ranking")
+    ap.add_argument("--sweep-table", default="auto",
+                    help="per-class sweep tile table: 'auto' (the cached table of this device, tuned and "
+                         "cached first when absent), 'off' (class-free shapes), or a JSON path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the accuracy report, the sign check and the rounds curve")
@@ -400,6 +404,18 @@ def main():
         print(json.dumps(rec))
         return
     fsm = rhz.FSM_REFERENCE if args.fsm == "reference" else rhz.FSM_CONVERGE
+    sweep_table = None
+    if args.sweep_table != "off":
+        import importlib
+
+        tune = importlib.import_module(entry.PKG_NAME + ".autotune")
+        t_tab = time.perf_counter()
+        path = None if args.sweep_table == "auto" else args.sweep_table
+        table, cached = tune.ensure_sweep_table(rhz, torch, entry.load_geometry(), path=path, log=log)
+        sweep_table = {"cached": cached, "seconds": time.perf_counter() - t_tab, "device": table["device"],
+                       "classes": table["classes"],
+                       "path": path or tune.default_table_path()}
+        log(f"sweep table ({'cached' if cached else 'tuned'}): {table['classes']}")
 
     t0 = time.perf_counter()
     v, f, mn, mx = make_workload(args.h)
@@ -518,6 +534,8 @@ def main():
             saved = {k: rhz.tuning_get(k) for k in ("sweep_shape_x", "sweep_shape_y", "sweep_shape_z", "sweep_step",
                                                     "sweep_rows", "fsm_batch_rounds_max_voxels", "fsm_graph",
                                                     "bake_batched")}
+            saved.update({f"{k}@{c}": rhz.tuning_get(f"{k}@{c}") for c in range(4)
+                          for k in ("batch_shape_x", "batch_shape_y", "batch_shape_z", "batch_step")})
             batch = batched_bake(rhz, torch, params, log=log)
             for k, v in saved.items():
                 rhz.tuning_set(k, v)
@@ -554,6 +572,7 @@ def main():
                    "eikonal_rounds": st.eikonal_rounds, "sweeps_per_step": st.sweeps,
                    "patched_voxels": st.patched_voxels, "reprobed_voxels": st.reprobed_voxels},
         "roofline": dominant,
+        "sweep_table": sweep_table,
         "roofline_fsm": rl_fsm,
         "roofline_fsm_round": rl_round,
         "roofline_winding": rl_wind,

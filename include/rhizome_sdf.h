@@ -201,6 +201,18 @@ int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
  *                                  ms_eikonal, and ms_total is their union. */
 int rhz_tuning_set(const char* key, int64_t value);
 int rhz_tuning_get(const char* key, int64_t* value);
+/* Per lattice-size class: "sweep_shape_x@C", "sweep_shape_y@C",
+ * "sweep_shape_z@C" and "sweep_step@C" (C = 0..3) set the tile shape and the
+ * steps per load group of the single builds' sweeps over lattices of class
+ * C; "batch_shape_x@C", "batch_shape_y@C", "batch_shape_z@C" and
+ * "batch_step@C" the same for the batched bake's sweep launches (one launch
+ * per class present).  0 (the default) uses the class-free key.  Every
+ * setting gives identical results; only the time differs.  No
+ * counterpart in the reference (whose launch shapes are fixed): it serves
+ * the autotuner's cached per-shape table. */
+/* The size class of a lattice: 0 up to 2^21 voxels, 1 up to 2^24, 2 up to
+ * 2^27, 3 above. */
+int rhz_sweep_shape_class(const int32_t dims[3]);
 /* Frees the cached device workspaces of the current device: the one the
  * single-lattice calls share and the batched bake lanes' own. */
 void rhz_release_workspace(void);

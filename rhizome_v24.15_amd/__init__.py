@@ -236,6 +236,7 @@ def lib():
     L.rhz_sdf_accuracy.argtypes = [vp, i64, f32, vp, vp, vp, i64, i32, C.c_uint64, vp, C.POINTER(Accuracy)]
     L.rhz_tuning_set.argtypes = [C.c_char_p, i64]
     L.rhz_tuning_get.argtypes = [C.c_char_p, C.POINTER(C.c_int64)]
+    L.rhz_sweep_shape_class.argtypes = [C.POINTER(C.c_int32)]
     L.rhz_mlp_param_count.restype = C.c_int
     L.rhz_mlp_blob_count.restype = C.c_int
     L.rhz_mlp_builtin.argtypes = [vp]
@@ -291,6 +292,12 @@ def tuning_get(key: str) -> int:
     v = C.c_int64(0)
     check(lib().rhz_tuning_get(key.encode(), C.byref(v)), "rhz_tuning_get")
     return v.value
+
+
+def sweep_shape_class(dims) -> int:
+    """Lattice-size class of the per-class sweep tuning keys ("sweep_shape_x@C",
+    ...): 0 up to 2^21 voxels, 1 up to 2^24, 2 up to 2^27, 3 above."""
+    return check(lib().rhz_sweep_shape_class(_i32x3(dims)), "rhz_sweep_shape_class")
 
 
 def device_count() -> int:

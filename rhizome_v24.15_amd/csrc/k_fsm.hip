@@ -554,25 +554,11 @@ __global__ __launch_bounds__(256) void sweep_tile_batch_kernel(const SweepJob* _
     if (__ballot(chg) != 0 && threadIdx.x == 0) atomicOr(J.flags + r, 1);
 }
 
-template <int SF, int SW, int SC, int DIR, int STEP, int ROWS, bool FMAJ>
-__global__ __launch_bounds__(64 * ROWS) void sweep_lane_kernel(const float* __restrict__ src,
-                                                               float* __restrict__ dst, int nf, int nw, int nc,
-                                                               int64_t ws, int64_t cs, int nbf, unsigned nblocks,
-                                                               float h, int* __restrict__ changed) {
-    // FMAJ: the ROWS waves of a block take adjacent 64-cell segments of ONE
-    // line row (long contiguous accesses); otherwise ROWS adjacent rows.
-    const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
-    int bf, c;
-    if (FMAJ) {
-        const int nbg = (nbf + ROWS - 1) / ROWS;
-        bf = __builtin_amdgcn_readfirstlane((int)(lb % (unsigned)nbg) * ROWS + (int)threadIdx.y);
-        c = (int)(lb / (unsigned)nbg);
-        if (bf >= nbf) return;
-    } else {
-        bf = (int)(lb % (unsigned)nbf);
-        c = __builtin_amdgcn_readfirstlane((int)(lb / (unsigned)nbf) * ROWS + (int)threadIdx.y);
-    }
-    if (c >= nc) return;
+// One wave walks line row c, cells bf * 64 + lane of the fast axis (the
+// one-line walker).  Returns whether this lane lowered a cell.
+template <int SF, int SW, int SC, int DIR, int STEP>
+__device__ __forceinline__ bool lane_walk(const float* __restrict__ src, float* __restrict__ dst, int nf, int nw,
+                                          int nc, int64_t ws, int64_t cs, float h, int bf, int c) {
     const int lane = threadIdx.x;
     const int f = bf * 64 + lane;
     const bool live = f < nf;
@@ -632,7 +618,50 @@ __global__ __launch_bounds__(64 * ROWS) void sweep_lane_kernel(const float* __re
         cur = next;
         row += step;
     }
-    if (__ballot(chg) != 0 && lane == 0) atomicOr(changed, 1);
+    return chg;
+}
+
+template <int SF, int SW, int SC, int DIR, int STEP, int ROWS, bool FMAJ>
+__global__ __launch_bounds__(64 * ROWS) void sweep_lane_kernel(const float* __restrict__ src,
+                                                               float* __restrict__ dst, int nf, int nw, int nc,
+                                                               int64_t ws, int64_t cs, int nbf, unsigned nblocks,
+                                                               float h, int* __restrict__ changed) {
+    // FMAJ: the ROWS waves of a block take adjacent 64-cell segments of ONE
+    // line row (long contiguous accesses); otherwise ROWS adjacent rows.
+    const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
+    int bf, c;
+    if (FMAJ) {
+        const int nbg = (nbf + ROWS - 1) / ROWS;
+        bf = __builtin_amdgcn_readfirstlane((int)(lb % (unsigned)nbg) * ROWS + (int)threadIdx.y);
+        c = (int)(lb / (unsigned)nbg);
+        if (bf >= nbf) return;
+    } else {
+        bf = (int)(lb % (unsigned)nbf);
+        c = __builtin_amdgcn_readfirstlane((int)(lb / (unsigned)nbf) * ROWS + (int)threadIdx.y);
+    }
+    if (c >= nc) return;
+    const bool chg = lane_walk<SF, SW, SC, DIR, STEP>(src, dst, nf, nw, nc, ws, cs, h, bf, c);
+    if (__ballot(chg) != 0 && threadIdx.x == 0) atomicOr(changed, 1);
+}
+
+// The one-line walker over many lattices in one launch (4 rows per block, a
+// job's blocks as sweep_lane_kernel's for it alone; J.nbf = 64-cell
+// segments per row), gated and flagged per round like
+// sweep_tile_batch_kernel.
+template <int SF, int SW, int SC, int DIR, int STEP>
+__global__ __launch_bounds__(256) void sweep_lane_batch_kernel(const SweepJob* __restrict__ jobs,
+                                                               const unsigned* __restrict__ starts, int njobs,
+                                                               unsigned nblocks, int r) {
+    const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
+    const int j = __builtin_amdgcn_readfirstlane(batch_job(starts, njobs, lb));
+    const SweepJob J = jobs[j];
+    if (r > 0 && J.flags[r - 1] == 0) return;
+    const unsigned local = lb - starts[j];
+    const int bf = (int)(local % (unsigned)J.nbf);
+    const int c = __builtin_amdgcn_readfirstlane((int)(local / (unsigned)J.nbf) * 4 + (int)threadIdx.y);
+    if (c >= J.nc) return;
+    const bool chg = lane_walk<SF, SW, SC, DIR, STEP>(J.src, J.dst, J.nf, J.nw, J.nc, J.ws, J.cs, J.h, bf, c);
+    if (__ballot(chg) != 0 && threadIdx.x == 0) atomicOr(J.flags + r, 1);
 }
 
 // ---- plane transpose (x-fastest <-> y-fastest) ----------------------------------
@@ -746,6 +775,44 @@ bool set_sweep_shape(int axis, int shape) {
     g_sweep_tuning.shape[axis] = shape;
     return true;
 }
+int shape_class(int64_t voxels) {
+    if (voxels <= (int64_t)1 << 21) return 0;
+    if (voxels <= (int64_t)1 << 24) return 1;
+    if (voxels <= (int64_t)1 << 27) return 2;
+    return 3;
+}
+int effective_shape(int axis, int64_t voxels) {
+    const int v = g_sweep_tuning.cls_shape[shape_class(voxels)][axis];
+    return v ? v : g_sweep_tuning.shape[axis];
+}
+int effective_step(int64_t voxels) {
+    const int v = g_sweep_tuning.cls_step[shape_class(voxels)];
+    return v ? v : g_sweep_tuning.step;
+}
+bool set_class_shape(int cls, int axis, int shape) {
+    if (cls < 0 || cls >= kShapeClasses || axis < 0 || axis > 2 ||
+        (shape != 0 && shape != 1 && shape != 21 && shape != 22 && shape != 12 && shape != 14))
+        return false;
+    g_sweep_tuning.cls_shape[cls][axis] = shape;
+    return true;
+}
+bool set_class_step(int cls, int step) {
+    if (cls < 0 || cls >= kShapeClasses || (step != 0 && step != 2 && step != 4 && step != 8)) return false;
+    g_sweep_tuning.cls_step[cls] = step;
+    return true;
+}
+bool set_batch_class_shape(int cls, int axis, int shape) {
+    if (cls < 0 || cls >= kShapeClasses || axis < 0 || axis > 2 ||
+        (shape != 0 && shape != 1 && shape != 21 && shape != 22 && shape != 12 && shape != 14))
+        return false;
+    g_sweep_tuning.bcls_shape[cls][axis] = shape;
+    return true;
+}
+bool set_batch_class_step(int cls, int step) {
+    if (cls < 0 || cls >= kShapeClasses || (step != 0 && step != 2 && step != 4 && step != 8)) return false;
+    g_sweep_tuning.bcls_step[cls] = step;
+    return true;
+}
 void set_fsm_batching(int64_t max_voxels, int graph) {
     g_sweep_tuning.batch_rounds_max_voxels = max_voxels;
     g_sweep_tuning.fsm_graph = graph ? 1 : 0;
@@ -785,12 +852,15 @@ template <int SF, int SW, int SC, int DIR>
 hipError_t launch_lane_d(const float* src, float* dst, int nf, int nw, int nc, int64_t ws, int64_t cs, float h,
                          int* changed, hipStream_t s) {
     const SweepTuning t = sweep_tuning();
-    // the walk axis names the orientation (the x-sweep runs on the y-fastest copy)
-    const int shape = t.shape[SW];
+    // the walk axis names the orientation (the x-sweep runs on the y-fastest
+    // copy); shape and step are those of the lattice's size class
+    const int64_t voxels = (int64_t)nf * nw * nc;
+    const int shape = effective_shape(SW, voxels);
+    const int step = effective_step(voxels);
 #define RHZ_TILE(NF, NC)                                                                                    \
     if (shape == 10 * NF + NC) {                                                                             \
-        if (t.step == 2) return launch_tile<SF, SW, SC, DIR, 2, NF, NC>(src, dst, nf, nw, nc, ws, cs, h, changed, s); \
-        if (t.step == 8) return launch_tile<SF, SW, SC, DIR, 8, NF, NC>(src, dst, nf, nw, nc, ws, cs, h, changed, s); \
+        if (step == 2) return launch_tile<SF, SW, SC, DIR, 2, NF, NC>(src, dst, nf, nw, nc, ws, cs, h, changed, s); \
+        if (step == 8) return launch_tile<SF, SW, SC, DIR, 8, NF, NC>(src, dst, nf, nw, nc, ws, cs, h, changed, s); \
         return launch_tile<SF, SW, SC, DIR, 4, NF, NC>(src, dst, nf, nw, nc, ws, cs, h, changed, s);         \
     }
     RHZ_TILE(2, 1)
@@ -798,12 +868,12 @@ hipError_t launch_lane_d(const float* src, float* dst, int nf, int nw, int nc, i
     RHZ_TILE(1, 2)
     RHZ_TILE(1, 4)
 #undef RHZ_TILE
-    if (t.step == 8) {
+    if (step == 8) {
         if (t.rows == 4) return launch_lane_v<SF, SW, SC, DIR, 8, 4>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
         if (t.rows == 16) return launch_lane_v<SF, SW, SC, DIR, 8, 16>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
         return launch_lane_v<SF, SW, SC, DIR, 8, 8>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
     }
-    if (t.step == 2) {
+    if (step == 2) {
         if (t.rows == 4) return launch_lane_v<SF, SW, SC, DIR, 2, 4>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
         if (t.rows == 16) return launch_lane_v<SF, SW, SC, DIR, 2, 16>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
         return launch_lane_v<SF, SW, SC, DIR, 2, 8>(src, dst, nf, nw, nc, ws, cs, h, changed, s);
@@ -915,22 +985,28 @@ void fsm_layouts(const Grid& g, FieldLayout* xy, FieldLayout* yx) {
     *yx = ok ? FieldLayout{py, py * dx} : dense_yx(g);
 }
 
-// Tile shape (NF x NC) of the batched sweeps per orientation: the tuning's
-// tile shape for that walk axis, or, for the one-line walker, the default
-// tiles (1 x 4 for x on the y-fastest copy, 2 x 1 for y and z).
-static void batch_shape(int orient, int* nf_tile, int* nc_tile) {
+// Tile shape (NF x NC) of the batched sweeps per orientation and size class:
+// the batched stage's own per-class shape for that walk axis, else the
+// class-free one; NC = 0 for the one-line walker (sweep_lane_batch_kernel,
+// 4 rows per block).
+static void batch_shape(int orient, int cls, int* nf_tile, int* nc_tile) {
     const int walk = orient == 0 ? 0 : orient;  // walk axis: x, y, z
-    int shape = sweep_tuning().shape[walk];
-    if (shape != 21 && shape != 22 && shape != 12 && shape != 14) shape = orient == 0 ? 14 : 21;
+    const SweepTuning t = sweep_tuning();
+    const int shape = t.bcls_shape[cls][walk] ? t.bcls_shape[cls][walk] : t.shape[walk];
+    if (shape == 1) {  // one-line walker: one 64-cell segment, one row per wave
+        *nf_tile = 1;
+        *nc_tile = 0;
+        return;
+    }
     *nf_tile = shape / 10;
     *nc_tile = shape % 10;
 }
 
-int64_t sweep_batch_blocks(int orient, int nf, int nc, int* nbf) {
+int64_t sweep_batch_blocks(int orient, int cls, int nf, int nc, int* nbf) {
     int NF = 2, NC = 1;
-    batch_shape(orient, &NF, &NC);
+    batch_shape(orient, cls, &NF, &NC);
     *nbf = (nf + 64 * NF - 1) / (64 * NF);
-    const int ngroups = (nc + NC - 1) / NC;
+    const int ngroups = NC ? (nc + NC - 1) / NC : nc;
     return (int64_t)*nbf * ((ngroups + 3) / 4);
 }
 
@@ -938,6 +1014,10 @@ namespace {
 template <int SF, int SW, int SC, int DIR, int STEP>
 hipError_t launch_sweep_batch_s(int NF, int NC, const SweepJob* jobs, const unsigned* starts, int njobs,
                                 unsigned nblocks, int r, hipStream_t s) {
+    if (NC == 0) {
+        sweep_lane_batch_kernel<SF, SW, SC, DIR, STEP><<<nblocks, dim3(64, 4), 0, s>>>(jobs, starts, njobs, nblocks, r);
+        return hipGetLastError();
+    }
 #define RHZ_BTILE(A, B)                                                                                  \
     if (NF == A && NC == B) {                                                                            \
         sweep_tile_batch_kernel<SF, SW, SC, DIR, STEP, A, B><<<nblocks, dim3(64, 4), 0, s>>>(jobs, starts, njobs, \
@@ -953,29 +1033,30 @@ hipError_t launch_sweep_batch_s(int NF, int NC, const SweepJob* jobs, const unsi
 }
 
 template <int SF, int SW, int SC, int DIR>
-hipError_t launch_sweep_batch_d(int NF, int NC, const SweepJob* jobs, const unsigned* starts, int njobs,
+hipError_t launch_sweep_batch_d(int NF, int NC, int step, const SweepJob* jobs, const unsigned* starts, int njobs,
                                 unsigned nblocks, int r, hipStream_t s) {
-    const int step = sweep_tuning().step;
     if (step == 4) return launch_sweep_batch_s<SF, SW, SC, DIR, 4>(NF, NC, jobs, starts, njobs, nblocks, r, s);
     if (step == 8) return launch_sweep_batch_s<SF, SW, SC, DIR, 8>(NF, NC, jobs, starts, njobs, nblocks, r, s);
     return launch_sweep_batch_s<SF, SW, SC, DIR, 2>(NF, NC, jobs, starts, njobs, nblocks, r, s);
 }
 }  // namespace
 
-hipError_t launch_sweep_batch(int orient, int dir, const SweepJob* jobs, const unsigned* starts, int njobs,
+hipError_t launch_sweep_batch(int orient, int cls, int dir, const SweepJob* jobs, const unsigned* starts, int njobs,
                               unsigned nblocks, int r, hipStream_t s) {
     if (njobs <= 0 || nblocks == 0) return hipSuccess;
     int NF = 2, NC = 1;
-    batch_shape(orient, &NF, &NC);
+    batch_shape(orient, cls, &NF, &NC);
+    const SweepTuning t = sweep_tuning();
+    const int st = t.bcls_step[cls] ? t.bcls_step[cls] : t.step;
     if (orient == 0)  // y-fastest copy: lanes y (slot 1), walk x (slot 0), cross z (slot 2)
-        return dir > 0 ? launch_sweep_batch_d<1, 0, 2, 1>(NF, NC, jobs, starts, njobs, nblocks, r, s)
-                       : launch_sweep_batch_d<1, 0, 2, -1>(NF, NC, jobs, starts, njobs, nblocks, r, s);
+        return dir > 0 ? launch_sweep_batch_d<1, 0, 2, 1>(NF, NC, st, jobs, starts, njobs, nblocks, r, s)
+                       : launch_sweep_batch_d<1, 0, 2, -1>(NF, NC, st, jobs, starts, njobs, nblocks, r, s);
     if (orient == 1)  // lanes x, walk y, cross z
-        return dir > 0 ? launch_sweep_batch_d<0, 1, 2, 1>(NF, NC, jobs, starts, njobs, nblocks, r, s)
-                       : launch_sweep_batch_d<0, 1, 2, -1>(NF, NC, jobs, starts, njobs, nblocks, r, s);
+        return dir > 0 ? launch_sweep_batch_d<0, 1, 2, 1>(NF, NC, st, jobs, starts, njobs, nblocks, r, s)
+                       : launch_sweep_batch_d<0, 1, 2, -1>(NF, NC, st, jobs, starts, njobs, nblocks, r, s);
     // lanes x, walk z, cross y
-    return dir > 0 ? launch_sweep_batch_d<0, 2, 1, 1>(NF, NC, jobs, starts, njobs, nblocks, r, s)
-                   : launch_sweep_batch_d<0, 2, 1, -1>(NF, NC, jobs, starts, njobs, nblocks, r, s);
+    return dir > 0 ? launch_sweep_batch_d<0, 2, 1, 1>(NF, NC, st, jobs, starts, njobs, nblocks, r, s)
+                   : launch_sweep_batch_d<0, 2, 1, -1>(NF, NC, st, jobs, starts, njobs, nblocks, r, s);
 }
 
 int64_t transpose_batch_blocks(int rows, int cols, int planes, int* tiles_r, int* tiles_c) {

@@ -124,8 +124,8 @@ __device__ __forceinline__ bool segment_may_cross(float px, float py, float pz, 
 // scanline supercell list), and risk[i] = 1 when the segment certificate
 // above cannot rule out a scanline sign error:
 //  0 d/h  1 scan sign (the z ray)  2-4 nearest-triangle normal
-//  5 inside by the signed crossings of a +x ray (axis_parity_kernel)
-//  6 planar margin  7 inside by a +y ray (axis_parity_kernel)
+//  5 inside by the signed crossings of an x ray (axis_gather_kernel)
+//  6 planar margin  7 inside by a y ray (axis_gather_kernel)
 //  8 log1p(#triangles within 1 h)  9 normal spread within 1.5 h
 //  10 the certificate flag itself  11 distance from the voxel centre to the
 //  nearest crossing of its jittered column ray (voxels)
@@ -294,7 +294,7 @@ __global__ void features_kernel(const float* __restrict__ tris, const float4* __
     f[2] = nx;
     f[3] = ny;
     f[4] = nz;
-    f[6] = s_out * t_off - dist;  // f[5] and f[7]: axis_parity_kernel
+    f[6] = s_out * t_off - dist;  // f[5] and f[7]: axis_gather_kernel
     f[8] = log1pf((float)near_cnt);
     f[9] = spread;
     f[10] = risky ? 1.0f : 0.0f;
@@ -329,19 +329,78 @@ __global__ void rotate_tris_kernel(const float* __restrict__ tris, int64_t nt, i
     dst[2] = make_float4(r[8], r[9], r[10], r[11]);
 }
 
-// Features 5 (axis 0, a +x ray) and 7 (axis 1, a +y ray): the signed
-// crossing count of the jittered ray through the candidate's voxel along
-// that axis, with the z scan's crossing test and rule (a crossing counts as
-// below the voxel when its height is < the voxel centre's), over the column
-// list of the rotated triangles.  1 when the crossings below the voxel do not
-// cancel (inside by that ray), 0 when they do, 0.5 when the crossings of the
-// whole line do not cancel (an open line gives no answer).  With the z scan
-// sign (feature 1) these are the three axis parity counts.
-__global__ void axis_parity_kernel(const float* __restrict__ rtris, const float4* __restrict__ xyab,
-                                   const int64_t* __restrict__ start, const int32_t* __restrict__ ids,
-                                   const int64_t* __restrict__ list, int64_t n, int dx, int dy, int axis, int mnu,
-                                   int mnv, int mnw, float h, int sup, int sdu, float jx, float jy,
-                                   float* __restrict__ feats, int slot) {
+// Row lists for features 5 (axis 0, a ray along x) and 7 (axis 1, along y),
+// over the rotated triangles and lattice (rotated_grid): row (iu, iv) holds
+// exactly the triangles whose (u, v) box contains the row's jittered ray
+// point ((mnu + iu + jx) h, (mnv + iv + jy) h), the box reject of the
+// crossing test, with the same float predicate (the first and last index
+// from an estimate and exact steps; the point moves monotonically with the
+// index).  A triangle smaller than a voxel lands in about one row.
+__device__ __forceinline__ float ray_at(int mn, int i, float j, float h) { return ((float)(mn + i) + j) * h; }
+
+__device__ __forceinline__ int to_int_clamped(float v) { return (int)fminf(fmaxf(v, -2.0e9f), 2.0e9f); }
+
+__device__ __forceinline__ void ray_rows(float lo, float hi, int mn, int d, float j, float h, int* first, int* last) {
+    int i0 = to_int_clamped(ceilf(lo / h - j)) - mn;
+    i0 = min(max(i0, 0), d);
+    while (i0 > 0 && ray_at(mn, i0 - 1, j, h) >= lo) --i0;
+    while (i0 < d && ray_at(mn, i0, j, h) < lo) ++i0;
+    int i1 = to_int_clamped(floorf(hi / h - j)) - mn;
+    i1 = min(max(i1, -1), d - 1);
+    while (i1 < d - 1 && ray_at(mn, i1 + 1, j, h) <= hi) ++i1;
+    while (i1 >= 0 && ray_at(mn, i1, j, h) > hi) --i1;
+    *first = i0;
+    *last = i1;
+}
+
+__global__ void axis_rows_count_kernel(const float* __restrict__ rtris, int64_t nt, int mnu, int mnv, int du, int dv,
+                                       float h, float jx, float jy, int4* __restrict__ cover,
+                                       uint32_t* __restrict__ counts, unsigned long long* total) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long mine = 0;
+    if (t < nt) {
+        const Tri9 tr = load_tri(rtris, t);
+        const float u0 = fminf(tr.ax, fminf(tr.bx, tr.cx)), u1 = fmaxf(tr.ax, fmaxf(tr.bx, tr.cx));
+        const float v0 = fminf(tr.ay, fminf(tr.by, tr.cy)), v1 = fmaxf(tr.ay, fmaxf(tr.by, tr.cy));
+        int4 cv;
+        ray_rows(u0, u1, mnu, du, jx, h, &cv.x, &cv.y);
+        ray_rows(v0, v1, mnv, dv, jy, h, &cv.z, &cv.w);
+        cover[t] = cv;
+        if (cv.y >= cv.x && cv.w >= cv.z) {
+            for (int iv = cv.z; iv <= cv.w; ++iv)
+                for (int iu = cv.x; iu <= cv.y; ++iu) atomicAdd(counts + (int64_t)iv * du + iu, 1u);
+            mine = (unsigned long long)(cv.y - cv.x + 1) * (unsigned long long)(cv.w - cv.z + 1);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
+    if ((threadIdx.x & 63) == 0 && mine) atomicAdd(total, mine);
+}
+
+__global__ void axis_rows_fill_kernel(int64_t nt, int du, const int4* __restrict__ cover,
+                                      const int64_t* __restrict__ start, uint32_t* __restrict__ cursor,
+                                      int32_t* __restrict__ ids) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int4 cv = cover[t];
+    if (cv.y < cv.x || cv.w < cv.z) return;
+    for (int iv = cv.z; iv <= cv.w; ++iv)
+        for (int iu = cv.x; iu <= cv.y; ++iu) {
+            const int64_t r = (int64_t)iv * du + iu;
+            ids[start[r] + atomicAdd(cursor + r, 1u)] = (int32_t)t;
+        }
+}
+
+// Features 5 / 7: the signed crossing count of the candidate's row ray, with
+// the z scan's crossing test and rule (a crossing counts as below the voxel
+// when its height is < the voxel centre's).  1 when the crossings below the
+// voxel do not cancel (inside by that ray), 0 when they do, 0.5 when the
+// crossings of the whole line do not cancel (an open line gives no answer).
+// With the z scan sign (feature 1) these are the three axis parity counts.
+// The sum does not depend on the list order.
+__global__ void axis_parity_kernel(const float* __restrict__ rtris, const int64_t* __restrict__ start,
+                                   const int32_t* __restrict__ ids, const int64_t* __restrict__ list, int64_t n,
+                                   int dx, int dy, int axis, int mnu, int mnv, int mnw, int du, float h, float jx,
+                                   float jy, float* __restrict__ feats, int slot) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t v = list[i];
@@ -350,26 +409,19 @@ __global__ void axis_parity_kernel(const float* __restrict__ rtris, const float4
     const int y = (int)(r % dy);
     const int z = (int)(r / dy);
     const int iu = axis == 0 ? y : z, iv = axis == 0 ? z : x, iw = axis == 0 ? x : y;
-    const float ru = ((float)(mnu + iu) + jx) * h;
-    const float rv = ((float)(mnv + iv) + jy) * h;
+    const float ru = ray_at(mnu, iu, jx, h), rv = ray_at(mnv, iv, jy, h);
     const float pw = lattice(mnw, iw, h);
-    const int64_t s = (int64_t)(iv / sup) * sdu + (iu / sup);
-    const int64_t e0 = start[s], e1 = start[s + 1];
+    const int64_t row = (int64_t)iv * du + iu;
+    const int64_t e0 = start[row], e1 = start[row + 1];
     int total = 0, below = 0;
     for (int64_t eb = e0; eb < e1; eb += kFeatBatch) {
         // every index is clamped into [e0, e1), so the loads need no guard
         int tid[kFeatBatch];
 #pragma unroll
         for (int j = 0; j < kFeatBatch; ++j) tid[j] = ids[min(eb + j, e1 - 1)];
-        bool in_box[kFeatBatch];
 #pragma unroll
         for (int j = 0; j < kFeatBatch; ++j) {
-            const float4 ab = xyab[tid[j]];
-            in_box[j] = (eb + j < e1) && !(ru < ab.x || ru > ab.z || rv < ab.y || rv > ab.w);
-        }
-#pragma unroll
-        for (int j = 0; j < kFeatBatch; ++j) {
-            if (!in_box[j]) continue;
+            if (eb + j >= e1) continue;
             const Tri9 t = load_tri(rtris, tid[j]);
             const float d0 = (t.bx - t.ax) * (rv - t.ay) - (t.by - t.ay) * (ru - t.ax);
             const float d1 = (t.cx - t.bx) * (rv - t.by) - (t.cy - t.by) * (ru - t.bx);
@@ -593,15 +645,31 @@ hipError_t launch_rotate_tris(const float* tris, int64_t nt, int axis, float* ou
     return hipGetLastError();
 }
 
-hipError_t launch_axis_parity(const float* rtris, const CsrBuffers& b, const int64_t* list, int64_t n,
-                              const Grid& g, int axis, int sup, float jx, float jy, float* feats, int slot,
+hipError_t launch_axis_rows_count(const float* rtris, int64_t nt, const Grid& r, float jx, float jy, int4* cover,
+                                  uint32_t* counts, unsigned long long* total, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(counts, 0, sizeof(uint32_t) * (size_t)r.dims[0] * r.dims[1], s);
+    if (e != hipSuccess || nt <= 0) return e;
+    axis_rows_count_kernel<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(rtris, nt, r.mn[0], r.mn[1], r.dims[0],
+                                                                        r.dims[1], r.h, jx, jy, cover, counts, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_axis_rows_fill(int64_t nt, const Grid& r, const int4* cover, const int64_t* start, uint32_t* cursor,
+                                 int32_t* ids, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(cursor, 0, sizeof(uint32_t) * (size_t)r.dims[0] * r.dims[1], s);
+    if (e != hipSuccess || nt <= 0) return e;
+    axis_rows_fill_kernel<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(nt, r.dims[0], cover, start, cursor, ids);
+    return hipGetLastError();
+}
+
+hipError_t launch_axis_parity(const float* rtris, const int64_t* start, const int32_t* ids, const int64_t* list,
+                              int64_t n, const Grid& g, int axis, float jx, float jy, float* feats, int slot,
                               hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const Grid r = rotated_grid(g, axis);
-    const int sdu = (r.dims[0] + sup - 1) / sup;
-    axis_parity_kernel<<<(unsigned)((n + 127) / 128), 128, 0, s>>>(rtris, b.xyab, b.start, b.ids, list, n,
-                                                                   g.dims[0], g.dims[1], axis, r.mn[0], r.mn[1],
-                                                                   r.mn[2], g.h, sup, sdu, jx, jy, feats, slot);
+    axis_parity_kernel<<<(unsigned)((n + 127) / 128), 128, 0, s>>>(rtris, start, ids, list, n, g.dims[0], g.dims[1],
+                                                                   axis, r.mn[0], r.mn[1], r.mn[2], r.dims[0], g.h,
+                                                                   jx, jy, feats, slot);
     return hipGetLastError();
 }
 

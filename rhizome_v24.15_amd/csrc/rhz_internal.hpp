@@ -120,10 +120,31 @@ struct SweepTuning {
     int shape[3];
     int64_t batch_rounds_max_voxels;  // lattices up to this size enqueue all FSM rounds at once
     int fsm_graph;                    // 1: those rounds are captured and replayed as one hipGraph
+    // Per lattice-size class (shape_class): tile shape per orientation and
+    // steps per load group; 0 = the class-free value above.
+    int cls_shape[4][3];
+    int cls_step[4];
+    // The same for the batched bake's sweep launches (a stage of its own: they
+    // share the GPU with the sign stages); 0 = the class-free value.
+    int bcls_shape[4][3];
+    int bcls_step[4];
 };
+// Lattice-size classes of the per-class sweep tuning: <= 2^21, <= 2^24,
+// <= 2^27 voxels, larger.
+constexpr int kShapeClasses = 4;
+int shape_class(int64_t voxels);
+// The tile shape (orientation `axis`: the walk axis) and steps per load group
+// a sweep over a lattice of `voxels` cells uses.
+int effective_shape(int axis, int64_t voxels);
+int effective_step(int64_t voxels);
 SweepTuning sweep_tuning();
 bool set_sweep_tuning(int step, int rows, int fmajor);
 bool set_sweep_shape(int axis, int shape);
+// shape / step 0 clears the class's own value.
+bool set_class_shape(int cls, int axis, int shape);
+bool set_class_step(int cls, int step);
+bool set_batch_class_shape(int cls, int axis, int shape);
+bool set_batch_class_step(int cls, int step);
 void set_fsm_batching(int64_t max_voxels, int graph);
 // Per-plane transpose between the x-fastest and the y-fastest layouts (dense,
 // or the layouts of the input and the output).
@@ -160,9 +181,10 @@ struct TransJob {
 };
 // Sweep orientations: 0 = x-sweep on the y-fastest copy, 1 = y-sweep, 2 =
 // z-sweep (x-fastest copy).  Blocks of one job, and its fast-axis block
-// count (the job's nbf), for the batched tile shape of that orientation.
-int64_t sweep_batch_blocks(int orient, int nf, int nc, int* nbf);
-hipError_t launch_sweep_batch(int orient, int dir, const SweepJob* jobs, const unsigned* starts, int njobs,
+// count (the job's nbf), for the batched tile shape of that orientation and
+// lattice-size class; one launch covers jobs of one class.
+int64_t sweep_batch_blocks(int orient, int cls, int nf, int nc, int* nbf);
+hipError_t launch_sweep_batch(int orient, int cls, int dir, const SweepJob* jobs, const unsigned* starts, int njobs,
                               unsigned nblocks, int r, hipStream_t s);
 int64_t transpose_batch_blocks(int rows, int cols, int planes, int* tiles_r, int* tiles_c);
 hipError_t launch_transpose_batch(const TransJob* jobs, const unsigned* starts, int njobs, unsigned nblocks, int r,
@@ -285,11 +307,18 @@ hipError_t launch_margin_score(const float* feats, int64_t n, float* score, hipS
 // Triangles with axis `axis` (0 = x, 1 = y) rotated to the third coordinate
 // (as rotated_grid), for column lists along that axis.
 hipError_t launch_rotate_tris(const float* tris, int64_t nt, int axis, float* out, hipStream_t s);
-// Features `slot` of the n candidates: inside by the signed crossings of a ray
-// along `axis` (1), outside (0), open line (0.5); `b` holds the column lists of
-// the rotated triangles `rtris` over rotated_grid(g, axis).
-hipError_t launch_axis_parity(const float* rtris, const CsrBuffers& b, const int64_t* list, int64_t n,
-                              const Grid& g, int axis, int sup, float jx, float jy, float* feats, int slot,
+// Row lists of the rotated lattice r = rotated_grid(g, axis): row (iu, iv)
+// holds the rotated triangles whose (u, v) box contains its jittered ray
+// point.  Count (cover: int4 per triangle; counts: u32 per row; *total +=
+// entries), then, after the exclusive scan of counts into start, fill.
+hipError_t launch_axis_rows_count(const float* rtris, int64_t nt, const Grid& r, float jx, float jy, int4* cover,
+                                  uint32_t* counts, unsigned long long* total, hipStream_t s);
+hipError_t launch_axis_rows_fill(int64_t nt, const Grid& r, const int4* cover, const int64_t* start, uint32_t* cursor,
+                                 int32_t* ids, hipStream_t s);
+// Features `slot` of the n candidates: inside by the signed crossings of the
+// ray along `axis` (1), outside (0), open line (0.5).
+hipError_t launch_axis_parity(const float* rtris, const int64_t* start, const int32_t* ids, const int64_t* list,
+                              int64_t n, const Grid& g, int axis, float jx, float jy, float* feats, int slot,
                               hipStream_t s);
 hipError_t launch_fill_feature(float* feats, int64_t n, int slot, float value, hipStream_t s);
 // A uniform position in [0, 0.5) hashed from each candidate's voxel index, plus

@@ -141,8 +141,7 @@ enum Slot {
     S_ZKEY_TMP,
     S_ZSORT,
     S_ZEXT,
-    S_AX_TRIS,     // triangles rotated for x / y rays, and their column lists
-    S_AX_XYAB,
+    S_AX_TRIS,     // triangles rotated for x / y rays, and their row lists
     S_AX_COVER,
     S_AX_COUNTS,
     S_AX_START,
@@ -1158,43 +1157,41 @@ int learned_candidate_list(Ctx& c, int64_t** cand, int64_t* n_cand) {
 
 // Features of the candidates (and their certificate flags).
 // Features 5 and 7 (the x and y axis parity counts): the triangles rotated
-// so that x, then y, is the ray axis, their column lists built as for the z
-// scan, and one crossing count per candidate.  A line set whose lists would
-// exceed csr_max_entries gives every candidate 0.5 (no answer) instead.
+// so that x, then y, is the ray axis, exact row lists of the rotated lattice
+// (each triangle in the rows whose ray point its box contains), and one
+// crossing count per candidate.  Lists that would exceed csr_max_entries give
+// every candidate 0.5 (no answer) instead.
 int axis_parity_features(Ctx& c, const int64_t* list, int64_t n, float* f) {
     const rhz_sdf_params* p = c.p;
-    const int sup = p->sup;
     const int slots[2] = {5, 7};
     if (n <= 0) return RHZ_OK;
     float* rt = (float*)ws_get(c.w, S_AX_TRIS, sizeof(float) * 12 * (size_t)std::max<int64_t>(c.nt, 1));
-    if (!rt) return set_err(RHZ_ERR_NOMEM, "rotated triangle allocation failed");
+    int4* cover = (int4*)ws_get(c.w, S_AX_COVER, sizeof(int4) * (size_t)std::max<int64_t>(c.nt, 1));
+    if (!rt || !cover) return set_err(RHZ_ERR_NOMEM, "axis row-list allocation failed");
     for (int axis = 0; axis < 2; ++axis) {
         const Grid r = rotated_grid(c.g, axis);
-        const int64_t nsup = num_supercells(r, sup);
-        CsrBuffers b;
-        b.xyab = (float4*)ws_get(c.w, S_AX_XYAB, sizeof(float4) * (size_t)std::max<int64_t>(c.nt, 1));
-        b.cover = (int4*)ws_get(c.w, S_AX_COVER, sizeof(int4) * (size_t)std::max<int64_t>(c.nt, 1));
-        b.counts = (uint32_t*)ws_get(c.w, S_AX_COUNTS, sizeof(uint32_t) * (size_t)nsup);
-        b.start = (int64_t*)ws_get(c.w, S_AX_START, sizeof(int64_t) * (size_t)(nsup + 1));
-        b.cursor = (uint32_t*)ws_get(c.w, S_AX_CURSOR, sizeof(uint32_t) * (size_t)nsup);
-        void* scan = ws_get(c.w, S_AX_SCAN, scan_scratch_bytes(nsup));
-        if (!b.xyab || !b.cover || !b.counts || !b.start || !b.cursor || !scan)
-            return set_err(RHZ_ERR_NOMEM, "axis column-list allocation failed");
+        const int64_t rows = (int64_t)r.dims[0] * r.dims[1];
+        uint32_t* counts = (uint32_t*)ws_get(c.w, S_AX_COUNTS, sizeof(uint32_t) * (size_t)rows);
+        int64_t* start = (int64_t*)ws_get(c.w, S_AX_START, sizeof(int64_t) * (size_t)(rows + 1));
+        uint32_t* cursor = (uint32_t*)ws_get(c.w, S_AX_CURSOR, sizeof(uint32_t) * (size_t)rows);
+        void* scan = ws_get(c.w, S_AX_SCAN, scan_scratch_bytes(rows));
+        if (!counts || !start || !cursor || !scan) return set_err(RHZ_ERR_NOMEM, "axis row-list allocation failed");
         HIP_TRY(launch_rotate_tris(c.tris, c.nt, axis, rt, c.s));
         HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_AXCSR), 0, sizeof(unsigned long long), c.s));
-        HIP_TRY(launch_csr_count(rt, c.nt, r, sup, b, cnt_ptr(c.w, C_AXCSR), c.s));
+        HIP_TRY(launch_axis_rows_count(rt, c.nt, r, p->jitter_x, p->jitter_y, cover, counts, cnt_ptr(c.w, C_AXCSR),
+                                       c.s));
         const int rc = read_counters(c);
         if (rc) return rc;
         const int64_t entries = (int64_t)hc(c, C_AXCSR);
-        if (entries <= 0 || entries > p->csr_max_entries) {
-            HIP_TRY(launch_fill_feature(f, n, slots[axis], entries <= 0 ? 0.0f : 0.5f, c.s));
+        if (entries > p->csr_max_entries) {
+            HIP_TRY(launch_fill_feature(f, n, slots[axis], 0.5f, c.s));
             continue;
         }
-        b.ids = (int32_t*)ws_get(c.w, S_AX_IDS, sizeof(int32_t) * (size_t)entries);
-        if (!b.ids) return set_err(RHZ_ERR_NOMEM, "axis column-list allocation failed");
-        HIP_TRY(launch_scan_u32(b.counts, nsup, b.start, scan, c.s));
-        HIP_TRY(launch_csr_fill(c.nt, r, sup, b, c.s));
-        HIP_TRY(launch_axis_parity(rt, b, list, n, c.g, axis, sup, p->jitter_x, p->jitter_y, f, slots[axis], c.s));
+        int32_t* ids = (int32_t*)ws_get(c.w, S_AX_IDS, sizeof(int32_t) * (size_t)std::max<int64_t>(entries, 1));
+        if (!ids) return set_err(RHZ_ERR_NOMEM, "axis row-list allocation failed");
+        HIP_TRY(launch_scan_u32(counts, rows, start, scan, c.s));
+        HIP_TRY(launch_axis_rows_fill(c.nt, r, cover, start, cursor, ids, c.s));
+        HIP_TRY(launch_axis_parity(rt, start, ids, list, n, c.g, axis, p->jitter_x, p->jitter_y, f, slots[axis], c.s));
     }
     return RHZ_OK;
 }
@@ -1793,8 +1790,60 @@ int rhz_device_count(void) {
     return n;
 }
 
+// "<base>@<class>": a sweep tuning key of one lattice-size class (0..3).
+// Returns the class, or -1 when the key has no class suffix, or -2 when the
+// suffix is malformed; *base_len receives the length of the base name.
+static int class_suffix(const char* key, size_t* base_len) {
+    const char* at = std::strchr(key, '@');
+    *base_len = at ? (size_t)(at - key) : std::strlen(key);
+    if (!at) return -1;
+    if (at[1] < '0' || at[1] >= '0' + kShapeClasses || at[2] != '\0') return -2;
+    return at[1] - '0';
+}
+
+static bool base_is(const char* key, size_t len, const char* name) {
+    return std::strlen(name) == len && !std::strncmp(key, name, len);
+}
+
+static int class_tuning(const char* key, int cls, size_t len, int64_t* get, int64_t set) {
+    static const char* kShapeKeys[3] = {"sweep_shape_x", "sweep_shape_y", "sweep_shape_z"};
+    static const char* kBatchKeys[3] = {"batch_shape_x", "batch_shape_y", "batch_shape_z"};
+    const SweepTuning t = sweep_tuning();
+    if (cls < 0) return set_err(RHZ_ERR_ARG, "malformed class suffix in tuning key '%s'", key);
+    for (int b = 0; b < 2; ++b) {
+        if (base_is(key, len, b ? "batch_step" : "sweep_step")) {
+            if (get) {
+                *get = b ? t.bcls_step[cls] : t.cls_step[cls];
+                return RHZ_OK;
+            }
+            if (!(b ? set_batch_class_step(cls, (int)set) : set_class_step(cls, (int)set)))
+                return set_err(RHZ_ERR_ARG, "%s must be 0, 2, 4 or 8", key);
+            return RHZ_OK;
+        }
+        for (int a = 0; a < 3; ++a)
+            if (base_is(key, len, b ? kBatchKeys[a] : kShapeKeys[a])) {
+                if (get) {
+                    *get = b ? t.bcls_shape[cls][a] : t.cls_shape[cls][a];
+                    return RHZ_OK;
+                }
+                if (!(b ? set_batch_class_shape(cls, a, (int)set) : set_class_shape(cls, a, (int)set)))
+                    return set_err(RHZ_ERR_ARG, "%s must be 0, 1, 21, 22, 12 or 14", key);
+                return RHZ_OK;
+            }
+    }
+    return set_err(RHZ_ERR_ARG, "tuning key '%s' has no per-class form", key);
+}
+
+int rhz_sweep_shape_class(const int32_t dims[3]) {
+    if (!dims || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return set_err(RHZ_ERR_ARG, "dims must be positive");
+    return shape_class((int64_t)dims[0] * dims[1] * dims[2]);
+}
+
 int rhz_tuning_set(const char* key, int64_t value) {
     if (!key) return set_err(RHZ_ERR_ARG, "null tuning key");
+    size_t len = 0;
+    const int cls = class_suffix(key, &len);
+    if (cls != -1) return class_tuning(key, cls, len, nullptr, value);
     const SweepTuning t = sweep_tuning();
     if (!std::strcmp(key, "sweep_step")) {
         if (!set_sweep_tuning((int)value, t.rows, t.fmajor)) return set_err(RHZ_ERR_ARG, "sweep_step must be 2, 4 or 8");
@@ -1840,6 +1889,9 @@ int rhz_tuning_set(const char* key, int64_t value) {
 
 int rhz_tuning_get(const char* key, int64_t* value) {
     if (!key || !value) return set_err(RHZ_ERR_ARG, "null argument");
+    size_t len = 0;
+    const int cls = class_suffix(key, &len);
+    if (cls != -1) return class_tuning(key, cls, len, value, 0);
     const SweepTuning t = sweep_tuning();
     if (!std::strcmp(key, "sweep_step")) {
         *value = t.step;
@@ -2065,44 +2117,69 @@ static int bake_batched(const rhz_bake_item* items, int32_t n_items, const rhz_s
         }
     }
     // job tables: transposes (round 0 from the dense band, later rounds from
-    // a; the way back), sweeps x+/x- (y-fastest), y+/y-, z+/z-
-    enum { T_FIRST0 = 0, T_FIRST, T_BACK, S_XP, S_XM, S_YP, S_YM, S_ZP, S_ZM, kTables };
+    // a; the way back), sweeps x+/x- (y-fastest), y+/y-, z+/z-.  Jobs are
+    // ordered by lattice-size class (shape_class): a sweep launch covers the
+    // jobs of one class with that class's tile shape, and its block offsets
+    // (starts) begin at 0 for the class's first job.
+    std::vector<int> pos;  // item of every job position
+    int cb[kShapeClasses + 1] = {0};
+    for (int c = 0; c < kShapeClasses; ++c) {
+        cb[c] = (int)pos.size();
+        for (int i = 0; i < n; ++i)
+            if (shape_class(it[(size_t)i].g.n()) == c) pos.push_back(i);
+    }
+    cb[kShapeClasses] = n;
     const int nT = 3, nS = 6;
     std::vector<TransJob> tj((size_t)nT * n);
     std::vector<SweepJob> sj((size_t)nS * n);
-    std::vector<unsigned> starts((size_t)5 * (n + 1));  // transposes to y-fastest / back; sweeps x, y, z
-    unsigned nblk[5] = {0, 0, 0, 0, 0};
-    for (int i = 0; i < n; ++i) {
+    // transposes to y-fastest / back: n + 1 offsets each; sweeps per
+    // orientation and class: (jobs of the class) + 1 offsets
+    const size_t sweep_base = (size_t)2 * (n + 1);
+    auto sw_off = [&](int o, int c) { return sweep_base + (size_t)o * (n + kShapeClasses) + (size_t)(cb[c] + c); };
+    std::vector<unsigned> starts(sweep_base + (size_t)3 * (n + kShapeClasses));
+    unsigned ntr[2] = {0, 0};
+    unsigned nsw[3][kShapeClasses] = {};
+    for (int q = 0; q < n; ++q) {
+        const int i = pos[(size_t)q];
         const BakeItemState& s = it[(size_t)i];
+        const int cls = shape_class(s.g.n());
         const int dx = s.g.dims[0], dy = s.g.dims[1], dz = s.g.dims[2];
         const FieldLayout dense = dense_xy(s.g);
         int tr = 0, tc = 0;
         const int64_t b0 = transpose_batch_blocks(dy, dx, dz, &tr, &tc);
-        tj[(size_t)0 * n + i] = TransJob{items[i].d_out, s.b, dense.pitch, dense.plane, s.ly.pitch, s.ly.plane,
+        tj[(size_t)0 * n + q] = TransJob{items[i].d_out, s.b, dense.pitch, dense.plane, s.ly.pitch, s.ly.plane,
                                         dy, dx, tr, tc, s.flags};
-        tj[(size_t)1 * n + i] = TransJob{s.a, s.b, s.lx.pitch, s.lx.plane, s.ly.pitch, s.ly.plane, dy, dx, tr, tc,
+        tj[(size_t)1 * n + q] = TransJob{s.a, s.b, s.lx.pitch, s.lx.plane, s.ly.pitch, s.ly.plane, dy, dx, tr, tc,
                                         s.flags};
         const int64_t b1 = transpose_batch_blocks(dx, dy, dz, &tr, &tc);
-        tj[(size_t)2 * n + i] = TransJob{s.b, s.a, s.ly.pitch, s.ly.plane, s.lx.pitch, s.lx.plane, dx, dy, tr, tc,
+        tj[(size_t)2 * n + q] = TransJob{s.b, s.a, s.ly.pitch, s.ly.plane, s.lx.pitch, s.lx.plane, dx, dy, tr, tc,
                                         s.flags};
         int nbf = 0;
-        const int64_t bx = sweep_batch_blocks(0, dy, dz, &nbf);
-        sj[(size_t)0 * n + i] = SweepJob{s.b, s.a, s.ly.pitch, s.ly.plane, dy, dx, dz, nbf, s.g.h, s.flags};
-        sj[(size_t)1 * n + i] = SweepJob{s.a, s.b, s.ly.pitch, s.ly.plane, dy, dx, dz, nbf, s.g.h, s.flags};
-        const int64_t by = sweep_batch_blocks(1, dx, dz, &nbf);
-        sj[(size_t)2 * n + i] = SweepJob{s.a, s.b, s.lx.pitch, s.lx.plane, dx, dy, dz, nbf, s.g.h, s.flags};
-        sj[(size_t)3 * n + i] = SweepJob{s.b, s.a, s.lx.pitch, s.lx.plane, dx, dy, dz, nbf, s.g.h, s.flags};
-        const int64_t bz = sweep_batch_blocks(2, dx, dy, &nbf);
-        sj[(size_t)4 * n + i] = SweepJob{s.a, s.b, s.lx.plane, s.lx.pitch, dx, dz, dy, nbf, s.g.h, s.flags};
-        sj[(size_t)5 * n + i] = SweepJob{s.b, s.a, s.lx.plane, s.lx.pitch, dx, dz, dy, nbf, s.g.h, s.flags};
-        const int64_t bl[5] = {b0, b1, bx, by, bz};
-        for (int k = 0; k < 5; ++k) {
-            starts[(size_t)k * (n + 1) + i] = nblk[k];
-            if ((int64_t)nblk[k] + bl[k] >= (int64_t)1 << 31) return set_err(RHZ_ERR_ARG, "batch too large");
-            nblk[k] += (unsigned)bl[k];
+        const int64_t bx = sweep_batch_blocks(0, cls, dy, dz, &nbf);
+        sj[(size_t)0 * n + q] = SweepJob{s.b, s.a, s.ly.pitch, s.ly.plane, dy, dx, dz, nbf, s.g.h, s.flags};
+        sj[(size_t)1 * n + q] = SweepJob{s.a, s.b, s.ly.pitch, s.ly.plane, dy, dx, dz, nbf, s.g.h, s.flags};
+        const int64_t by = sweep_batch_blocks(1, cls, dx, dz, &nbf);
+        sj[(size_t)2 * n + q] = SweepJob{s.a, s.b, s.lx.pitch, s.lx.plane, dx, dy, dz, nbf, s.g.h, s.flags};
+        sj[(size_t)3 * n + q] = SweepJob{s.b, s.a, s.lx.pitch, s.lx.plane, dx, dy, dz, nbf, s.g.h, s.flags};
+        const int64_t bz = sweep_batch_blocks(2, cls, dx, dy, &nbf);
+        sj[(size_t)4 * n + q] = SweepJob{s.a, s.b, s.lx.plane, s.lx.pitch, dx, dz, dy, nbf, s.g.h, s.flags};
+        sj[(size_t)5 * n + q] = SweepJob{s.b, s.a, s.lx.plane, s.lx.pitch, dx, dz, dy, nbf, s.g.h, s.flags};
+        const int64_t bt[2] = {b0, b1};
+        for (int k = 0; k < 2; ++k) {
+            starts[(size_t)k * (n + 1) + q] = ntr[k];
+            if ((int64_t)ntr[k] + bt[k] >= (int64_t)1 << 31) return set_err(RHZ_ERR_ARG, "batch too large");
+            ntr[k] += (unsigned)bt[k];
+        }
+        const int64_t bs[3] = {bx, by, bz};
+        for (int o = 0; o < 3; ++o) {
+            starts[sw_off(o, cls) + (size_t)(q - cb[cls])] = nsw[o][cls];
+            if ((int64_t)nsw[o][cls] + bs[o] >= (int64_t)1 << 31) return set_err(RHZ_ERR_ARG, "batch too large");
+            nsw[o][cls] += (unsigned)bs[o];
         }
     }
-    for (int k = 0; k < 5; ++k) starts[(size_t)k * (n + 1) + n] = nblk[k];
+    for (int k = 0; k < 2; ++k) starts[(size_t)k * (n + 1) + n] = ntr[k];
+    for (int o = 0; o < 3; ++o)
+        for (int c = 0; c < kShapeClasses; ++c) starts[sw_off(o, c) + (size_t)(cb[c + 1] - cb[c])] = nsw[o][c];
     const size_t tj_bytes = sizeof(TransJob) * tj.size(), sj_bytes = sizeof(SweepJob) * sj.size();
     char* jobs = (char*)ws_get(&bw->ws, B_JOBS, tj_bytes + sj_bytes);
     unsigned* d_starts = (unsigned*)ws_get(&bw->ws, B_STARTS, sizeof(unsigned) * starts.size());
@@ -2112,7 +2189,19 @@ static int bake_batched(const rhz_bake_item* items, int32_t n_items, const rhz_s
     HIP_TRY(hipMemcpyAsync(d_starts, starts.data(), sizeof(unsigned) * starts.size(), hipMemcpyHostToDevice, ms));
     const TransJob* d_tj = reinterpret_cast<const TransJob*>(jobs);
     const SweepJob* d_sj = reinterpret_cast<const SweepJob*>(jobs + tj_bytes);
-    auto st_of = [&](int k) { return d_starts + (size_t)k * (n + 1); };
+    auto tr_starts = [&](int k) { return d_starts + (size_t)k * (n + 1); };
+    // the sweeps of orientation o, direction dir, over every class with jobs
+    auto sweep_all = [&](int o, int dir, int r) -> hipError_t {
+        const int tab = 2 * o + (dir > 0 ? 0 : 1);
+        for (int c = 0; c < kShapeClasses; ++c) {
+            const int cnt = cb[c + 1] - cb[c];
+            if (cnt == 0) continue;
+            const hipError_t e = launch_sweep_batch(o, c, dir, d_sj + (size_t)tab * n + cb[c], d_starts + sw_off(o, c),
+                                                    cnt, nsw[o][c], r, ms);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
 
     // phase 1 (lanes): fill and splat of every item
     const int n_lanes = (int)lanes.size();
@@ -2161,14 +2250,14 @@ static int bake_batched(const rhz_bake_item* items, int32_t n_items, const rhz_s
         hipError_t e = hipStreamBeginCapture(ms, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) e = launch_zero_i32(flags, n * kBakeRoundStride, ms);
         for (int r = 0; r < cap && e == hipSuccess; ++r) {
-            e = launch_transpose_batch(d_tj + (size_t)(r == 0 ? 0 : 1) * n, st_of(0), n, nblk[0], r, ms);
-            if (e == hipSuccess) e = launch_sweep_batch(0, +1, d_sj + (size_t)0 * n, st_of(2), n, nblk[2], r, ms);
-            if (e == hipSuccess) e = launch_sweep_batch(0, -1, d_sj + (size_t)1 * n, st_of(2), n, nblk[2], r, ms);
-            if (e == hipSuccess) e = launch_transpose_batch(d_tj + (size_t)2 * n, st_of(1), n, nblk[1], r, ms);
-            if (e == hipSuccess) e = launch_sweep_batch(1, +1, d_sj + (size_t)2 * n, st_of(3), n, nblk[3], r, ms);
-            if (e == hipSuccess) e = launch_sweep_batch(1, -1, d_sj + (size_t)3 * n, st_of(3), n, nblk[3], r, ms);
-            if (e == hipSuccess) e = launch_sweep_batch(2, +1, d_sj + (size_t)4 * n, st_of(4), n, nblk[4], r, ms);
-            if (e == hipSuccess) e = launch_sweep_batch(2, -1, d_sj + (size_t)5 * n, st_of(4), n, nblk[4], r, ms);
+            e = launch_transpose_batch(d_tj + (size_t)(r == 0 ? 0 : 1) * n, tr_starts(0), n, ntr[0], r, ms);
+            if (e == hipSuccess) e = sweep_all(0, +1, r);
+            if (e == hipSuccess) e = sweep_all(0, -1, r);
+            if (e == hipSuccess) e = launch_transpose_batch(d_tj + (size_t)2 * n, tr_starts(1), n, ntr[1], r, ms);
+            if (e == hipSuccess) e = sweep_all(1, +1, r);
+            if (e == hipSuccess) e = sweep_all(1, -1, r);
+            if (e == hipSuccess) e = sweep_all(2, +1, r);
+            if (e == hipSuccess) e = sweep_all(2, -1, r);
         }
         const hipError_t ec = hipStreamEndCapture(ms, &graph);
         if (e == hipSuccess) e = ec;
