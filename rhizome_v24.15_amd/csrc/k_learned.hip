@@ -353,11 +353,20 @@ __device__ __forceinline__ void ray_rows(float lo, float hi, int mn, int d, floa
     *last = i1;
 }
 
+// A triangle covering at most kSmallRows rows is handled by one lane; a
+// larger one goes to the `big` list, whose triangles grid-stride waves take
+// (the lanes striding over the rows), so a few large triangles covering
+// thousands of rows do not serialise on one lane, and many small ones do not
+// leave 63 lanes of a wave idle.
+constexpr int kSmallRows = 16;
+
 __global__ void axis_rows_count_kernel(const float* __restrict__ rtris, int64_t nt, int mnu, int mnv, int du, int dv,
                                        float h, float jx, float jy, int4* __restrict__ cover,
-                                       uint32_t* __restrict__ counts, unsigned long long* total) {
+                                       uint32_t* __restrict__ counts, int32_t* __restrict__ big,
+                                       unsigned long long* n_big, unsigned long long* total) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long mine = 0;
+    bool is_big = false;
     if (t < nt) {
         const Tri9 tr = load_tri(rtris, t);
         const float u0 = fminf(tr.ax, fminf(tr.bx, tr.cx)), u1 = fmaxf(tr.ax, fmaxf(tr.bx, tr.cx));
@@ -367,13 +376,32 @@ __global__ void axis_rows_count_kernel(const float* __restrict__ rtris, int64_t 
         ray_rows(v0, v1, mnv, dv, jy, h, &cv.z, &cv.w);
         cover[t] = cv;
         if (cv.y >= cv.x && cv.w >= cv.z) {
-            for (int iv = cv.z; iv <= cv.w; ++iv)
-                for (int iu = cv.x; iu <= cv.y; ++iu) atomicAdd(counts + (int64_t)iv * du + iu, 1u);
             mine = (unsigned long long)(cv.y - cv.x + 1) * (unsigned long long)(cv.w - cv.z + 1);
+            is_big = mine > (unsigned long long)kSmallRows;
+            if (!is_big)
+                for (int iv = cv.z; iv <= cv.w; ++iv)
+                    for (int iu = cv.x; iu <= cv.y; ++iu) atomicAdd(counts + (int64_t)iv * du + iu, 1u);
         }
     }
+    const long long slot = wave_append(n_big, is_big);
+    if (is_big) big[slot] = (int32_t)t;
     for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
-    if ((threadIdx.x & 63) == 0 && mine) atomicAdd(total, mine);
+    if (__lane_id() == 0 && mine) atomicAdd(total, mine);
+}
+
+// The big list's rows: one wave per triangle, the waves striding over the list.
+__global__ __launch_bounds__(256) void axis_rows_big_count_kernel(const int4* __restrict__ cover,
+                                                                  const int32_t* __restrict__ big,
+                                                                  const unsigned long long* n_big, int du,
+                                                                  uint32_t* __restrict__ counts) {
+    const int64_t nb = (int64_t)*n_big;
+    for (int64_t q = (int64_t)blockIdx.x * 4 + threadIdx.y; q < nb; q += (int64_t)gridDim.x * 4) {
+        const int4 cv = cover[big[q]];
+        const int w = cv.y - cv.x + 1;
+        const int64_t area = (int64_t)w * (cv.w - cv.z + 1);
+        for (int64_t k = threadIdx.x; k < area; k += 64)
+            atomicAdd(counts + (int64_t)(cv.z + k / w) * du + cv.x + k % w, 1u);
+    }
 }
 
 __global__ void axis_rows_fill_kernel(int64_t nt, int du, const int4* __restrict__ cover,
@@ -383,11 +411,31 @@ __global__ void axis_rows_fill_kernel(int64_t nt, int du, const int4* __restrict
     if (t >= nt) return;
     const int4 cv = cover[t];
     if (cv.y < cv.x || cv.w < cv.z) return;
+    if ((int64_t)(cv.y - cv.x + 1) * (cv.w - cv.z + 1) > kSmallRows) return;  // the big list's
     for (int iv = cv.z; iv <= cv.w; ++iv)
         for (int iu = cv.x; iu <= cv.y; ++iu) {
             const int64_t r = (int64_t)iv * du + iu;
             ids[start[r] + atomicAdd(cursor + r, 1u)] = (int32_t)t;
         }
+}
+
+__global__ __launch_bounds__(256) void axis_rows_big_fill_kernel(const int4* __restrict__ cover,
+                                                                 const int32_t* __restrict__ big,
+                                                                 const unsigned long long* n_big, int du,
+                                                                 const int64_t* __restrict__ start,
+                                                                 uint32_t* __restrict__ cursor,
+                                                                 int32_t* __restrict__ ids) {
+    const int64_t nb = (int64_t)*n_big;
+    for (int64_t q = (int64_t)blockIdx.x * 4 + threadIdx.y; q < nb; q += (int64_t)gridDim.x * 4) {
+        const int32_t t = big[q];
+        const int4 cv = cover[t];
+        const int w = cv.y - cv.x + 1;
+        const int64_t area = (int64_t)w * (cv.w - cv.z + 1);
+        for (int64_t k = threadIdx.x; k < area; k += 64) {
+            const int64_t r = (int64_t)(cv.z + k / w) * du + cv.x + k % w;
+            ids[start[r] + atomicAdd(cursor + r, 1u)] = t;
+        }
+    }
 }
 
 // Features 5 / 7: the signed crossing count of the candidate's row ray, with
@@ -646,19 +694,29 @@ hipError_t launch_rotate_tris(const float* tris, int64_t nt, int axis, float* ou
 }
 
 hipError_t launch_axis_rows_count(const float* rtris, int64_t nt, const Grid& r, float jx, float jy, int4* cover,
-                                  uint32_t* counts, unsigned long long* total, hipStream_t s) {
+                                  uint32_t* counts, int32_t* big, unsigned long long* n_big,
+                                  unsigned long long* total, hipStream_t s) {
     hipError_t e = hipMemsetAsync(counts, 0, sizeof(uint32_t) * (size_t)r.dims[0] * r.dims[1], s);
+    if (e == hipSuccess) e = hipMemsetAsync(n_big, 0, sizeof(unsigned long long), s);
     if (e != hipSuccess || nt <= 0) return e;
     axis_rows_count_kernel<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(rtris, nt, r.mn[0], r.mn[1], r.dims[0],
-                                                                        r.dims[1], r.h, jx, jy, cover, counts, total);
+                                                                        r.dims[1], r.h, jx, jy, cover, counts, big,
+                                                                        n_big, total);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    axis_rows_big_count_kernel<<<1024, dim3(64, 4), 0, s>>>(cover, big, n_big, r.dims[0], counts);
     return hipGetLastError();
 }
 
-hipError_t launch_axis_rows_fill(int64_t nt, const Grid& r, const int4* cover, const int64_t* start, uint32_t* cursor,
-                                 int32_t* ids, hipStream_t s) {
+hipError_t launch_axis_rows_fill(int64_t nt, const Grid& r, const int4* cover, const int32_t* big,
+                                 const unsigned long long* n_big, const int64_t* start, uint32_t* cursor, int32_t* ids,
+                                 hipStream_t s) {
     hipError_t e = hipMemsetAsync(cursor, 0, sizeof(uint32_t) * (size_t)r.dims[0] * r.dims[1], s);
     if (e != hipSuccess || nt <= 0) return e;
     axis_rows_fill_kernel<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(nt, r.dims[0], cover, start, cursor, ids);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    axis_rows_big_fill_kernel<<<1024, dim3(64, 4), 0, s>>>(cover, big, n_big, r.dims[0], start, cursor, ids);
     return hipGetLastError();
 }
 

@@ -309,12 +309,15 @@ hipError_t launch_margin_score(const float* feats, int64_t n, float* score, hipS
 hipError_t launch_rotate_tris(const float* tris, int64_t nt, int axis, float* out, hipStream_t s);
 // Row lists of the rotated lattice r = rotated_grid(g, axis): row (iu, iv)
 // holds the rotated triangles whose (u, v) box contains its jittered ray
-// point.  Count (cover: int4 per triangle; counts: u32 per row; *total +=
-// entries), then, after the exclusive scan of counts into start, fill.
+// point.  Count (cover: int4 per triangle; counts: u32 per row; big: int32
+// per triangle, with its count n_big, for the triangles of many rows; *total
+// += entries), then, after the exclusive scan of counts into start, fill.
 hipError_t launch_axis_rows_count(const float* rtris, int64_t nt, const Grid& r, float jx, float jy, int4* cover,
-                                  uint32_t* counts, unsigned long long* total, hipStream_t s);
-hipError_t launch_axis_rows_fill(int64_t nt, const Grid& r, const int4* cover, const int64_t* start, uint32_t* cursor,
-                                 int32_t* ids, hipStream_t s);
+                                  uint32_t* counts, int32_t* big, unsigned long long* n_big,
+                                  unsigned long long* total, hipStream_t s);
+hipError_t launch_axis_rows_fill(int64_t nt, const Grid& r, const int4* cover, const int32_t* big,
+                                 const unsigned long long* n_big, const int64_t* start, uint32_t* cursor, int32_t* ids,
+                                 hipStream_t s);
 // Features `slot` of the n candidates: inside by the signed crossings of the
 // ray along `axis` (1), outside (0), open line (0.5).
 hipError_t launch_axis_parity(const float* rtris, const int64_t* start, const int32_t* ids, const int64_t* list,

@@ -148,6 +148,7 @@ enum Slot {
     S_AX_CURSOR,
     S_AX_IDS,
     S_AX_SCAN,
+    S_AX_BIG,      // triangles covering many rows (handled a wave each)
     S_COUNT
 };
 
@@ -297,7 +298,7 @@ void* ws_get(Workspace* w, int slot, size_t bytes) {
 // counters: ull slots 0..15, int slots at byte offset 128 (32 ints)
 enum CounterId {
     C_CSR = 0, C_LIST = 1, C_FP64 = 2, C_FAR = 3, C_MISM = 4, C_SEL = 5, C_GROW = 6, C_RISKY = 7, C_NCL = 8,
-    C_UNDEC = 9, C_MISM2 = 10, C_INSIDE = 11, C_UNDEC2 = 12, C_AXCSR = 13
+    C_UNDEC = 9, C_MISM2 = 10, C_INSIDE = 11, C_UNDEC2 = 12, C_AXCSR = 13, C_AXBIG = 14
 };
 enum FlagId { F_UNKNOWN = 0, F_CHANGED = 1, F_ROUND0 = 2 };  // F_ROUND0 + r: round r changed a voxel
 constexpr int kMaxBatchedRounds = 24;
@@ -1167,7 +1168,8 @@ int axis_parity_features(Ctx& c, const int64_t* list, int64_t n, float* f) {
     if (n <= 0) return RHZ_OK;
     float* rt = (float*)ws_get(c.w, S_AX_TRIS, sizeof(float) * 12 * (size_t)std::max<int64_t>(c.nt, 1));
     int4* cover = (int4*)ws_get(c.w, S_AX_COVER, sizeof(int4) * (size_t)std::max<int64_t>(c.nt, 1));
-    if (!rt || !cover) return set_err(RHZ_ERR_NOMEM, "axis row-list allocation failed");
+    int32_t* big = (int32_t*)ws_get(c.w, S_AX_BIG, sizeof(int32_t) * (size_t)std::max<int64_t>(c.nt, 1));
+    if (!rt || !cover || !big) return set_err(RHZ_ERR_NOMEM, "axis row-list allocation failed");
     for (int axis = 0; axis < 2; ++axis) {
         const Grid r = rotated_grid(c.g, axis);
         const int64_t rows = (int64_t)r.dims[0] * r.dims[1];
@@ -1178,8 +1180,8 @@ int axis_parity_features(Ctx& c, const int64_t* list, int64_t n, float* f) {
         if (!counts || !start || !cursor || !scan) return set_err(RHZ_ERR_NOMEM, "axis row-list allocation failed");
         HIP_TRY(launch_rotate_tris(c.tris, c.nt, axis, rt, c.s));
         HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_AXCSR), 0, sizeof(unsigned long long), c.s));
-        HIP_TRY(launch_axis_rows_count(rt, c.nt, r, p->jitter_x, p->jitter_y, cover, counts, cnt_ptr(c.w, C_AXCSR),
-                                       c.s));
+        HIP_TRY(launch_axis_rows_count(rt, c.nt, r, p->jitter_x, p->jitter_y, cover, counts, big,
+                                       cnt_ptr(c.w, C_AXBIG), cnt_ptr(c.w, C_AXCSR), c.s));
         const int rc = read_counters(c);
         if (rc) return rc;
         const int64_t entries = (int64_t)hc(c, C_AXCSR);
@@ -1190,7 +1192,7 @@ int axis_parity_features(Ctx& c, const int64_t* list, int64_t n, float* f) {
         int32_t* ids = (int32_t*)ws_get(c.w, S_AX_IDS, sizeof(int32_t) * (size_t)std::max<int64_t>(entries, 1));
         if (!ids) return set_err(RHZ_ERR_NOMEM, "axis row-list allocation failed");
         HIP_TRY(launch_scan_u32(counts, rows, start, scan, c.s));
-        HIP_TRY(launch_axis_rows_fill(c.nt, r, cover, start, cursor, ids, c.s));
+        HIP_TRY(launch_axis_rows_fill(c.nt, r, cover, big, cnt_ptr(c.w, C_AXBIG), start, cursor, ids, c.s));
         HIP_TRY(launch_axis_parity(rt, start, ids, list, n, c.g, axis, p->jitter_x, p->jitter_y, f, slots[axis], c.s));
     }
     return RHZ_OK;
