@@ -54,7 +54,14 @@ enum rhz_fsm_policy {
     RHZ_FSM_REFERENCE = 0, /* <= max_rounds rounds (exit on no change), sentinel audit,
                               <= max_extra_rounds audit-gated rounds, safe clamp
                               (cuda_rasterizer.cu:1010-1063) */
-    RHZ_FSM_CONVERGE = 1   /* rounds until no voxel changes (cap fsm_round_cap) */
+    RHZ_FSM_CONVERGE = 1,  /* rounds until no voxel changes (cap fsm_round_cap) */
+    RHZ_FSM_OCTANT = 2     /* rounds of the 8 classical orderings (Zhao 2005,
+                              rhz_stage_sweep_octant) until no voxel changes (cap
+                              fsm_round_cap); the audit, extension and clamp of
+                              RHZ_FSM_REFERENCE follow.  No counterpart in the
+                              reference's pipeline (its 6-axis rounds,
+                              cuda_rasterizer.cu:1010-1023); the 8-direction
+                              form the plain small configuration names */
 };
 
 /* Winding-number arithmetic used by every exact-sign decision. */
@@ -403,6 +410,14 @@ int rhz_stage_fixup(const rhz_triangle* d_tris, int64_t n_tris, float h,
  * (cuda_rasterizer.cu:405-513).  *changed |= 1 if any voxel decreased. */
 int rhz_stage_sweep(const float* src, float* dst, const int32_t dims[3], float h,
                     int32_t axis, int32_t dir, int32_t* changed, void* stream);
+/* [D] one classical fast-sweeping ordering (sx, sy, sz each +1 or -1) in place
+ * on the dense field d: the lexicographic Gauss-Seidel sweep (z outermost, x
+ * innermost, each axis in its direction), run as one launch per diagonal
+ * level kx + ky + kz (identical results).  *changed |= 1 if any voxel
+ * decreased.  No counterpart in the reference (its sweeps are the 6 axis
+ * sweeps, cuda_rasterizer.cu:405-513). */
+int rhz_stage_sweep_octant(float* d, const int32_t dims[3], float h, int32_t sx, int32_t sy, int32_t sz,
+                           int32_t* changed, void* stream);
 /* [D] one round x+, x-, y+, y-, z+, z- exactly as the pipeline runs it
  * (cuda_rasterizer.cu:1012-1017): the field is in `a` before and after; `b`
  * is the second buffer (same size). */

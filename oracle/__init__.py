@@ -132,6 +132,8 @@ def lib():
         L.orc_fsm_sweep_inplace.restype = C.c_int
         L.orc_fsm_sweep_octant.argtypes = [_f32p, _i32p, C.c_float, C.c_int, C.c_int, C.c_int]
         L.orc_fsm_sweep_octant.restype = C.c_int
+        L.orc_fsm_octant_round.argtypes = [_f32p, _i32p, C.c_float]
+        L.orc_fsm_octant_round.restype = C.c_int
         L.orc_count_far.argtypes = [_f32p, C.c_int64]
         L.orc_count_far.restype = C.c_int64
         L.orc_clamp_far.argtypes = [_f32p, _u8p, C.c_int64, C.c_float, C.c_int]
@@ -240,6 +242,15 @@ def fsm_round(d, dims, h):
     """One round x+, x-, y+, y-, z+, z- of the pinned schedule, in place on
     float32 ``d``; returns 1 if a voxel decreased."""
     return lib().orc_fsm_round(d, _i32(dims), h)
+
+
+def octant_converge(d, dims, h, max_rounds=512):
+    """Rounds of the 8 classical orderings until nothing changes; returns rounds."""
+    dm = _i32(dims)
+    for r in range(max_rounds):
+        if not lib().orc_fsm_octant_round(d, dm, h):
+            return r + 1
+    return max_rounds
 
 
 def fsm_converge(d, dims, h, max_rounds=512):

@@ -996,6 +996,14 @@ int orc_fsm_sweep_octant(float* g, const int32_t dims[3], float h, int sx, int s
     return changed;
 }
 
+int orc_fsm_octant_round(float* d, const int32_t dims[3], float h) {
+    int ch = 0;
+    for (int sz = 1; sz >= -1; sz -= 2)
+        for (int sy = 1; sy >= -1; sy -= 2)
+            for (int sx = 1; sx >= -1; sx -= 2) ch |= orc_fsm_sweep_octant(d, dims, h, sx, sy, sz);
+    return ch;
+}
+
 /* gpu_count_far_sentinels_kernel; cuda_rasterizer.cu:523-534. */
 static void count_far_range(void* vctx, int64_t b, int64_t e, int tid) {
     (void)tid;
@@ -1103,16 +1111,20 @@ static void fsm_rounds(float* d, const int32_t dims[3], float h, float bw,
     int64_t n = (int64_t)dims[0] * dims[1] * dims[2];
     float* tmp = (float*)malloc(sizeof(float) * (size_t)n);
     int rounds = 0;
+    const int octant = p->fsm_converge == 2;
     int cap = p->fsm_converge ? p->fsm_round_cap : p->max_rounds;
     for (int r = 0; r < cap; ++r) {
-        int ch = fsm_round_db(d, tmp, dims, h);
+        int ch = octant ? orc_fsm_octant_round(d, dims, h) : fsm_round_db(d, tmp, dims, h);
         rounds = r + 1;
         if (!ch) break;
     }
     int64_t resid = orc_count_far(d, n);
     int extra = 0;
     while (resid > 0 && extra < p->max_extra_rounds) {
-        (void)fsm_round_db(d, tmp, dims, h);
+        if (octant)
+            (void)orc_fsm_octant_round(d, dims, h);
+        else
+            (void)fsm_round_db(d, tmp, dims, h);
         ++extra;
         resid = orc_count_far(d, n);
     }

@@ -33,6 +33,11 @@
  *                                    a voxel, so the schedule itself is pinned
  *                                    (double-buffered lines, Gauss-Seidel
  *                                    along each line)
+ *   orc_fsm_sweep_octant / _round    planar seeds give the exact linear field
+ *                                    in one round; every ordering equals an
+ *                                    independent diagonal-level traversal
+ *                                    bit for bit, and a planted level
+ *                                    reversal differs
  *   orc_parity_signs                 brute-force ray/triangle counting; equals
  *                                    exact winding sign away from the surface on
  *                                    watertight meshes
@@ -104,7 +109,10 @@ typedef struct orc_params {
                                   (intended, -bw inside); 0: reference
                                   double-sign behaviour (:553 + :659) */
     int fsm_converge;          /* 1: run rounds until nothing changes
-                                  (cap fsm_round_cap); 0: reference cap */
+                                  (cap fsm_round_cap); 0: reference cap;
+                                  2: rounds of the 8 classical orderings
+                                  (orc_fsm_octant_round) until nothing
+                                  changes (cap fsm_round_cap) */
     int fsm_round_cap;
     int winding_mode;          /* sign decisions of fixup/verify: 0 plain FP32
                                   (reference), 3 compensated FP32 with a
@@ -202,6 +210,8 @@ int orc_fsm_sweep_inplace(float* d, const int32_t dims[3], float h, int axis, in
 /* Classical 8-ordering sweep (sx,sy,sz in {+1,-1}); returns changed. */
 int orc_fsm_sweep_octant(float* d, const int32_t dims[3], float h,
                          int sx, int sy, int sz);
+/* The 8 orderings in turn, sz outermost (+1 first), then sy, then sx. */
+int orc_fsm_octant_round(float* d, const int32_t dims[3], float h);
 int64_t orc_count_far(const float* d, int64_t n);
 void orc_clamp_far(float* d, const uint8_t* sign, int64_t n, float bw,
                    int intended_sign);

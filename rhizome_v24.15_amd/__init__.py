@@ -33,6 +33,7 @@ METHOD_FAST_LEARNED = 4
 
 FSM_REFERENCE = 0
 FSM_CONVERGE = 1
+FSM_OCTANT = 2
 
 WINDING_FP32 = 0
 WINDING_FP32_KAHAN = 1
@@ -220,6 +221,7 @@ def lib():
     L.rhz_stage_fixup.argtypes = [vp, i64, f32, vp, vp, i32, vp, C.POINTER(C.c_int64), vp]
     L.rhz_stage_sweep.argtypes = [vp, vp, vp, f32, i32, i32, vp, vp]
     L.rhz_stage_sweep_round.argtypes = [vp, vp, vp, f32, vp, vp]
+    L.rhz_stage_sweep_octant.argtypes = [vp, vp, f32, i32, i32, i32, vp, vp]
     L.rhz_stage_count_far.argtypes = [vp, i64, C.POINTER(C.c_int64), vp]
     L.rhz_stage_clamp_far.argtypes = [vp, vp, i64, f32, i32, vp]
     L.rhz_stage_verify.argtypes = [vp, i64, f32, vp, vp, i32, i32, i32, vp, vp, C.POINTER(C.c_int32), vp]
@@ -550,6 +552,11 @@ class Stages:
     def sweep(src, dst, dims, h, axis, direction, d_changed, stream=None):
         check(lib().rhz_stage_sweep(_ptr(src), _ptr(dst), _i32x3(dims), float(h), int(axis),
                                     int(direction), _ptr(d_changed), stream), "rhz_stage_sweep")
+
+    @staticmethod
+    def sweep_octant(d, dims, h, sx, sy, sz, d_changed, stream=None):
+        check(lib().rhz_stage_sweep_octant(_ptr(d), _i32x3(dims), float(h), int(sx), int(sy), int(sz),
+                                           _ptr(d_changed), stream), "rhz_stage_sweep_octant")
 
     @staticmethod
     def sweep_round(a, b, dims, h, d_changed, stream=None):

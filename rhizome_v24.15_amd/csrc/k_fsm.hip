@@ -735,6 +735,44 @@ __global__ __launch_bounds__(256) void transpose_batch_kernel(const TransJob* __
                     J.tiles_c, lb - starts[j], tile);
 }
 
+// ---- classical 8-ordering fast sweeping (Zhao 2005) ---------------------------
+// Ordering (sx, sy, sz) visits the cells lexicographically (z outermost, x
+// innermost), each axis in its direction, updating in place: a cell reads its
+// three upwind neighbours already updated and its three downwind ones not yet.
+// Its diagonal levels kx + ky + kz = L (k = position along each axis in the
+// ordering's direction) run one launch each: a cell of level L has its upwind
+// neighbours on level L - 1 and its downwind ones on level L + 1, so it reads
+// exactly what it reads in the lexicographic order, and the sweep equals the
+// in-place lexicographic one bit for bit.  Cells of one level never read each
+// other.  One thread per (kx, ky) of the level's box; kz follows.
+__global__ void octant_level_kernel(float* __restrict__ g, int dx, int dy, int dz, float h, int sx, int sy, int sz,
+                                    int level, int kx0, int nkx, int ky0, int64_t nk, int* __restrict__ changed) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool chg = false;
+    if (t < nk) {
+        const int kx = kx0 + (int)(t % nkx), ky = ky0 + (int)(t / nkx);
+        const int kz = level - kx - ky;
+        if (kz >= 0 && kz < dz) {
+            const int x = sx > 0 ? kx : dx - 1 - kx;
+            const int y = sy > 0 ? ky : dy - 1 - ky;
+            const int z = sz > 0 ? kz : dz - 1 - kz;
+            const int64_t ys = dx, zs = (int64_t)dx * dy;
+            const int64_t i = (int64_t)z * zs + (int64_t)y * ys + x;
+            const float S = 1e30f;
+            const float cur = g[i];
+            const float ax = fminf(x > 0 ? g[i - 1] : S, x < dx - 1 ? g[i + 1] : S);
+            const float ay = fminf(y > 0 ? g[i - ys] : S, y < dy - 1 ? g[i + ys] : S);
+            const float az = fminf(z > 0 ? g[i - zs] : S, z < dz - 1 ? g[i + zs] : S);
+            const float u = godunov(ax, ay, az, h);
+            if (u < cur) {
+                g[i] = u;
+                chg = true;
+            }
+        }
+    }
+    if (__ballot(chg) != 0 && (threadIdx.x & 63) == 0) atomicOr(changed, 1);
+}
+
 // ---- standalone red-black eikonal (cuda_rasterizer.cu:1536-1578) ---------------
 __global__ void redblack_kernel(float* __restrict__ g, int dx, int dy, int dz, float h, int parity) {
     int64_t n = (int64_t)dx * dy * dz;
@@ -1070,6 +1108,24 @@ hipError_t launch_transpose_batch(const TransJob* jobs, const unsigned* starts, 
     if (njobs <= 0 || nblocks == 0) return hipSuccess;
     transpose_batch_kernel<<<nblocks, 256, 0, s>>>(jobs, starts, njobs, nblocks, r);
     return hipGetLastError();
+}
+
+hipError_t launch_sweep_octant(float* g, const Grid& gr, int sx, int sy, int sz, int* changed, hipStream_t s) {
+    const int dx = gr.dims[0], dy = gr.dims[1], dz = gr.dims[2];
+    if (dx <= 0 || dy <= 0 || dz <= 0) return hipSuccess;
+    const int levels = dx + dy + dz - 2;
+    for (int L = 0; L < levels; ++L) {
+        const int kx0 = std::max(0, L - (dy - 1) - (dz - 1)), kx1 = std::min(dx - 1, L);
+        const int ky0 = std::max(0, L - (dx - 1) - (dz - 1)), ky1 = std::min(dy - 1, L);
+        if (kx1 < kx0 || ky1 < ky0) continue;
+        const int nkx = kx1 - kx0 + 1;
+        const int64_t nk = (int64_t)nkx * (ky1 - ky0 + 1);
+        octant_level_kernel<<<(unsigned)((nk + 255) / 256), 256, 0, s>>>(g, dx, dy, dz, gr.h, sx, sy, sz, L, kx0, nkx,
+                                                                          ky0, nk, changed);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_redblack(float* g, const int32_t dims[3], float h, int parity, hipStream_t s) {

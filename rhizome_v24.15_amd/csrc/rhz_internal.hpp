@@ -146,6 +146,10 @@ bool set_class_step(int cls, int step);
 bool set_batch_class_shape(int cls, int axis, int shape);
 bool set_batch_class_step(int cls, int step);
 void set_fsm_batching(int64_t max_voxels, int graph);
+// One classical fast-sweeping ordering (sx, sy, sz in {+1, -1}) in place on
+// the dense x-fastest field g: the lexicographic Gauss-Seidel sweep, run as
+// one launch per diagonal level.  *changed |= 1 if a voxel decreased.
+hipError_t launch_sweep_octant(float* g, const Grid& gr, int sx, int sy, int sz, int* changed, hipStream_t s);
 // Per-plane transpose between the x-fastest and the y-fastest layouts (dense,
 // or the layouts of the input and the output).
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s);

@@ -512,6 +512,57 @@ int sweep_round(Ctx& c, float* a, float* b, int* changed) {
     return RHZ_OK;
 }
 
+// One round of the 8 classical orderings (sz outermost, +1 first, then sy,
+// then sx) in place on the dense field c.d.
+int octant_round(Ctx& c, int* changed) {
+    for (int sz = 1; sz >= -1; sz -= 2)
+        for (int sy = 1; sy >= -1; sy -= 2)
+            for (int sx = 1; sx >= -1; sx -= 2) {
+                if (c.sweeps_t) c.sweeps_t->begin(c.s);
+                HIP_TRY(launch_sweep_octant(c.d, c.g, sx, sy, sz, changed, c.s));
+                if (c.sweeps_t) c.sweeps_t->end(c.s);
+            }
+    c.st->sweeps += 8;
+    return RHZ_OK;
+}
+
+// Octant rounds until one changes nothing (at most cap), each a replay of one
+// captured round.  The graph is recorded on c.s without per-sweep events;
+// ms_sweeps then counts each replay as one interval.
+int octant_rounds_graph(Ctx& c, int cap, int* changed, int* rounds_out) {
+    EventAcc* sw = c.sweeps_t;
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+    c.sweeps_t = nullptr;  // no event records inside the capture
+    int rc = octant_round(c, changed);
+    c.st->sweeps -= 8;  // counted per replay below
+    const hipError_t ec = hipStreamEndCapture(c.s, &gr);
+    c.sweeps_t = sw;
+    hipError_t e = ec;
+    if (rc == RHZ_OK && e == hipSuccess) e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    for (int r = 0; rc == RHZ_OK && e == hipSuccess && r < cap; ++r) {
+        e = launch_zero_i32(changed, 1, c.s);
+        if (e != hipSuccess) break;
+        if (sw) sw->begin(c.s);
+        e = hipGraphLaunch(ge, c.s);
+        if (sw) sw->end(c.s);
+        if (e != hipSuccess) break;
+        c.st->sweeps += 8;
+        rc = read_counters(c);
+        *rounds_out = r + 1;
+        if (rc == RHZ_OK && hf(c, F_CHANGED) == 0) break;
+    }
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (gr) (void)hipGraphDestroy(gr);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(RHZ_ERR_HIP, "octant rounds: %s", hipGetErrorString(e));
+    }
+    return RHZ_OK;
+}
+
 // Sentinel count of the x-fastest field d in layout L (the caller's dense
 // field when d is null).
 int audit(Ctx& c, int64_t* far, const float* d = nullptr, const FieldLayout* L = nullptr) {
@@ -592,18 +643,22 @@ struct FsmField {
 int run_fsm_rounds(Ctx& c, int64_t* far_out, FsmField* keep = nullptr) {
     const int64_t V = c.g.n();
     const int64_t dx = c.g.dims[0], dz = c.g.dims[2];
-    const bool lanes = lane_sweeps_ok(c.g);
+    // RHZ_FSM_OCTANT: rounds of the 8 classical orderings in place on the
+    // dense field until nothing changes
+    const bool octant = c.p->fsm_policy == RHZ_FSM_OCTANT;
+    const bool lanes = !octant && lane_sweeps_ok(c.g);
     const FieldLayout dense = dense_xy(c.g);
     FieldLayout lx = dense, ly = dense_yx(c.g);
     if (lanes) fsm_layouts(c.g, &lx, &ly);
     const bool padded = lx.plane != dense.plane || ly.plane != dense.plane;
     const size_t cells = padded ? (size_t)(std::max(lx.plane, ly.plane) * dz) : (size_t)V;
-    float* q = (float*)ws_get(c.w, S_DIST2, sizeof(float) * cells);
+    float* q = octant ? nullptr : (float*)ws_get(c.w, S_DIST2, sizeof(float) * cells);
     float* pbuf = padded ? (float*)ws_get(c.w, S_DIST3, sizeof(float) * cells) : c.d;
-    if (!q || !pbuf) return set_err(RHZ_ERR_NOMEM, "sweep buffer allocation failed");
+    if ((!octant && !q) || !pbuf) return set_err(RHZ_ERR_NOMEM, "sweep buffer allocation failed");
     const float* cur = c.d;  // the field: the caller's dense buffer until a round has run
     FieldLayout cur_l = dense;
     auto round = [&](int* flag) -> int {
+        if (octant) return octant_round(c, flag);
         if (!lanes) return sweep_round(c, c.d, q, flag);
         const int rc = sweep_round_p(c, cur, cur_l, pbuf, q, lx, ly, flag);
         cur = pbuf;
@@ -611,11 +666,17 @@ int run_fsm_rounds(Ctx& c, int64_t* far_out, FsmField* keep = nullptr) {
         return rc;
     };
     int* changed = flag_ptr(c.w, F_CHANGED);
-    const bool converge = c.p->fsm_policy == RHZ_FSM_CONVERGE;
+    const bool converge = c.p->fsm_policy == RHZ_FSM_CONVERGE || octant;
     const int cap = converge ? c.p->fsm_round_cap : c.p->max_rounds;
     int rounds = 0;
     const SweepTuning tune = sweep_tuning();
-    if (!converge && cap > 0 && cap <= kMaxBatchedRounds && V <= tune.batch_rounds_max_voxels) {
+    if (octant && cap > 0 && c.s != nullptr) {
+        // one round (8 orderings, a launch per diagonal level) captured once
+        // and replayed per round: the flag is cleared by a kernel before each
+        // replay and read after it
+        int rc = octant_rounds_graph(c, cap, changed, &rounds);
+        if (rc) return rc;
+    } else if (!converge && cap > 0 && cap <= kMaxBatchedRounds && V <= tune.batch_rounds_max_voxels) {
         int rc = fsm_rounds_batched(c, cap, &rounds, round);
         if (rc) return rc;
     } else
@@ -1645,6 +1706,8 @@ int check_params(const rhz_sdf_params* p) {
         return set_err(RHZ_ERR_ARG, "parity_max_hits must be in [1, 128]");
     if (p->sup < 1 || p->band_voxels < 0 || p->verify_samples < 1 || p->verify_gens < 1 || p->verify_gens > 3)
         return set_err(RHZ_ERR_ARG, "invalid policy constants");
+    if (p->fsm_policy < RHZ_FSM_REFERENCE || p->fsm_policy > RHZ_FSM_OCTANT)
+        return set_err(RHZ_ERR_ARG, "fsm_policy must be RHZ_FSM_REFERENCE, RHZ_FSM_CONVERGE or RHZ_FSM_OCTANT");
     return RHZ_OK;
 }
 
@@ -2942,6 +3005,21 @@ int rhz_stage_sweep(const float* src, float* dst, const int32_t dims[3], float h
     }
     g.h = h;
     HIP_TRY(launch_sweep(src, dst, g, axis, dir, changed, (hipStream_t)stream));
+    return RHZ_OK;
+}
+
+int rhz_stage_sweep_octant(float* d, const int32_t dims[3], float h, int32_t sx, int32_t sy, int32_t sz,
+                           int32_t* changed, void* stream) {
+    if ((sx != 1 && sx != -1) || (sy != 1 && sy != -1) || (sz != 1 && sz != -1))
+        return set_err(RHZ_ERR_ARG, "orderings are +1 / -1 per axis");
+    if (!dims || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return set_err(RHZ_ERR_ARG, "dims must be positive");
+    Grid g;
+    for (int j = 0; j < 3; ++j) {
+        g.mn[j] = 0;
+        g.dims[j] = dims[j];
+    }
+    g.h = h;
+    HIP_TRY(launch_sweep_octant(d, g, sx, sy, sz, changed, (hipStream_t)stream));
     return RHZ_OK;
 }
 

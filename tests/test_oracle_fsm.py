@@ -211,3 +211,88 @@ def test_textbook_update_fixed_point_is_order_independent():
             break
     assert np.all(a < 1e29)
     np.testing.assert_allclose(a, b, rtol=0, atol=1e-12)
+
+
+ORDERINGS = [(sx, sy, sz) for sz in (1, -1) for sy in (1, -1) for sx in (1, -1)]
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_octant_planar_seed_gives_linear_field(orc, axis):
+    """The 8 classical orderings from a planar seed reach the exact linear
+    field (h a power of two: every sum is exact) in one round; a second
+    round changes nothing."""
+    dims = np.array([30, 20, 25], np.int32)
+    h = 0.5
+    d, idx = planar_seed(dims, h, axis)
+    assert orc.octant_converge(d, dims, h) == 2
+    assert np.array_equal(d, (idx * np.float32(h)).astype(np.float32))
+
+
+def _wavefront_sweep(orc, g, dims, h, sx, sy, sz, wrong=False):
+    """The ordering as diagonal levels kx + ky + kz = L (each axis counted in
+    its direction), cell by cell with the oracle's Godunov update.  A cell of
+    level L reads level L - 1 (updated) and L + 1 (not yet): the same values
+    as the lexicographic in-place order.  wrong=True reverses the level order
+    (the downwind side goes first), a planted mistake."""
+    nx, ny, nz = (int(v) for v in dims)
+    a = g.reshape(nz, ny, nx)
+    big = np.float32(1e30)
+    levels = range(nx + ny + nz - 2)
+    for L in (reversed(levels) if wrong else levels):
+        for kx in range(nx):
+            for ky in range(ny):
+                kz = L - kx - ky
+                if not 0 <= kz < nz:
+                    continue
+                x = kx if sx > 0 else nx - 1 - kx
+                y = ky if sy > 0 else ny - 1 - ky
+                z = kz if sz > 0 else nz - 1 - kz
+                ax = min(a[z, y, x - 1] if x > 0 else big, a[z, y, x + 1] if x < nx - 1 else big)
+                ay = min(a[z, y - 1, x] if y > 0 else big, a[z, y + 1, x] if y < ny - 1 else big)
+                az = min(a[z - 1, y, x] if z > 0 else big, a[z + 1, y, x] if z < nz - 1 else big)
+                u = np.float32(orc.lib().orc_godunov(ax, ay, az, h))
+                if u < a[z, y, x]:
+                    a[z, y, x] = u
+
+
+def _seeded_field(dims, seed):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(dims))
+    d = np.full(n, 1e30, np.float32)
+    pick = rng.choice(n, 5, replace=False)
+    d[pick] = rng.uniform(0.0, 1.5, 5).astype(np.float32)
+    return d
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_octant_sweep_equals_wavefront_restatement(orc, seed):
+    """orc_fsm_sweep_octant against an independent traversal (diagonal
+    levels) for every ordering, bit for bit; the planted level reversal must
+    differ."""
+    dims = np.array([9, 7, 6], np.int32)
+    h = 0.7
+    d = _seeded_field(dims, seed)
+    ref = d.copy()
+    for o in ORDERINGS:
+        before = d.copy()
+        orc.lib().orc_fsm_sweep_octant(d, orc._i32(dims), h, *o)
+        w = before.copy()
+        _wavefront_sweep(orc, w, dims, h, *o)
+        assert np.array_equal(d, w), o
+    bad = ref.copy()
+    for o in ORDERINGS:
+        _wavefront_sweep(orc, bad, dims, h, *o, wrong=True)
+    assert not np.array_equal(bad, d)
+
+
+def test_octant_policy_in_the_oracle_pipeline(orc, small_sphere):
+    """fsm_converge = 2 runs octant rounds until nothing changes: the
+    pipeline's field equals octant_converge on its band, with the signs."""
+    tris, h, mn, mx = small_sphere
+    dims = orc.dims_of(mn, mx)
+    rc, out, sg, st = orc.fast_sdf(tris, h, mn, mx, orc.default_params(fsm_converge=2), want_sign=True)
+    assert rc == 0 and st.far_residual == 0
+    band = orc.splat_band(tris, h, mn, dims)
+    rounds = orc.octant_converge(band, dims, h)
+    assert st.eikonal_rounds == rounds
+    assert np.array_equal(np.abs(out), band)
