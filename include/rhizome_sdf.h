@@ -197,6 +197,13 @@ int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
  *   "sweep_rows", "sweep_fmajor"   waves per block (4, 8, 16) and 1: a block's
  *                                  waves take adjacent segments of one row, for
  *                                  the one-line walker;
+ *   "fsm_layout"                   1 (default): every sweep of the rounds on a
+ *                                  y-fastest copy, one plane transpose before the
+ *                                  first round and one after the last; 0: the x-
+ *                                  sweeps on a y-fastest copy, the others x
+ *                                  fastest, two transposes every round;
+ *   "sweep_run_chunk"              16, 32 (default): cells per register chunk of
+ *                                  the y-sweep walking along the y-fastest rows;
  *   "fsm_batch_rounds_max_voxels"  lattices up to this size enqueue all FSM
  *                                  rounds without host round trips;
  *   "fsm_graph"                    1: those rounds replay as one captured graph;

@@ -11,13 +11,15 @@
 // Godunov update is not monotone in the largest neighbour and therefore
 // schedule-dependent.
 //
-// Memory layout is x fastest.  The y- and z-sweeps put 64 consecutive x in
-// one wave, so every step is a coalesced 256-byte row access, and issue the
-// loads of kStep steps together so each wave keeps several rows in flight.
-// The x-sweeps run the same kernel on a y-fastest copy of the field (plane
-// transposes through LDS before and after).  Block order is remapped so
-// neighbouring tiles share an XCD's L2.  A generic x-sweep kernel covers
-// lattices too large for 32-bit lane offsets.
+// The field arrives x fastest.  By default the rounds run on a y-fastest
+// copy (one plane transpose before the first round and one after the last):
+// the x- and z-sweeps put 64 consecutive y in one wave, so every step is a
+// coalesced row access, and the y-sweep walks along the rows themselves, a
+// row per lane, in register chunks staged through LDS.  The other layout
+// (tuning "fsm_layout" 0) keeps the field x fastest for the y- and z-sweeps
+// and transposes around the x-sweeps of every round.  Block order is
+// remapped so neighbouring tiles share an XCD's L2.  A generic x-sweep kernel
+// covers lattices too large for 32-bit lane offsets.
 #include "rhz_device.hpp"
 #include "rhz_internal.hpp"
 
@@ -517,6 +519,165 @@ __global__ __launch_bounds__(256) void sweep_tile_kernel(const float* __restrict
     if (__ballot(chg) != 0 && threadIdx.x == 0) atomicOr(changed, 1);
 }
 
+// ---- y-sweep on the y-fastest copy (the walk runs along the contiguous axis) ----
+// Layout y-fastest: cell (x, y, z) at z * plane + x * pitch + y, rows padded to
+// a multiple of 32 floats (128 B).  One wave = 64 consecutive x (one y-row per
+// lane) of one z; a block = 4 consecutive z.  Each lane walks its row in
+// chunks of CH cells held in registers: its own row, the rows at z - 1 and
+// z + 1 (the opposite one at a face: fminf(v, v) = fminf(1e30, v) for every
+// stored value), and for lanes 0 and 63 the row just outside the wave in x.
+// The x neighbours of a cell are the adjacent lanes' cells of the same step
+// (DPP wave shifts), the y neighbours the new value behind the walker and the
+// old value ahead (the first cell of the next chunk read ahead), so the
+// update reads exactly what the pinned schedule prescribes: results equal
+// the y-sweep on the x-fastest layout bit for bit.  Chunk indices are static:
+// the one ragged chunk of a row (at its far end) guards each step.
+template <int DIR, int CH>
+__device__ __forceinline__ void run_chunk(float (&o)[CH], const float (&m)[CH], const float (&p)[CH],
+                                          const float (&e)[CH], float look, int nvalid, bool xlo_face,
+                                          bool xhi_face, float& behind, bool& chg, const SweepConst& K) {
+#pragma unroll
+    for (int t = 0; t < CH; ++t) {
+        const int k = DIR > 0 ? t : CH - 1 - t;
+        if (k < nvalid) {  // wave-uniform
+            const float cur = o[k];
+            float next;
+            if (DIR > 0)
+                next = (k + 1 < nvalid) ? o[k + 1 < CH ? k + 1 : k] : look;
+            else
+                next = (k > 0) ? o[k > 0 ? k - 1 : 0] : look;
+            const float l = wave_shr1(e[k], cur);  // lane 0 keeps its edge row
+            const float r = wave_shl1(e[k], cur);  // lane 63 keeps its edge row
+            const float fm = xlo_face ? r : l;
+            const float fp = xhi_face ? l : r;
+            // branch-free update (x, y, z slots in order): one chain per lane
+            // leaves no room for a wave-uniform branch every step
+            const float nv = godunov_select(cur, fminf(fm, fp), fminf(behind, next), fminf(m[k], p[k]), K);
+            chg |= nv < cur;
+            o[k] = nv;
+            behind = nv;
+        }
+    }
+}
+
+// A wave's rows move between global memory and registers through its own LDS
+// tile (64 rows x CH cells, one padding float per row: a lane's row and a
+// column of 8-row groups are both conflict-free).  Global accesses are then
+// coalesced: instruction i of a wave covers rows 8 i .. 8 i + 7, CH / 8 cells
+// of each per lane group.  The wave-scope fences keep the compiler from
+// moving a lane's LDS read of another lane's write across them.
+template <int CH>
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// rows x0 + r (r = 0..63, clamped to the last row) of `base` (row r at
+// base + r * pitch), cells [y0, y0 + CH), into v[] of lane r
+template <int CH>
+__device__ __forceinline__ void stage_in(float (&v)[CH], float* __restrict__ tile, const float* __restrict__ base,
+                                         int64_t pitch, int rmax, int y0) {
+    constexpr int Q = CH / 4;      // float4 per row
+    constexpr int RPI = 64 / Q;    // rows per instruction
+    const int lane = threadIdx.x;
+    const int rr = lane / Q, qq = lane % Q;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const int r = i * RPI + rr;
+        const float4 a = reinterpret_cast<const float4*>(base + (int64_t)min(r, rmax) * pitch + y0)[qq];
+        float* t = tile + r * (CH + 1) + 4 * qq;
+        t[0] = a.x;
+        t[1] = a.y;
+        t[2] = a.z;
+        t[3] = a.w;
+    }
+    wave_lds_sync<CH>();
+#pragma unroll
+    for (int k = 0; k < CH; ++k) v[k] = tile[lane * (CH + 1) + k];
+    wave_lds_sync<CH>();
+}
+
+// v[] of lane r to rows x0 + r (r < nlive) of `base`, cells [y0, y0 + CH)
+template <int CH>
+__device__ __forceinline__ void stage_out(const float (&v)[CH], float* __restrict__ tile, float* __restrict__ base,
+                                          int64_t pitch, int nlive, int y0) {
+    constexpr int Q = CH / 4;
+    constexpr int RPI = 64 / Q;
+    const int lane = threadIdx.x;
+    const int rr = lane / Q, qq = lane % Q;
+#pragma unroll
+    for (int k = 0; k < CH; ++k) tile[lane * (CH + 1) + k] = v[k];
+    wave_lds_sync<CH>();
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const int r = i * RPI + rr;
+        const float* t = tile + r * (CH + 1) + 4 * qq;
+        if (r < nlive) reinterpret_cast<float4*>(base + (int64_t)r * pitch + y0)[qq] = make_float4(t[0], t[1], t[2], t[3]);
+    }
+    wave_lds_sync<CH>();
+}
+
+template <int DIR, int CH>
+__global__ __launch_bounds__(256) void sweep_run_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                                        int nx, int ny, int nz, int64_t pitch, int64_t plane,
+                                                        int nbx, unsigned nblocks, float h,
+                                                        int* __restrict__ changed) {
+    __shared__ float tiles[4][64 * (CH + 1)];
+    const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
+    const int bx = (int)(lb % (unsigned)nbx);
+    const int z = __builtin_amdgcn_readfirstlane((int)(lb / (unsigned)nbx) * 4 + (int)threadIdx.y);
+    if (z >= nz) return;
+    float* tile = tiles[threadIdx.y];
+    const int lane = threadIdx.x;
+    const int x0 = bx * 64;
+    const int x = x0 + lane;
+    const int nlive = min(64, nx - x0);  // rows of this wave inside the lattice
+    const int rmax = nlive - 1;
+    const int xl = x < nx ? x : nx - 1;
+    int xe = xl;
+    if (lane == 0 && xl > 0) xe = xl - 1;
+    if (lane == 63 && xl < nx - 1) xe = xl + 1;
+    const bool xlo_face = x == 0, xhi_face = x >= nx - 1;
+    const int64_t zo = (int64_t)z * plane;
+    const float* own = src + zo + (int64_t)x0 * pitch;  // row r of the wave at own + r * pitch
+    const float* zm = own + (z > 0 ? -plane : plane);
+    const float* zp = own + (z < nz - 1 ? plane : -plane);
+    const float* ed = src + zo + (int64_t)xe * pitch;
+    const float* mine = src + zo + (int64_t)xl * pitch;
+    float* out = dst + zo + (int64_t)x0 * pitch;
+    const bool edge_lane = lane == 0 || lane == 63;
+    const SweepConst K = sweep_const(h);
+    float behind = kFar;
+    bool chg = false;
+    const int nch = (ny + CH - 1) / CH;
+    for (int c = 0; c < nch; ++c) {
+        const int ci = DIR > 0 ? c : nch - 1 - c;
+        const int y0 = ci * CH;
+        const int nvalid = min(CH, ny - y0);
+        float o[CH], m[CH], p[CH], e[CH];
+        stage_in<CH>(o, tile, own, pitch, rmax, y0);
+        stage_in<CH>(m, tile, zm, pitch, rmax, y0);
+        stage_in<CH>(p, tile, zp, pitch, rmax, y0);
+        if (edge_lane) {
+#pragma unroll
+            for (int q = 0; q < CH / 4; ++q) {
+                const float4 a = reinterpret_cast<const float4*>(ed + y0)[q];
+                e[4 * q] = a.x; e[4 * q + 1] = a.y; e[4 * q + 2] = a.z; e[4 * q + 3] = a.w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < CH; ++q) e[q] = 0.0f;
+        }
+        // the first cell past the chunk along the walk (1e30 past the row's end)
+        const int yl = DIR > 0 ? y0 + CH : y0 - 1;
+        const float look = (yl >= 0 && yl < ny) ? mine[yl] : kFar;
+        run_chunk<DIR, CH>(o, m, p, e, look, nvalid, xlo_face, xhi_face, behind, chg, K);
+        stage_out<CH>(o, tile, out, pitch, nlive, y0);
+    }
+    if (__ballot(chg) != 0 && threadIdx.x == 0) atomicOr(changed, 1);
+}
+
 // The job (lattice) that logical block lb belongs to: starts[j] <= lb <
 // starts[j + 1] (starts[0] = 0, starts[njobs] = the grid size).
 __device__ __forceinline__ int batch_job(const unsigned* __restrict__ starts, int njobs, unsigned lb) {
@@ -797,7 +958,7 @@ __global__ void redblack_kernel(float* __restrict__ g, int dx, int dy, int dz, f
 
 // Launch shape of the lane sweeps (steps per load group, waves per block);
 // set by the tuning entry point, read at every launch.
-static SweepTuning g_sweep_tuning = {2, 4, 0, {14, 21, 21}, 16ll << 20, 0};
+static SweepTuning g_sweep_tuning = {2, 4, 0, {14, 21, 21}, 16ll << 20, 0, {}, {}, {}, {}, 32, 1};
 SweepTuning sweep_tuning() { return g_sweep_tuning; }
 bool set_sweep_tuning(int step, int rows, int fmajor) {
     if ((step != 2 && step != 4 && step != 8) || (rows != 4 && rows != 8 && rows != 16) ||
@@ -826,6 +987,16 @@ int effective_shape(int axis, int64_t voxels) {
 int effective_step(int64_t voxels) {
     const int v = g_sweep_tuning.cls_step[shape_class(voxels)];
     return v ? v : g_sweep_tuning.step;
+}
+bool set_run_chunk(int cells) {
+    if (cells != 16 && cells != 32) return false;
+    g_sweep_tuning.run_chunk = cells;
+    return true;
+}
+bool set_fsm_layout(int layout) {
+    if (layout != 0 && layout != 1) return false;
+    g_sweep_tuning.fsm_layout = layout;
+    return true;
 }
 bool set_class_shape(int cls, int axis, int shape) {
     if (cls < 0 || cls >= kShapeClasses || axis < 0 || axis > 2 ||
@@ -989,6 +1160,31 @@ hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir,
 
 hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir, int* changed, hipStream_t s) {
     return launch_sweep_yx(src, dst, g, dir, changed, s, dense_yx(g));
+}
+
+namespace {
+template <int DIR, int CH>
+hipError_t launch_run(const float* src, float* dst, const Grid& g, const FieldLayout& L, int* changed, hipStream_t s) {
+    const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+    const int nbx = (nx + 63) / 64;
+    const int64_t blocks = (int64_t)nbx * ((nz + 3) / 4);
+    sweep_run_kernel<DIR, CH><<<(unsigned)blocks, dim3(64, 4), 0, s>>>(src, dst, nx, ny, nz, L.pitch, L.plane, nbx,
+                                                                       (unsigned)blocks, g.h, changed);
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_sweep_on_yx(const float* src, float* dst, const Grid& g, int axis, int dir, int* changed,
+                              hipStream_t s, const FieldLayout& L) {
+    const int dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
+    if (axis == 0) return launch_sweep_yx(src, dst, g, dir, changed, s, L);
+    if (axis == 2)  // lanes y (slot 1, contiguous), walk z (slot 2, plane), cross x (slot 0, row pitch)
+        return launch_lane<1, 2, 0>(src, dst, dy, dz, dx, L.plane, L.pitch, g.h, dir, changed, s);
+    // axis 1: the walk runs along the rows, which must be padded to 32 floats
+    if (L.pitch % 32 != 0 || !lane_sweeps_ok(g)) return hipErrorInvalidValue;
+    if (sweep_tuning().run_chunk == 16)
+        return dir > 0 ? launch_run<1, 16>(src, dst, g, L, changed, s) : launch_run<-1, 16>(src, dst, g, L, changed, s);
+    return dir > 0 ? launch_run<1, 32>(src, dst, g, L, changed, s) : launch_run<-1, 32>(src, dst, g, L, changed, s);
 }
 
 hipError_t launch_sweep(const float* src, float* dst, const Grid& g, int axis, int dir, int* changed,

@@ -128,6 +128,9 @@ struct SweepTuning {
     // share the GPU with the sign stages); 0 = the class-free value.
     int bcls_shape[4][3];
     int bcls_step[4];
+    int run_chunk;   // cells per register chunk of the y-sweep on the y-fastest copy (16, 32)
+    int fsm_layout;  // 0: x-sweeps on a y-fastest copy, the rest x-fastest (two transposes
+                     // a round); 1: every sweep on the y-fastest copy (no transposes between rounds)
 };
 // Lattice-size classes of the per-class sweep tuning: <= 2^21, <= 2^24,
 // <= 2^27 voxels, larger.
@@ -143,6 +146,8 @@ bool set_sweep_shape(int axis, int shape);
 // shape / step 0 clears the class's own value.
 bool set_class_shape(int cls, int axis, int shape);
 bool set_class_step(int cls, int step);
+bool set_run_chunk(int cells);
+bool set_fsm_layout(int layout);
 bool set_batch_class_shape(int cls, int axis, int shape);
 bool set_batch_class_step(int cls, int step);
 void set_fsm_batching(int64_t max_voxels, int graph);
@@ -155,6 +160,11 @@ hipError_t launch_sweep_octant(float* g, const Grid& gr, int sx, int sy, int sz,
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s);
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s,
                                const FieldLayout& lin, const FieldLayout& lout);
+// Any sweep on the y-fastest layout L (rows padded to 32 floats for axis 1):
+// x as launch_sweep_yx, y walking along the rows (sweep_run_kernel), z with
+// lanes over y; results equal the x-fastest sweeps bit for bit.
+hipError_t launch_sweep_on_yx(const float* src, float* dst, const Grid& g, int axis, int dir, int* changed,
+                              hipStream_t s, const FieldLayout& L);
 // x-sweep on the y-fastest layout (src/dst both y-fastest; dense, or layout L).
 hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir, int* changed, hipStream_t s);
 hipError_t launch_sweep_yx(const float* src, float* dst, const Grid& g, int dir, int* changed, hipStream_t s,

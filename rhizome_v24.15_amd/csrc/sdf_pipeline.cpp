@@ -495,6 +495,31 @@ int sweep_round_p(Ctx& c, const float* in, const FieldLayout& lin, float* a, flo
     return RHZ_OK;
 }
 
+// One round with every sweep on the y-fastest copy: the field (in, layout lin)
+// is transposed into a (layout ly, rows padded to 32 floats) unless it is
+// there already (in_y); then x+, x-, y+, y-, z+, z- alternate a and b and the
+// round ends in a, still y-fastest.  No transposes between rounds.
+int sweep_round_y(Ctx& c, const float* in, const FieldLayout& lin, bool in_y, float* a, float* b,
+                  const FieldLayout& ly, int* changed) {
+    if (!in_y) {
+        if (c.trans_t) c.trans_t->begin(c.s);
+        HIP_TRY(launch_transpose_xy(in, a, c.g, true, c.s, lin, ly));
+        if (c.trans_t) c.trans_t->end(c.s);
+    }
+    const int axes[6] = {0, 0, 1, 1, 2, 2};
+    const int dirs[6] = {+1, -1, +1, -1, +1, -1};
+    float* src = a;
+    float* dst = b;
+    for (int k = 0; k < 6; ++k) {
+        if (c.sweeps_t) c.sweeps_t->begin(c.s);
+        HIP_TRY(launch_sweep_on_yx(src, dst, c.g, axes[k], dirs[k], changed, c.s, ly));
+        if (c.sweeps_t) c.sweeps_t->end(c.s);
+        std::swap(src, dst);
+    }
+    c.st->sweeps += 6;
+    return RHZ_OK;
+}
+
 // One round on a dense field in a (b: the other buffer); the round ends in a.
 int sweep_round(Ctx& c, float* a, float* b, int* changed) {
     if (lane_sweeps_ok(c.g)) return sweep_round_p(c, a, dense_xy(c.g), a, b, dense_xy(c.g), dense_yx(c.g), changed);
@@ -657,13 +682,35 @@ int run_fsm_rounds(Ctx& c, int64_t* far_out, FsmField* keep = nullptr) {
     if ((!octant && !q) || !pbuf) return set_err(RHZ_ERR_NOMEM, "sweep buffer allocation failed");
     const float* cur = c.d;  // the field: the caller's dense buffer until a round has run
     FieldLayout cur_l = dense;
+    // every sweep on the y-fastest copy (tuning "fsm_layout" 1): the field stays
+    // y-fastest from the first round to the end of the rounds
+    const bool ylay = lanes && padded && sweep_tuning().fsm_layout == 1 && ly.pitch % 32 == 0;
+    bool cur_y = false;
     auto round = [&](int* flag) -> int {
         if (octant) return octant_round(c, flag);
         if (!lanes) return sweep_round(c, c.d, q, flag);
+        if (ylay) {
+            const int rc = sweep_round_y(c, cur, cur_l, cur_y, pbuf, q, ly, flag);
+            cur = pbuf;
+            cur_l = ly;
+            cur_y = true;
+            return rc;
+        }
         const int rc = sweep_round_p(c, cur, cur_l, pbuf, q, lx, ly, flag);
         cur = pbuf;
         cur_l = lx;
         return rc;
+    };
+    // sentinel count of the current field (rows along y on the y-fastest copy)
+    auto audit_cur = [&](int64_t* far) -> int {
+        if (!cur_y) return audit(c, far, cur, &cur_l);
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FAR), 0, sizeof(unsigned long long), c.s));
+        HIP_TRY(launch_count_far_rows(cur, cur_l.pitch, cur_l.plane, c.g.dims[1], c.g.dims[0], c.g.dims[2],
+                                      cnt_ptr(c.w, C_FAR), c.s));
+        const int rc = read_counters(c);
+        if (rc) return rc;
+        *far = (int64_t)hc(c, C_FAR);
+        return RHZ_OK;
     };
     int* changed = flag_ptr(c.w, F_CHANGED);
     const bool converge = c.p->fsm_policy == RHZ_FSM_CONVERGE || octant;
@@ -690,17 +737,27 @@ int run_fsm_rounds(Ctx& c, int64_t* far_out, FsmField* keep = nullptr) {
         if (hf(c, F_CHANGED) == 0) break;
     }
     int64_t far = 0;
-    int rc = audit(c, &far, cur, &cur_l);
+    int rc = audit_cur(&far);
     if (rc) return rc;
     int extra = 0;
     while (far > 0 && extra < c.p->max_extra_rounds) {
         rc = round(changed);
         if (rc) return rc;
         ++extra;
-        rc = audit(c, &far, cur, &cur_l);
+        rc = audit_cur(&far);
         if (rc) return rc;
     }
-    if (keep) {
+    if (cur_y) {
+        // back to the x-fastest layout: into q for the fused write-back, or
+        // straight into the caller's field
+        if (c.trans_t) c.trans_t->begin(c.s);
+        if (keep)
+            HIP_TRY(launch_transpose_xy(cur, q, c.g, false, c.s, ly, lx));
+        else
+            HIP_TRY(launch_transpose_xy(cur, c.d, c.g, false, c.s, ly, dense));
+        if (c.trans_t) c.trans_t->end(c.s);
+        if (keep) *keep = FsmField{q, lx};
+    } else if (keep) {
         *keep = FsmField{cur, cur_l};
     } else if (cur != c.d) {
         HIP_TRY(launch_copy_rows(cur, cur_l.pitch, cur_l.plane, c.d, dense.pitch, dense.plane, (int)dx, c.g.dims[1],
@@ -1932,6 +1989,14 @@ int rhz_tuning_set(const char* key, int64_t value) {
         set_fsm_batching(t.batch_rounds_max_voxels, (int)value);
         return RHZ_OK;
     }
+    if (!std::strcmp(key, "fsm_layout")) {
+        if (!set_fsm_layout((int)value)) return set_err(RHZ_ERR_ARG, "fsm_layout must be 0 or 1");
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "sweep_run_chunk")) {
+        if (!set_run_chunk((int)value)) return set_err(RHZ_ERR_ARG, "sweep_run_chunk must be 16 or 32");
+        return RHZ_OK;
+    }
     if (!std::strcmp(key, "verify_overlap")) {
         if (value != 0 && value != 1) return set_err(RHZ_ERR_ARG, "verify_overlap must be 0 or 1");
         g_verify_overlap.store((int)value, std::memory_order_relaxed);
@@ -1982,6 +2047,14 @@ int rhz_tuning_get(const char* key, int64_t* value) {
     }
     if (!std::strcmp(key, "fsm_graph")) {
         *value = t.fsm_graph;
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "fsm_layout")) {
+        *value = t.fsm_layout;
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "sweep_run_chunk")) {
+        *value = t.run_chunk;
         return RHZ_OK;
     }
     if (!std::strcmp(key, "verify_overlap")) {
