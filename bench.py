@@ -467,8 +467,9 @@ def main():
         per_ms = ms_sweeps / sweep_launches
         achieved = FSM_BYTES_PER_VOXEL * V / (per_ms * 1e-3) / 1e9
         layout = rhz.tuning_get("fsm_layout")
-        kname = ("one directional sweep (sweep_tile_kernel for x and z, sweep_run_kernel for y, on the y-fastest "
-                 "copy)" if layout == 1 else "one directional sweep (sweep_tile_kernel)")
+        ykern = "sweep_runp_kernel" if rhz.tuning_get("sweep_run_block") else "sweep_run_kernel"
+        kname = (f"one directional sweep (sweep_tile_kernel for x and z, {ykern} for y, on the y-fastest copy)"
+                 if layout == 1 else "one directional sweep (sweep_tile_kernel)")
         rl_fsm = {"bound": "hbm", "kernel": kname, "achieved": achieved,
                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                   "avg_launch_ms": per_ms, "launches": sweep_launches,

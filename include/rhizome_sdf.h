@@ -191,9 +191,9 @@ int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
  *   "sweep_shape_x" / "_y" / "_z"  tile each lane of an x-, y- or z-sweep walks:
  *                                  1 = one line (rows / fmajor apply), or
  *                                  10 NF + NC = NF fast-axis segments x NC cross
- *                                  lines: 21, 22, 12, 14 (defaults 14, 21, 21);
+ *                                  lines: 21, 22, 12, 14 (defaults 14, 21, 12);
  *   "sweep_step"                   2, 4, 8: steps whose loads a sweep wave issues
- *                                  together (default 2);
+ *                                  together (default 4);
  *   "sweep_rows", "sweep_fmajor"   waves per block (4, 8, 16) and 1: a block's
  *                                  waves take adjacent segments of one row, for
  *                                  the one-line walker;
@@ -202,8 +202,13 @@ int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
  *                                  first round and one after the last; 0: the x-
  *                                  sweeps on a y-fastest copy, the others x
  *                                  fastest, two transposes every round;
+ *   "sweep_run_block"              the y-sweep walking along the y-fastest rows:
+ *                                  41 (default) = the prefetching kernel, 4 waves
+ *                                  of one plane each per block (planes staged
+ *                                  once per block, next chunk loaded while one is
+ *                                  computed); 0 = the one-wave-per-plane kernel;
  *   "sweep_run_chunk"              16, 32 (default): cells per register chunk of
- *                                  the y-sweep walking along the y-fastest rows;
+ *                                  that one-wave-per-plane kernel;
  *   "fsm_batch_rounds_max_voxels"  lattices up to this size enqueue all FSM
  *                                  rounds without host round trips;
  *   "fsm_graph"                    1: those rounds replay as one captured graph;

@@ -300,18 +300,19 @@ __device__ __forceinline__ float lane_step(float cur, float fm, float fp, float 
 // (disc = RN(a - b) for floats a ~ 2h^2 or 3h^2 and b >= 0: either b <= a/2
 // and disc >= a/2, or a - b is exact and a multiple of b's unit in the last
 // place, > b 2^-24 > a 2^-25.)  A zero discriminant gives root 0; a negative
-// one is never selected.
+// one is never selected (whatever the short form makes of it is discarded).
 __device__ __forceinline__ float sqrt_disc(float disc) {
-    const bool pos = disc > 0.0f;
-    const float r = sqrt_rn_normal(pos ? disc : 1.0f);
-    return pos ? r : 0.0f;
+    const float r = sqrt_rn_normal(disc);
+    return disc > 0.0f ? r : 0.0f;
 }
 
 // godunov_min with every candidate root computed and then selected: the roots
 // the branchy form would not reach are discarded by the selects, and where its
 // early exit returns cur the full formula returns cur too, so the results are
-// identical bit for bit.
-__device__ __forceinline__ float godunov_select(float cur, float a0, float a1, float a2, const SweepConst& K) {
+// identical bit for bit.  godunov_cand is the candidate before the final
+// min with cur: callers that track changes take u < cur once, for the select
+// and for a wave mask (a scalar OR per step instead of two vector ops).
+__device__ __forceinline__ float godunov_cand(float a0, float a1, float a2, const SweepConst& K) {
     const float s0 = fminf(a0, fminf(a1, a2));
     const float s2 = fmaxf(a0, fmaxf(a1, a2));
     const float s1 = a0 + a1 + a2 - s0 - s2;
@@ -324,7 +325,10 @@ __device__ __forceinline__ float godunov_select(float cur, float a0, float a1, f
     const float u3 = div3(s0 + s1 + s2 + sqrt_disc(disc3));
     const bool two = (u1 > s1) && (disc >= 0.0f);
     const bool three = two && (u2 > s2) && (disc3 >= 0.0f);
-    const float u = three ? u3 : (two ? u2 : u1);
+    return three ? u3 : (two ? u2 : u1);
+}
+__device__ __forceinline__ float godunov_select(float cur, float a0, float a1, float a2, const SweepConst& K) {
+    const float u = godunov_cand(a0, a1, a2, K);
     return (u < cur) ? u : cur;
 }
 
@@ -391,7 +395,7 @@ __device__ __forceinline__ bool tile_walk(const float* __restrict__ src, float* 
     const int64_t step = (int64_t)DIR * ws;
     int64_t row = (int64_t)c0 * cs + (int64_t)((DIR > 0) ? 0 : nw - 1) * ws;
     float behind[NC][NF], cur[NC][NF], edge[NC], hlo[NF], hhi[NF];
-    bool chg = false;
+    unsigned long long chgm = 0;  // lanes that lowered a cell
     {
         const __amdgpu_buffer_rsrc_t r0 = rsrc_at(src + row - A, nrec);
         const uint32_t back = (uint32_t)(step * 4);
@@ -438,14 +442,16 @@ __device__ __forceinline__ bool tile_walk(const float* __restrict__ src, float* 
                 g[SF] = fminf(fm, fp);
                 g[SW] = fminf(behind[i][j], nx[i][j]);
                 g[SC] = fminf(cm, cp);
-                nv[i][j] = godunov_select(cur[i][j], g[0], g[1], g[2], K);
+                const float u = godunov_cand(g[0], g[1], g[2], K);
+                const bool lower = u < cur[i][j];
+                nv[i][j] = lower ? u : cur[i][j];
+                chgm |= __ballot(lower);
             }
         }
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
 #pragma unroll
             for (int j = 0; j < NF; ++j) {
-                chg |= nv[i][j] < cur[i][j];
                 bstore(w, v_st[i][j], nv[i][j]);
                 behind[i][j] = nv[i][j];
                 cur[i][j] = nx[i][j];
@@ -499,7 +505,7 @@ __device__ __forceinline__ bool tile_walk(const float* __restrict__ src, float* 
         }
         advance(nx, ed, lo, hi);
     }
-    return chg;
+    return chgm != 0;
 }
 
 // Block = 4 waves; wave (bf, g) takes fast-axis cells [64 NF bf, 64 NF (bf + 1))
@@ -532,6 +538,19 @@ __global__ __launch_bounds__(256) void sweep_tile_kernel(const float* __restrict
 // update reads exactly what the pinned schedule prescribes: results equal
 // the y-sweep on the x-fastest layout bit for bit.  Chunk indices are static:
 // the one ragged chunk of a row (at its far end) guards each step.
+// One cell of the walk: x neighbours from the adjacent lanes (DPP; lane 0 and
+// lane 63 keep their edge-row value e), y neighbours behind / next, z
+// neighbours m / p.  Branch-free (x, y, z slots in order): one chain per lane
+// leaves no room for a wave-uniform branch every step.
+__device__ __forceinline__ float run_cell(float cur, float mp, float e, float behind, float next, bool xlo_face,
+                                          bool xhi_face, const SweepConst& K) {
+    const float l = wave_shr1(e, cur);
+    const float r = wave_shl1(e, cur);
+    const float fm = xlo_face ? r : l;
+    const float fp = xhi_face ? l : r;
+    return godunov_select(cur, fminf(fm, fp), fminf(behind, next), mp, K);
+}
+
 template <int DIR, int CH>
 __device__ __forceinline__ void run_chunk(float (&o)[CH], const float (&m)[CH], const float (&p)[CH],
                                           const float (&e)[CH], float look, int nvalid, bool xlo_face,
@@ -546,13 +565,7 @@ __device__ __forceinline__ void run_chunk(float (&o)[CH], const float (&m)[CH], 
                 next = (k + 1 < nvalid) ? o[k + 1 < CH ? k + 1 : k] : look;
             else
                 next = (k > 0) ? o[k > 0 ? k - 1 : 0] : look;
-            const float l = wave_shr1(e[k], cur);  // lane 0 keeps its edge row
-            const float r = wave_shl1(e[k], cur);  // lane 63 keeps its edge row
-            const float fm = xlo_face ? r : l;
-            const float fp = xhi_face ? l : r;
-            // branch-free update (x, y, z slots in order): one chain per lane
-            // leaves no room for a wave-uniform branch every step
-            const float nv = godunov_select(cur, fminf(fm, fp), fminf(behind, next), fminf(m[k], p[k]), K);
+            const float nv = run_cell(cur, fminf(m[k], p[k]), e[k], behind, next, xlo_face, xhi_face, K);
             chg |= nv < cur;
             o[k] = nv;
             behind = nv;
@@ -677,6 +690,266 @@ __global__ __launch_bounds__(256) void sweep_run_kernel(const float* __restrict_
     }
     if (__ballot(chg) != 0 && threadIdx.x == 0) atomicOr(changed, 1);
 }
+
+// ---- y-sweep on the y-fastest copy, prefetching ----------------------------------
+// Same walk and arithmetic as sweep_run_kernel (so the same results bit for
+// bit), organised so that each plane's rows are fetched once per block and
+// no load is waited for inside a chunk:
+//  * every lane walks C rows at once, one in each of C adjacent planes;
+//  * a block is W waves over the planes z0 .. z0 + W C - 1 and 64 rows: the
+//    wave's 62 rows x0 .. x0 + 61 on lanes 1 .. 62, and on lanes 0 and 63 the
+//    rows just outside them, walked only to hand their old values to lanes 1
+//    and 62 (DPP) and never stored;
+//  * per chunk of CH cells, the rows of the W C + 2 planes z0 - 1 .. z0 + W C
+//    are staged once in LDS and shared (a row's z - 1 and z + 1 neighbours are
+//    the rows of the adjacent planes): sweep_run_kernel's waves each fetched
+//    their own z - 1 and z + 1 rows, 2.2x the field's bytes per sweep past L2;
+//  * the chunk after next is loaded into registers (coalesced float4, four
+//    lanes per 64-byte row piece) while a chunk is computed, and written into
+//    LDS one chunk later; a chunk's last cell along the walk, whose `next` is
+//    the first cell of the following chunk, is finished once that chunk is in
+//    registers, so no wait falls inside a chunk;
+//  * results go to memory through a per-plane LDS tile, so that a store
+//    instruction writes whole 16-byte-aligned row pieces (a lane storing its
+//    own row's pieces straight from registers measured 40% slower);
+//  * LDS rows are padded to CH + 4 floats, so 16-byte reads of a lane's own
+//    row and 16-byte writes of row pieces touch distinct banks.
+// Measured on the 883 x 823 x 923 lattice and not adopted: direct global->LDS
+// loads (global_load_lds_dwordx4; a lane's 16 bytes became a request of its
+// own), two planes per wave (C = 2: about 200 VGPRs, two waves per SIMD), and
+// whole 128-byte lines per row piece (32-cell chunks, stores straight from
+// registers: 3.2-3.7 GB read per sweep instead of 7.0, but 214-256 VGPRs and a
+// slower sweep).  The kernel is only marginally faster than sweep_run_kernel:
+// both are bound by the single dependency chain per lane at the occupancy
+// that LDS and registers allow.
+// Rows and planes past the lattice edge are loaded clamped into the lattice
+// and never stored; only stored cells count as changed.
+// The minimum of a cell's two x neighbours in the prefetching kernel, whose
+// lanes 0 and 63 are never stored: their DPP sources may be anything.  FACE
+// (the wave holds row 0 or row nx - 1): a missing neighbour is replaced by the
+// opposite one.
+template <bool FACE>
+__device__ __forceinline__ float run_xmin(float cur, bool xlo_face, bool xhi_face) {
+    if (FACE) {
+        const float l = wave_shr1(cur, cur);
+        const float r = wave_shl1(cur, cur);
+        return fminf(xlo_face ? r : l, xhi_face ? l : r);
+    }
+    const float l = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(cur), 0x138, 0xf, 0xf, true));
+    const float r = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(cur), 0x130, 0xf, 0xf, true));
+    return fminf(l, r);
+}
+
+// One cell of the prefetching kernel's walk; lanes that lower it go into chgm.
+template <bool FACE>
+__device__ __forceinline__ float run_cellm(float cur, float mp, float behind, float next, bool xlo_face,
+                                           bool xhi_face, unsigned long long& chgm, const SweepConst& K) {
+    const float u = godunov_cand(run_xmin<FACE>(cur, xlo_face, xhi_face), fminf(behind, next), mp, K);
+    const bool lower = u < cur;
+    chgm |= __ballot(lower);
+    return lower ? u : cur;
+}
+
+// C chains per lane: a chunk of C rows at once, steps interleaved.  FULL: a
+// whole chunk (no guard per step); SKIPLAST: the chunk's last cell along the
+// walk is left for the caller (its `next` is in the following chunk).
+template <int DIR, int CH, int C, bool FULL, bool SKIPLAST, bool FACE>
+__device__ __forceinline__ void run_chunks(float (&o)[C][CH], const float (&mp)[C][CH], int nvalid_, bool xlo_face,
+                                           bool xhi_face, float (&behind)[C], unsigned long long (&chgm)[C],
+                                           const SweepConst& K) {
+    const int nvalid = FULL ? CH : nvalid_;
+#pragma unroll
+    for (int t = 0; t < (SKIPLAST ? CH - 1 : CH); ++t) {
+        const int k = DIR > 0 ? t : CH - 1 - t;
+        if (k < nvalid) {  // wave-uniform
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                float next;
+                if (DIR > 0)
+                    next = (k + 1 < nvalid) ? o[c][k + 1 < CH ? k + 1 : k] : kFar;
+                else
+                    next = (k > 0) ? o[c][k > 0 ? k - 1 : 0] : kFar;
+                const float nv = run_cellm<FACE>(o[c][k], mp[c][k], behind[c], next, xlo_face, xhi_face, chgm[c], K);
+                o[c][k] = nv;
+                behind[c] = nv;
+            }
+        }
+    }
+}
+
+// A chunk of the prefetching kernel: the walk to the row's end (!more) or to
+// the chunk's last cell; only a y-descending sweep starts on the ragged chunk,
+// and its last cell (k = 0) is whole.
+template <int DIR, int CH, int C, bool FACE>
+__device__ __forceinline__ void run_chunk_any(float (&o)[C][CH], const float (&mp)[C][CH], int nvalid, bool more,
+                                              bool xlo_face, bool xhi_face, float (&behind)[C],
+                                              unsigned long long (&chgm)[C], const SweepConst& K) {
+    if (!more) {
+        if (nvalid == CH) run_chunks<DIR, CH, C, true, false, FACE>(o, mp, CH, xlo_face, xhi_face, behind, chgm, K);
+        else run_chunks<DIR, CH, C, false, false, FACE>(o, mp, nvalid, xlo_face, xhi_face, behind, chgm, K);
+    } else {
+        if (nvalid == CH) run_chunks<DIR, CH, C, true, true, FACE>(o, mp, CH, xlo_face, xhi_face, behind, chgm, K);
+        else run_chunks<DIR, CH, C, false, true, FACE>(o, mp, nvalid, xlo_face, xhi_face, behind, chgm, K);
+    }
+}
+
+template <int DIR, int CH, int W, int C>
+__global__ __launch_bounds__(64 * W) void sweep_runp_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                                            int nx, int ny, int nz, int64_t pitch, int64_t plane,
+                                                            int nbx, unsigned nblocks, float h,
+                                                            int* __restrict__ changed) {
+    constexpr int Q = CH / 4;     // float4 per row piece
+    constexpr int RPI = 64 / Q;   // rows per load instruction
+    constexpr int RS = CH + 4;    // LDS row stride (floats)
+    constexpr int P = W * C;      // planes per block
+    constexpr int NS = P + 2;     // staged planes z0 - 1 .. z0 + P
+    constexpr int NL = NS * Q;    // load instructions per chunk (Q per plane)
+    constexpr int NLW = NL / W;   // ... per wave
+    static_assert(NL % W == 0, "the waves share the loads evenly");
+    constexpr int PK = DIR > 0 ? CH - 1 : 0;  // a chunk's last cell along the walk
+    __shared__ __align__(16) float in_t[NS][64 * RS];
+    __shared__ __align__(16) float out_t[P][64 * RS];
+    const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
+    const int bx = (int)(lb % (unsigned)nbx);
+    const int z0 = (int)(lb / (unsigned)nbx) * P;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int lane = (int)threadIdx.x;
+    const int x0 = bx * 62;
+    const int x = x0 - 1 + lane;  // lanes 0 and 63: the rows beside the wave's 62
+    const bool xlo_face = x == 0, xhi_face = x == nx - 1;
+    const bool mine = lane >= 1 && lane <= 62 && x < nx;
+    const bool face = x0 == 0 || x0 + 62 >= nx;  // a stored lane holds row 0 or row nx - 1
+    bool plive[C];
+    int os[C], ms[C], ps[C];
+    float* out[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int z = z0 + w * C + c;
+        const int zc = z < nz ? z : nz - 1;
+        plive[c] = z < nz;
+        os[c] = w * C + c + 1;                         // this plane's staged rows
+        ms[c] = zc > 0 ? os[c] - 1 : os[c] + 1;        // z - 1 (the opposite plane at a face)
+        ps[c] = zc < nz - 1 ? os[c] + 1 : os[c] - 1;   // z + 1
+        out[c] = dst + (int64_t)zc * plane;
+    }
+    // load instruction t of this wave: flattened j = w + t W = s Q + i (plane s,
+    // instruction i); lane -> row i RPI + lane / Q, quad lane % Q
+    const int lr = lane / Q, lq = lane % Q;
+    const float* lsrc[NLW];
+    int lofs[NLW];
+#pragma unroll
+    for (int t = 0; t < NLW; ++t) {
+        const int j = w + t * W;
+        const int sp = j / Q, i = j % Q;
+        const int zs = min(max(z0 - 1 + sp, 0), nz - 1);
+        const int r = i * RPI + lr;
+        const int xr = min(max(x0 - 1 + r, 0), nx - 1);
+        lsrc[t] = src + (int64_t)zs * plane + (int64_t)xr * pitch + 4 * lq;
+        lofs[t] = sp * 64 * RS + r * RS + 4 * lq;
+    }
+    typedef float vf4 __attribute__((ext_vector_type(4)));  // a plain vector: stays in registers
+    vf4 pf[NLW];
+#define RHZ_RUNP_LOAD(Y0)                                                                   \
+    _Pragma("unroll") for (int t = 0; t < NLW; ++t) pf[t] = *reinterpret_cast<const vf4*>(lsrc[t] + (Y0));
+#define RHZ_RUNP_STAGE()                                                                    \
+    _Pragma("unroll") for (int t = 0; t < NLW; ++t) *reinterpret_cast<vf4*>(&in_t[0][0] + lofs[t]) = pf[t];
+    // out_t[plane] -> rows x0 .. x0 + 61 (lanes 1 .. 62) that lie in the lattice
+    auto store = [&](int y0) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            if (!plive[c]) continue;  // wave-uniform
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const int r = i * RPI + lr;
+                const int xr = x0 - 1 + r;
+                if (r >= 1 && r <= 62 && xr < nx)
+                    *reinterpret_cast<float4*>(out[c] + (int64_t)xr * pitch + y0 + 4 * lq) =
+                        *reinterpret_cast<const float4*>(&out_t[w * C + c][r * RS + 4 * lq]);
+            }
+        }
+    };
+    const SweepConst K = sweep_const(h);
+    const int nch = (ny + CH - 1) / CH;
+    auto chunk_y0 = [&](int ck) { return (DIR > 0 ? ck : nch - 1 - ck) * CH; };
+    RHZ_RUNP_LOAD(chunk_y0(0))
+    RHZ_RUNP_STAGE()
+    if (nch > 1) {
+        RHZ_RUNP_LOAD(chunk_y0(1))
+    }
+    float behind[C];
+    unsigned long long chgm[C];  // lanes that lowered a cell, per plane
+    float pc[C], pmp[C];  // the previous chunk's last cell: old value, z-neighbour minimum
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        behind[c] = kFar;
+        chgm[c] = 0;
+        pc[c] = pmp[c] = 0.0f;
+    }
+    for (int ck = 0; ck < nch; ++ck) {
+        const int y0 = chunk_y0(ck);
+        const bool more = ck + 1 < nch;
+        __syncthreads();  // every wave's rows of this chunk are staged
+        float o[C][CH], mp[C][CH];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const float4 a = *reinterpret_cast<const float4*>(&in_t[os[c]][lane * RS + 4 * q]);
+                const float4 b = *reinterpret_cast<const float4*>(&in_t[ms[c]][lane * RS + 4 * q]);
+                const float4 d = *reinterpret_cast<const float4*>(&in_t[ps[c]][lane * RS + 4 * q]);
+                o[c][4 * q] = a.x; o[c][4 * q + 1] = a.y; o[c][4 * q + 2] = a.z; o[c][4 * q + 3] = a.w;
+                mp[c][4 * q] = fminf(b.x, d.x); mp[c][4 * q + 1] = fminf(b.y, d.y);
+                mp[c][4 * q + 2] = fminf(b.z, d.z); mp[c][4 * q + 3] = fminf(b.w, d.w);
+            }
+        }
+        __syncthreads();  // the tiles are free for the next chunk
+        if (more) {  // the next chunk (loaded a chunk ago) into LDS, the one after into registers
+            RHZ_RUNP_STAGE()
+        }
+        if (ck + 2 < nch) {
+            RHZ_RUNP_LOAD(chunk_y0(ck + 2))
+        }
+        // keep the stores below: the staging waits for its loads, and a store
+        // issued before it would be waited for as well
+        __builtin_amdgcn_sched_barrier(0);
+        if (ck > 0) {  // finish the previous chunk: its last cells, then the chunk goes out
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const float next = DIR > 0 ? o[c][0] : o[c][CH - 1];
+                const float nv = face ? run_cellm<true>(pc[c], pmp[c], behind[c], next, xlo_face, xhi_face, chgm[c], K)
+                                      : run_cellm<false>(pc[c], pmp[c], behind[c], next, xlo_face, xhi_face, chgm[c], K);
+                behind[c] = nv;
+                out_t[w * C + c][lane * RS + PK] = nv;
+            }
+            wave_lds_sync<CH>();
+            store(chunk_y0(ck - 1));
+        }
+        const int nvalid = min(CH, ny - y0);
+        if (face) run_chunk_any<DIR, CH, C, true>(o, mp, nvalid, more, xlo_face, xhi_face, behind, chgm, K);
+        else run_chunk_any<DIR, CH, C, false>(o, mp, nvalid, more, xlo_face, xhi_face, behind, chgm, K);
+        wave_lds_sync<CH>();  // this wave's stores have read out_t
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                *reinterpret_cast<float4*>(&out_t[w * C + c][lane * RS + 4 * q]) =
+                    make_float4(o[c][4 * q], o[c][4 * q + 1], o[c][4 * q + 2], o[c][4 * q + 3]);
+            pc[c] = o[c][PK];
+            pmp[c] = mp[c][PK];
+        }
+        wave_lds_sync<CH>();
+    }
+    store(chunk_y0(nch - 1));
+#undef RHZ_RUNP_LOAD
+#undef RHZ_RUNP_STAGE
+    unsigned long long any = 0;
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        if (plive[c]) any |= chgm[c];
+    if ((any & __ballot(mine)) != 0 && lane == 0) atomicOr(changed, 1);
+}
+
+
 
 // The job (lattice) that logical block lb belongs to: starts[j] <= lb <
 // starts[j + 1] (starts[0] = 0, starts[njobs] = the grid size).
@@ -958,7 +1231,11 @@ __global__ void redblack_kernel(float* __restrict__ g, int dx, int dy, int dz, f
 
 // Launch shape of the lane sweeps (steps per load group, waves per block);
 // set by the tuning entry point, read at every launch.
-static SweepTuning g_sweep_tuning = {2, 4, 0, {14, 21, 21}, 16ll << 20, 0, {}, {}, {}, {}, 32, 1};
+// Class-free defaults: the shapes the autotuner picks for the 670M-voxel
+// vessel lattice on MI355X (x 1 x 4, z 1 x 2 tiles, four-step load groups;
+// the y-sweep of the y-fastest copy in the prefetching kernel, 4 waves of one
+// plane each).
+static SweepTuning g_sweep_tuning = {4, 4, 0, {14, 21, 12}, 16ll << 20, 0, {}, {}, {}, {}, 32, 1, 41};
 SweepTuning sweep_tuning() { return g_sweep_tuning; }
 bool set_sweep_tuning(int step, int rows, int fmajor) {
     if ((step != 2 && step != 4 && step != 8) || (rows != 4 && rows != 8 && rows != 16) ||
@@ -991,6 +1268,11 @@ int effective_step(int64_t voxels) {
 bool set_run_chunk(int cells) {
     if (cells != 16 && cells != 32) return false;
     g_sweep_tuning.run_chunk = cells;
+    return true;
+}
+bool set_run_block(int shape) {
+    if (shape != 0 && shape != 41) return false;
+    g_sweep_tuning.run_block = shape;
     return true;
 }
 bool set_fsm_layout(int layout) {
@@ -1172,6 +1454,23 @@ hipError_t launch_run(const float* src, float* dst, const Grid& g, const FieldLa
                                                                        (unsigned)blocks, g.h, changed);
     return hipGetLastError();
 }
+template <int DIR, int W, int CH, int C>
+hipError_t launch_runp(const float* src, float* dst, const Grid& g, const FieldLayout& L, int* changed,
+                       hipStream_t s) {
+    const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+    const int nbx = (nx + 61) / 62;
+    const int64_t blocks = (int64_t)nbx * ((nz + W * C - 1) / (W * C));
+    sweep_runp_kernel<DIR, CH, W, C><<<(unsigned)blocks, dim3(64, W), 0, s>>>(src, dst, nx, ny, nz, L.pitch, L.plane,
+                                                                             nbx, (unsigned)blocks, g.h, changed);
+    return hipGetLastError();
+}
+// "sweep_run_block" 41: 4 waves of one plane each, 16-cell chunks
+template <int DIR>
+hipError_t launch_runp(int shape, const float* src, float* dst, const Grid& g, const FieldLayout& L, int* changed,
+                       hipStream_t s) {
+    (void)shape;
+    return launch_runp<DIR, 4, 16, 1>(src, dst, g, L, changed, s);
+}
 }  // namespace
 
 hipError_t launch_sweep_on_yx(const float* src, float* dst, const Grid& g, int axis, int dir, int* changed,
@@ -1182,6 +1481,9 @@ hipError_t launch_sweep_on_yx(const float* src, float* dst, const Grid& g, int a
         return launch_lane<1, 2, 0>(src, dst, dy, dz, dx, L.plane, L.pitch, g.h, dir, changed, s);
     // axis 1: the walk runs along the rows, which must be padded to 32 floats
     if (L.pitch % 32 != 0 || !lane_sweeps_ok(g)) return hipErrorInvalidValue;
+    const int wb = sweep_tuning().run_block;
+    if (wb != 0)  // the prefetching kernel
+        return dir > 0 ? launch_runp<1>(wb, src, dst, g, L, changed, s) : launch_runp<-1>(wb, src, dst, g, L, changed, s);
     if (sweep_tuning().run_chunk == 16)
         return dir > 0 ? launch_run<1, 16>(src, dst, g, L, changed, s) : launch_run<-1, 16>(src, dst, g, L, changed, s);
     return dir > 0 ? launch_run<1, 32>(src, dst, g, L, changed, s) : launch_run<-1, 32>(src, dst, g, L, changed, s);

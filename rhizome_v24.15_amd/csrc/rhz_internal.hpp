@@ -128,9 +128,11 @@ struct SweepTuning {
     // share the GPU with the sign stages); 0 = the class-free value.
     int bcls_shape[4][3];
     int bcls_step[4];
-    int run_chunk;   // cells per register chunk of the y-sweep on the y-fastest copy (16, 32)
+    int run_chunk;   // cells per register chunk of sweep_run_kernel (16, 32)
     int fsm_layout;  // 0: x-sweeps on a y-fastest copy, the rest x-fastest (two transposes
                      // a round); 1: every sweep on the y-fastest copy (no transposes between rounds)
+    int run_block;   // y-sweep on the y-fastest copy: 0 = sweep_run_kernel, 41 = the prefetching
+                     // kernel (4 waves of one plane each per block)
 };
 // Lattice-size classes of the per-class sweep tuning: <= 2^21, <= 2^24,
 // <= 2^27 voxels, larger.
@@ -147,6 +149,7 @@ bool set_sweep_shape(int axis, int shape);
 bool set_class_shape(int cls, int axis, int shape);
 bool set_class_step(int cls, int step);
 bool set_run_chunk(int cells);
+bool set_run_block(int shape);
 bool set_fsm_layout(int layout);
 bool set_batch_class_shape(int cls, int axis, int shape);
 bool set_batch_class_step(int cls, int step);

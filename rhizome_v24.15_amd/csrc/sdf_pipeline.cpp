@@ -1997,6 +1997,10 @@ int rhz_tuning_set(const char* key, int64_t value) {
         if (!set_run_chunk((int)value)) return set_err(RHZ_ERR_ARG, "sweep_run_chunk must be 16 or 32");
         return RHZ_OK;
     }
+    if (!std::strcmp(key, "sweep_run_block")) {
+        if (!set_run_block((int)value)) return set_err(RHZ_ERR_ARG, "sweep_run_block must be 0 or 41");
+        return RHZ_OK;
+    }
     if (!std::strcmp(key, "verify_overlap")) {
         if (value != 0 && value != 1) return set_err(RHZ_ERR_ARG, "verify_overlap must be 0 or 1");
         g_verify_overlap.store((int)value, std::memory_order_relaxed);
@@ -2055,6 +2059,10 @@ int rhz_tuning_get(const char* key, int64_t* value) {
     }
     if (!std::strcmp(key, "sweep_run_chunk")) {
         *value = t.run_chunk;
+        return RHZ_OK;
+    }
+    if (!std::strcmp(key, "sweep_run_block")) {
+        *value = t.run_block;
         return RHZ_OK;
     }
     if (!std::strcmp(key, "verify_overlap")) {
