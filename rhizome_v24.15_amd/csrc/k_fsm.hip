@@ -1106,10 +1106,12 @@ __global__ __launch_bounds__(256) void sweep_lane_batch_kernel(const SweepJob* _
 // keeps the transpose at copy speed.
 constexpr int kTrTiles = 2;
 
+// With a sign volume (dense, laid out like `out`) every stored value gets the
+// [F] apply on the way: the write-back of the rounds' y-fastest copy.
 __device__ __forceinline__ void transpose_group(const float* __restrict__ in, float* __restrict__ out, int rows,
                                                 int cols, int64_t in_pitch, int64_t in_plane, int64_t out_pitch,
                                                 int64_t out_plane, int tiles_r, int tiles_c, unsigned lb,
-                                                float (&tile)[2][64][65]) {
+                                                float (&tile)[2][64][65], const uint8_t* __restrict__ sign = nullptr) {
     const int groups_c = (tiles_c + kTrTiles - 1) / kTrTiles;
     const int per_plane = tiles_r * groups_c;
     const int z = (int)(lb / (unsigned)per_plane);
@@ -1117,6 +1119,7 @@ __device__ __forceinline__ void transpose_group(const float* __restrict__ in, fl
     const int tr = t / groups_c, tg = t % groups_c;
     const float* inz = in + (int64_t)z * in_plane;
     float* outz = out + (int64_t)z * out_plane;
+    const uint8_t* signz = sign ? sign + (int64_t)z * out_plane : nullptr;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
     const int r0 = tr * 64;
     float v[16];
@@ -1140,7 +1143,10 @@ __device__ __forceinline__ void transpose_group(const float* __restrict__ in, fl
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int c = tc * 64 + ty + 4 * k;
-            if (r < rows && c < cols) outz[(int64_t)c * out_pitch + r] = tile[buf][tx][ty + 4 * k];
+            if (r < rows && c < cols) {
+                const int64_t o = (int64_t)c * out_pitch + r;
+                outz[o] = signz ? apply_sign(tile[buf][tx][ty + 4 * k], signz[o]) : tile[buf][tx][ty + 4 * k];
+            }
         }
     }
 }
@@ -1152,6 +1158,16 @@ __global__ __launch_bounds__(256) void transpose_planes_kernel(const float* __re
     __shared__ float tile[2][64][65];
     transpose_group(in, out, rows, cols, in_pitch, in_plane, out_pitch, out_plane, tiles_r, tiles_c,
                     xcd_logical_block(blockIdx.x, nblocks), tile);
+}
+
+// y-fastest copy -> the dense x-fastest field with the signs applied.
+__global__ __launch_bounds__(256) void transpose_apply_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                              const uint8_t* __restrict__ sign, int rows, int cols,
+                                                              int64_t in_pitch, int64_t in_plane, int tiles_r,
+                                                              int tiles_c, unsigned nblocks) {
+    __shared__ float tile[2][64][65];
+    transpose_group(in, out, rows, cols, in_pitch, in_plane, rows, (int64_t)rows * cols, tiles_r, tiles_c,
+                    xcd_logical_block(blockIdx.x, nblocks), tile, sign);
 }
 
 // The plane transposes of many lattices in one launch (the blocks of job j
@@ -1425,6 +1441,18 @@ hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool 
     const int64_t blocks = (int64_t)tr * ((tc + kTrTiles - 1) / kTrTiles) * dz;
     transpose_planes_kernel<<<(unsigned)blocks, 256, 0, s>>>(in, out, rows, cols, lin.pitch, lin.plane, lout.pitch,
                                                             lout.plane, tr, tc, (unsigned)blocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_apply(const float* in, const FieldLayout& lin, float* out, const uint8_t* sign,
+                                  const Grid& g, hipStream_t s) {
+    const int dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
+    const int rows = dx, cols = dy;  // in: rows along y (y-fastest); out: dense x-fastest
+    if (lin.pitch < cols || lin.plane < lin.pitch * rows) return hipErrorInvalidValue;
+    const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
+    const int64_t blocks = (int64_t)tr * ((tc + kTrTiles - 1) / kTrTiles) * dz;
+    transpose_apply_kernel<<<(unsigned)blocks, 256, 0, s>>>(in, out, sign, rows, cols, lin.pitch, lin.plane, tr, tc,
+                                                           (unsigned)blocks);
     return hipGetLastError();
 }
 

@@ -40,10 +40,7 @@ __global__ void fill_kernel(float* __restrict__ d, int64_t n, float v) {
 }
 
 // [F] signed = inside ? -d : d, with d > 1e29 -> 1e6 (cuda_rasterizer.cu:650-661).
-__device__ __forceinline__ float apply_one(float d, uint8_t s) {
-    if (d > kFarTest) d = 1e6f;
-    return (s == 1) ? -d : d;
-}
+__device__ __forceinline__ float apply_one(float d, uint8_t s) { return apply_sign(d, s); }
 
 __global__ void apply_kernel(float* __restrict__ d, const uint8_t* __restrict__ sign, int64_t n) {
     const int64_t n4 = n >> 2;

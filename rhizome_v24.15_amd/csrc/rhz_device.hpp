@@ -13,6 +13,13 @@ namespace rhz {
 
 constexpr float kFar = 1e30f;            // unreached marker (cuda_rasterizer.cu:77)
 constexpr float kFarTest = 1e29f;        // audit threshold (:531)
+
+// [F] one voxel of the sign apply (cuda_rasterizer.cu:650-661): a voxel still
+// at the sentinel ships as 1e6, an inside voxel (sign 1) negated.
+__device__ __forceinline__ float apply_sign(float d, uint8_t s) {
+    if (d > kFarTest) d = 1e6f;
+    return (s == 1) ? -d : d;
+}
 constexpr float kInv4Pi = 0.07957747154594767f;
 constexpr int kWave = 64;
 

@@ -161,6 +161,9 @@ hipError_t launch_sweep_octant(float* g, const Grid& gr, int sx, int sy, int sz,
 // Per-plane transpose between the x-fastest and the y-fastest layouts (dense,
 // or the layouts of the input and the output).
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s);
+// y-fastest field (layout lin) -> dense x-fastest field with the [F] sign apply
+hipError_t launch_transpose_apply(const float* in, const FieldLayout& lin, float* out, const uint8_t* sign,
+                                  const Grid& g, hipStream_t s);
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s,
                                const FieldLayout& lin, const FieldLayout& lout);
 // Any sweep on the y-fastest layout L (rows padded to 32 floats for axis 1):

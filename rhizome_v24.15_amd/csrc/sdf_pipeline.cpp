@@ -652,10 +652,11 @@ int fsm_rounds_batched(Ctx& c, int cap, int* rounds_out, Round&& round) {
 }
 
 // Where the FSM rounds left the field: the caller's dense buffer, or a padded
-// working copy in layout l.
+// working copy in layout l (y fastest when yfast).
 struct FsmField {
     const float* d;
     FieldLayout l;
+    bool yfast;
 };
 
 // [D] far-field re-distancing with the reference round policy: the rounds
@@ -747,18 +748,15 @@ int run_fsm_rounds(Ctx& c, int64_t* far_out, FsmField* keep = nullptr) {
         rc = audit_cur(&far);
         if (rc) return rc;
     }
-    if (cur_y) {
-        // back to the x-fastest layout: into q for the fused write-back, or
-        // straight into the caller's field
+    if (cur_y && keep) {
+        *keep = FsmField{cur, ly, true};  // run_fsm_finish transposes it back
+    } else if (cur_y) {
+        // back to the x-fastest layout, straight into the caller's field
         if (c.trans_t) c.trans_t->begin(c.s);
-        if (keep)
-            HIP_TRY(launch_transpose_xy(cur, q, c.g, false, c.s, ly, lx));
-        else
-            HIP_TRY(launch_transpose_xy(cur, c.d, c.g, false, c.s, ly, dense));
+        HIP_TRY(launch_transpose_xy(cur, c.d, c.g, false, c.s, ly, dense));
         if (c.trans_t) c.trans_t->end(c.s);
-        if (keep) *keep = FsmField{q, lx};
     } else if (keep) {
-        *keep = FsmField{cur, cur_l};
+        *keep = FsmField{cur, cur_l, false};
     } else if (cur != c.d) {
         HIP_TRY(launch_copy_rows(cur, cur_l.pitch, cur_l.plane, c.d, dense.pitch, dense.plane, (int)dx, c.g.dims[1],
                                  (int)dz, c.s));
@@ -793,19 +791,26 @@ int run_fsm(Ctx& c, const uint8_t* sign) {
 // [D3] and [F] after run_fsm_rounds kept its result in field f.  With voxels
 // left at the sentinel, f goes back into the dense field and the clamp and the
 // apply run there; otherwise (the clamp has nothing to do) the write-back
-// applies the signs on the way: one pass over the field instead of two.
+// applies the signs on the way: one pass over the field instead of two (for a
+// y-fastest copy, the transpose back to x fastest is that pass).
 // mark() runs on the stream just before the apply.
 template <typename Mark>
 int run_fsm_finish(Ctx& c, const uint8_t* sign, int64_t far, const FsmField& f, Mark&& mark) {
     const FieldLayout dense = dense_xy(c.g);
     const bool fused = f.d != c.d && far == 0;
-    if (f.d != c.d && !fused)
-        HIP_TRY(launch_copy_rows(f.d, f.l.pitch, f.l.plane, c.d, dense.pitch, dense.plane, c.g.dims[0], c.g.dims[1],
-                                 c.g.dims[2], c.s));
+    if (f.d != c.d && !fused) {
+        if (f.yfast)
+            HIP_TRY(launch_transpose_xy(f.d, c.d, c.g, false, c.s, f.l, dense));
+        else
+            HIP_TRY(launch_copy_rows(f.d, f.l.pitch, f.l.plane, c.d, dense.pitch, dense.plane, c.g.dims[0],
+                                     c.g.dims[1], c.g.dims[2], c.s));
+    }
     const int rc = run_fsm_clamp(c, sign, far);
     if (rc) return rc;
     mark();
-    if (fused)
+    if (fused && f.yfast)
+        HIP_TRY(launch_transpose_apply(f.d, f.l, c.d, sign, c.g, c.s));
+    else if (fused)
         HIP_TRY(launch_copy_apply_rows(f.d, f.l.pitch, f.l.plane, c.d, sign, c.g.dims[0], c.g.dims[1], c.g.dims[2],
                                        c.s));
     else
