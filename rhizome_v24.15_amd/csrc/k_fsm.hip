@@ -299,11 +299,22 @@ __device__ __forceinline__ float lane_step(float cur, float fm, float fp, float 
 // 2^-30 <= h <= 2^30 every positive discriminant is >= h^2 2^-25 >= 2^-96.
 // (disc = RN(a - b) for floats a ~ 2h^2 or 3h^2 and b >= 0: either b <= a/2
 // and disc >= a/2, or a - b is exact and a multiple of b's unit in the last
-// place, > b 2^-24 > a 2^-25.)  A zero discriminant gives root 0; a negative
-// one is never selected (whatever the short form makes of it is discarded).
-__device__ __forceinline__ float sqrt_disc(float disc) {
-    const float r = sqrt_rn_normal(disc);
-    return disc > 0.0f ? r : 0.0f;
+// place, > b 2^-24 > a 2^-25.)  A negative discriminant is never selected.
+// A zero one is raised to 2^-126 (one max instead of a compare and a select):
+// its root, about 2^-63, is then added to a sum >= h sqrt(2) >= 2^-30.5 (zero
+// 2-D discriminant: s1 - s0 = sqrt(2) h; zero 3-D one: s2 - s0 >= sqrt(2) h),
+// less than half that sum's unit in the last place, so the sum and the
+// candidate are those of root 0 bit for bit.
+__device__ __forceinline__ float sqrt_disc(float disc) { return sqrt_rn_normal(fmaxf(disc, 0x1.0p-126f)); }
+
+// x / 3 correctly rounded in two operations, RN(x zh + RN(x zl)) with
+// zh = RN(1/3) and zl = RN(1/3 - zh) (multiplication by a reciprocal split in
+// two, with one FMA; Brisebarre and Muller, IEEE TC 2008).  Compared with IEEE
+// division over every non-negative finite binary32 value: equal for every
+// x > 2^-123 (below that x zl underflows).  The sweeps' 3-D numerator is
+// s0 + s1 + s2 + root >= h / 2 >= 2^-31 wherever it is selected.
+__device__ __forceinline__ float div3_split(float x) {
+    return __builtin_fmaf(x, 0x1.555556p-2f, x * -0x1.555556p-27f);
 }
 
 // godunov_min with every candidate root computed and then selected: the roots
@@ -312,6 +323,8 @@ __device__ __forceinline__ float sqrt_disc(float disc) {
 // identical bit for bit.  godunov_cand is the candidate before the final
 // min with cur: callers that track changes take u < cur once, for the select
 // and for a wave mask (a scalar OR per step instead of two vector ops).
+// The 3-D discriminant h2x3 - RN(q 0.5) is one FMA, RN(h2x3 - q 0.5): q 0.5
+// is exact for q >= 2^-125, and below that both forms round to h2x3.
 __device__ __forceinline__ float godunov_cand(float a0, float a1, float a2, const SweepConst& K) {
     const float s0 = fminf(a0, fminf(a1, a2));
     const float s2 = fmaxf(a0, fmaxf(a1, a2));
@@ -321,8 +334,8 @@ __device__ __forceinline__ float godunov_cand(float a0, float a1, float a2, cons
     const float disc = K.h2x2 - d10 * d10;
     const float u2 = (s0 + s1 + sqrt_disc(disc)) * 0.5f;
     const float d21 = s2 - s1, d20 = s2 - s0;
-    const float disc3 = K.h2x3 - (d10 * d10 + d21 * d21 + d20 * d20) * 0.5f;
-    const float u3 = div3(s0 + s1 + s2 + sqrt_disc(disc3));
+    const float disc3 = __builtin_fmaf(d10 * d10 + d21 * d21 + d20 * d20, -0.5f, K.h2x3);
+    const float u3 = div3_split(s0 + s1 + s2 + sqrt_disc(disc3));
     const bool two = (u1 > s1) && (disc >= 0.0f);
     const bool three = two && (u2 > s2) && (disc3 >= 0.0f);
     return three ? u3 : (two ? u2 : u1);
