@@ -203,10 +203,13 @@ int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
  *                                  sweeps on a y-fastest copy, the others x
  *                                  fastest, two transposes every round;
  *   "sweep_run_block"              the y-sweep walking along the y-fastest rows:
- *                                  41 (default) = the prefetching kernel, 4 waves
+ *                                  42 (default) = the prefetching kernel, 4 waves
  *                                  of one plane each per block (planes staged
  *                                  once per block, next chunk loaded while one is
- *                                  computed); 0 = the one-wave-per-plane kernel;
+ *                                  computed), adjacent plane groups in
+ *                                  consecutive blocks; 41 = the same kernel, blocks
+ *                                  row-group-major; 0 = the one-wave-per-plane
+ *                                  kernel;
  *   "sweep_run_chunk"              16, 32 (default): cells per register chunk of
  *                                  that one-wave-per-plane kernel;
  *   "fsm_batch_rounds_max_voxels"  lattices up to this size enqueue all FSM

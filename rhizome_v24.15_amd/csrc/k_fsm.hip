@@ -806,7 +806,7 @@ __device__ __forceinline__ void run_chunk_any(float (&o)[C][CH], const float (&m
     }
 }
 
-template <int DIR, int CH, int W, int C>
+template <int DIR, int CH, int W, int C, bool ZMAJ = false>
 __global__ __launch_bounds__(64 * W) void sweep_runp_kernel(const float* __restrict__ src, float* __restrict__ dst,
                                                             int nx, int ny, int nz, int64_t pitch, int64_t plane,
                                                             int nbx, unsigned nblocks, float h,
@@ -823,8 +823,11 @@ __global__ __launch_bounds__(64 * W) void sweep_runp_kernel(const float* __restr
     __shared__ __align__(16) float in_t[NS][64 * RS];
     __shared__ __align__(16) float out_t[P][64 * RS];
     const unsigned lb = xcd_logical_block(blockIdx.x, nblocks);
-    const int bx = (int)(lb % (unsigned)nbx);
-    const int z0 = (int)(lb / (unsigned)nbx) * P;
+    const unsigned nzg = nblocks / (unsigned)nbx;  // plane groups
+    // ZMAJ: consecutive logical blocks (one XCD) are adjacent plane groups,
+    // which stage each other's boundary planes
+    const int bx = ZMAJ ? (int)(lb / nzg) : (int)(lb % (unsigned)nbx);
+    const int z0 = (ZMAJ ? (int)(lb % nzg) : (int)(lb / (unsigned)nbx)) * P;
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int lane = (int)threadIdx.x;
     const int x0 = bx * 62;
@@ -1264,7 +1267,7 @@ __global__ void redblack_kernel(float* __restrict__ g, int dx, int dy, int dz, f
 // vessel lattice on MI355X (x 1 x 4, z 1 x 2 tiles, four-step load groups;
 // the y-sweep of the y-fastest copy in the prefetching kernel, 4 waves of one
 // plane each).
-static SweepTuning g_sweep_tuning = {4, 4, 0, {14, 21, 12}, 16ll << 20, 0, {}, {}, {}, {}, 32, 1, 41};
+static SweepTuning g_sweep_tuning = {4, 4, 0, {14, 21, 12}, 16ll << 20, 0, {}, {}, {}, {}, 32, 1, 42};
 SweepTuning sweep_tuning() { return g_sweep_tuning; }
 bool set_sweep_tuning(int step, int rows, int fmajor) {
     if ((step != 2 && step != 4 && step != 8) || (rows != 4 && rows != 8 && rows != 16) ||
@@ -1300,7 +1303,7 @@ bool set_run_chunk(int cells) {
     return true;
 }
 bool set_run_block(int shape) {
-    if (shape != 0 && shape != 41) return false;
+    if (shape != 0 && shape != 41 && shape != 42) return false;
     g_sweep_tuning.run_block = shape;
     return true;
 }
@@ -1495,22 +1498,23 @@ hipError_t launch_run(const float* src, float* dst, const Grid& g, const FieldLa
                                                                        (unsigned)blocks, g.h, changed);
     return hipGetLastError();
 }
-template <int DIR, int W, int CH, int C>
+template <int DIR, int W, int CH, int C, bool ZMAJ>
 hipError_t launch_runp(const float* src, float* dst, const Grid& g, const FieldLayout& L, int* changed,
                        hipStream_t s) {
     const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
     const int nbx = (nx + 61) / 62;
     const int64_t blocks = (int64_t)nbx * ((nz + W * C - 1) / (W * C));
-    sweep_runp_kernel<DIR, CH, W, C><<<(unsigned)blocks, dim3(64, W), 0, s>>>(src, dst, nx, ny, nz, L.pitch, L.plane,
-                                                                             nbx, (unsigned)blocks, g.h, changed);
+    sweep_runp_kernel<DIR, CH, W, C, ZMAJ><<<(unsigned)blocks, dim3(64, W), 0, s>>>(
+        src, dst, nx, ny, nz, L.pitch, L.plane, nbx, (unsigned)blocks, g.h, changed);
     return hipGetLastError();
 }
-// "sweep_run_block" 41: 4 waves of one plane each, 16-cell chunks
+// "sweep_run_block" 41: 4 waves of one plane each, 16-cell chunks; 42: the
+// same with plane-group-major block order
 template <int DIR>
 hipError_t launch_runp(int shape, const float* src, float* dst, const Grid& g, const FieldLayout& L, int* changed,
                        hipStream_t s) {
-    (void)shape;
-    return launch_runp<DIR, 4, 16, 1>(src, dst, g, L, changed, s);
+    if (shape == 42) return launch_runp<DIR, 4, 16, 1, true>(src, dst, g, L, changed, s);
+    return launch_runp<DIR, 4, 16, 1, false>(src, dst, g, L, changed, s);
 }
 }  // namespace
 

@@ -132,7 +132,8 @@ struct SweepTuning {
     int fsm_layout;  // 0: x-sweeps on a y-fastest copy, the rest x-fastest (two transposes
                      // a round); 1: every sweep on the y-fastest copy (no transposes between rounds)
     int run_block;   // y-sweep on the y-fastest copy: 0 = sweep_run_kernel, 41 = the prefetching
-                     // kernel (4 waves of one plane each per block)
+                     // kernel (4 waves of one plane each per block), 42 = the same with
+                     // plane-group-major block order (the default)
 };
 // Lattice-size classes of the per-class sweep tuning: <= 2^21, <= 2^24,
 // <= 2^27 voxels, larger.

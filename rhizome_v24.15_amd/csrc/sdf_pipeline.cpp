@@ -2003,7 +2003,7 @@ int rhz_tuning_set(const char* key, int64_t value) {
         return RHZ_OK;
     }
     if (!std::strcmp(key, "sweep_run_block")) {
-        if (!set_run_block((int)value)) return set_err(RHZ_ERR_ARG, "sweep_run_block must be 0 or 41");
+        if (!set_run_block((int)value)) return set_err(RHZ_ERR_ARG, "sweep_run_block must be 0, 41 or 42");
         return RHZ_OK;
     }
     if (!std::strcmp(key, "verify_overlap")) {

@@ -23,7 +23,7 @@ def layout(rhz):
 
 
 # (41: the prefetching kernel, 0: sweep_run_kernel; cells per chunk of sweep_run_kernel)
-RUN_VARIANTS = [(0, 32), (0, 16), (41, 32)]
+RUN_VARIANTS = [(0, 32), (0, 16), (41, 32), (42, 32)]
 
 
 # lattices cut from the small sphere's frame: x past one 64-row block, y
