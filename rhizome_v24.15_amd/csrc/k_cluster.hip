@@ -254,10 +254,12 @@ __device__ __forceinline__ void voxel_pos(const Pts& P, int64_t v, float& px, fl
 
 // Lane k takes list position order[k]; lanes of a wave are spatially close,
 // so the clusters near one lane are near the others.  Far clusters: dipole
-// term + bound (binary32 with hardware reciprocal square root, accumulated
-// in binary64; the bound is inflated by 1% and a 2e-3 margin covers the
-// rounding of both); near clusters: exact solid angles of their triangles
-// (wave-uniform loop; lanes for which the cluster is far sit it out).
+// term + bound (binary32 with hardware reciprocal square root, compensated
+// (Kahan) binary32 sum; the bound is inflated by 1% and a 2e-3 margin covers
+// the rounding of both); near clusters: exact solid angles of their triangles
+// (wave-uniform loop; lanes for which the cluster is far sit it out).  Only
+// the finalize step, which adds a probe's partial rows and checks the sum
+// against its bound, works in binary64.
 // Orders a wave's LDS writes before its later reads (and the reverse).
 __device__ __forceinline__ void wave_fence() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -266,10 +268,11 @@ __device__ __forceinline__ void wave_fence() {
 }
 
 // Terms of one cluster (info at ci) for a lane whose position is (px, py,
-// pz): far (d > beta r) adds the two-term expansion to `far` and its bound to
-// `err` and returns false; near returns true and adds nothing.
+// pz): far (d > beta r) adds the two-term expansion to the compensated sum
+// (far, farc) and its bound to `err` and returns false; near returns true and
+// adds nothing.
 __device__ __forceinline__ bool cluster_term(const float4* __restrict__ ci, float px, float py, float pz,
-                                             float beta2, bool live, double& far, float& err) {
+                                             float beta2, bool live, float& far, float& farc, float& err) {
     const float4 a = ci[0];
     const float dx = a.x - px, dy = a.y - py, dz = a.z - pz;
     const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
@@ -286,7 +289,10 @@ __device__ __forceinline__ bool cluster_term(const float4* __restrict__ ci, floa
         const float my = __builtin_fmaf(m1.y, hz, __builtin_fmaf(m1.x, hy, m0.w * hx));
         const float mz = __builtin_fmaf(m2.x, hz, __builtin_fmaf(m1.w, hy, m1.z * hx));
         const float xmx = __builtin_fmaf(hz, mz, __builtin_fmaf(hy, my, hx * mx));
-        far += (double)((dip + __builtin_fmaf(-3.0f, xmx, m2.y)) * inv3);
+        const float y = (dip + __builtin_fmaf(-3.0f, xmx, m2.y)) * inv3 - farc;
+        const float t = far + y;
+        farc = (t - far) - y;
+        far = t;
         // remainder bound 3 r^2 A / (d - r)^4; the hardware reciprocal
         // (within 1 ulp) is covered by the bound's 1% inflation
         const float g = d - a.w;
@@ -319,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void winding_far_kernel(const float* __rest
     float px = 0.f, py = 0.f, pz = 0.f;
     if (live) voxel_pos(P, P.idx[P.order[k0]], px, py, pz);
     float sum = 0.0f, comp = 0.0f;  // near field, compensated
-    double far = 0.0;
+    float far = 0.0f, farc = 0.0f;  // far field, compensated
     float err = 0.0f;
     const float beta2 = beta * beta;
     const int qb = (int)blockIdx.y * per_part;
@@ -332,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void winding_far_kernel(const float* __rest
             nearp[u] = false;
             if (q >= qe) continue;
             // without parents (the second, tighter pass) every parent is "near"
-            nearp[u] = !use_parents || cluster_term(pinfo + kInfo * q, px, py, pz, beta2, live, far, err);
+            nearp[u] = !use_parents || cluster_term(pinfo + kInfo * q, px, py, pz, beta2, live, far, farc, err);
         }
 #pragma unroll
         for (int u = 0; u < kCl; ++u) {
@@ -349,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void winding_far_kernel(const float* __rest
                     const int c = cb + v;
                     nearc[v] = false;
                     if (c >= ch.x + ch.y) continue;
-                    nearc[v] = cluster_term(info + kInfo * c, px, py, pz, beta2, want, far, err);
+                    nearc[v] = cluster_term(info + kInfo * c, px, py, pz, beta2, want, far, farc, err);
                 }
 #pragma unroll
                 for (int v = 0; v < kCl; ++v) {
@@ -390,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void winding_far_kernel(const float* __rest
     }
     if (live) {
         const int64_t at = (int64_t)blockIdx.y * n + k0;
-        part_w[at] = ((double)sum - (double)comp + far);
+        part_w[at] = ((double)sum - (double)comp) + ((double)far - (double)farc);
         part_err[at] = err;
     }
 }

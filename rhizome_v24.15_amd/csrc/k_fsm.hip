@@ -1127,7 +1127,8 @@ constexpr int kTrTiles = 2;
 __device__ __forceinline__ void transpose_group(const float* __restrict__ in, float* __restrict__ out, int rows,
                                                 int cols, int64_t in_pitch, int64_t in_plane, int64_t out_pitch,
                                                 int64_t out_plane, int tiles_r, int tiles_c, unsigned lb,
-                                                float (&tile)[2][64][65], const uint8_t* __restrict__ sign = nullptr) {
+                                                float (&tile)[2][64][65], const uint8_t* __restrict__ sign = nullptr,
+                                                unsigned long long* __restrict__ far = nullptr) {
     const int groups_c = (tiles_c + kTrTiles - 1) / kTrTiles;
     const int per_plane = tiles_r * groups_c;
     const int z = (int)(lb / (unsigned)per_plane);
@@ -1148,6 +1149,7 @@ __device__ __forceinline__ void transpose_group(const float* __restrict__ in, fl
         }
     };
     const int tc0 = tg * kTrTiles, tc1 = min(tiles_c, tc0 + kTrTiles);
+    unsigned nfar = 0;  // stored cells still at the sentinel (when `far` is given)
     load(tc0);
     for (int tc = tc0; tc < tc1; ++tc) {
         const int buf = (tc - tc0) & 1;
@@ -1161,9 +1163,15 @@ __device__ __forceinline__ void transpose_group(const float* __restrict__ in, fl
             const int c = tc * 64 + ty + 4 * k;
             if (r < rows && c < cols) {
                 const int64_t o = (int64_t)c * out_pitch + r;
-                outz[o] = signz ? apply_sign(tile[buf][tx][ty + 4 * k], signz[o]) : tile[buf][tx][ty + 4 * k];
+                const float v = tile[buf][tx][ty + 4 * k];
+                if (far) nfar += v >= kFarTest;
+                outz[o] = signz ? apply_sign(v, signz[o]) : v;
             }
         }
+    }
+    if (far) {
+        for (int off = 32; off > 0; off >>= 1) nfar += __shfl_xor(nfar, off);
+        if (__lane_id() == 0 && nfar) atomicAdd(far, (unsigned long long)nfar);
     }
 }
 
@@ -1176,14 +1184,16 @@ __global__ __launch_bounds__(256) void transpose_planes_kernel(const float* __re
                     xcd_logical_block(blockIdx.x, nblocks), tile);
 }
 
-// y-fastest copy -> the dense x-fastest field with the signs applied.
+// y-fastest copy -> the dense x-fastest field with the signs applied; with
+// `far`, also the [D2] audit of the copy (cells >= 1e29 added to *far).
 __global__ __launch_bounds__(256) void transpose_apply_kernel(const float* __restrict__ in, float* __restrict__ out,
                                                               const uint8_t* __restrict__ sign, int rows, int cols,
                                                               int64_t in_pitch, int64_t in_plane, int tiles_r,
-                                                              int tiles_c, unsigned nblocks) {
+                                                              int tiles_c, unsigned nblocks,
+                                                              unsigned long long* __restrict__ far) {
     __shared__ float tile[2][64][65];
     transpose_group(in, out, rows, cols, in_pitch, in_plane, rows, (int64_t)rows * cols, tiles_r, tiles_c,
-                    xcd_logical_block(blockIdx.x, nblocks), tile, sign);
+                    xcd_logical_block(blockIdx.x, nblocks), tile, sign, far);
 }
 
 // The plane transposes of many lattices in one launch (the blocks of job j
@@ -1461,14 +1471,14 @@ hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool 
 }
 
 hipError_t launch_transpose_apply(const float* in, const FieldLayout& lin, float* out, const uint8_t* sign,
-                                  const Grid& g, hipStream_t s) {
+                                  const Grid& g, hipStream_t s, unsigned long long* far) {
     const int dx = g.dims[0], dy = g.dims[1], dz = g.dims[2];
     const int rows = dx, cols = dy;  // in: rows along y (y-fastest); out: dense x-fastest
     if (lin.pitch < cols || lin.plane < lin.pitch * rows) return hipErrorInvalidValue;
     const int tr = (rows + 63) / 64, tc = (cols + 63) / 64;
     const int64_t blocks = (int64_t)tr * ((tc + kTrTiles - 1) / kTrTiles) * dz;
     transpose_apply_kernel<<<(unsigned)blocks, 256, 0, s>>>(in, out, sign, rows, cols, lin.pitch, lin.plane, tr, tc,
-                                                           (unsigned)blocks);
+                                                           (unsigned)blocks, far);
     return hipGetLastError();
 }
 

@@ -164,7 +164,7 @@ hipError_t launch_sweep_octant(float* g, const Grid& gr, int sx, int sy, int sz,
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s);
 // y-fastest field (layout lin) -> dense x-fastest field with the [F] sign apply
 hipError_t launch_transpose_apply(const float* in, const FieldLayout& lin, float* out, const uint8_t* sign,
-                                  const Grid& g, hipStream_t s);
+                                  const Grid& g, hipStream_t s, unsigned long long* far = nullptr);
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s,
                                const FieldLayout& lin, const FieldLayout& lout);
 // Any sweep on the y-fastest layout L (rows padded to 32 floats for axis 1):

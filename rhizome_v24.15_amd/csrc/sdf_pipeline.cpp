@@ -652,12 +652,27 @@ int fsm_rounds_batched(Ctx& c, int cap, int* rounds_out, Round&& round) {
 }
 
 // Where the FSM rounds left the field: the caller's dense buffer, or a padded
-// working copy in layout l (y fastest when yfast).
+// working copy in layout l (y fastest when yfast).  deferred: the [D2] audit
+// and the extension rounds are left to run_fsm_finish, which audits while it
+// writes the copy back; `other` is the second working buffer for those rounds.
 struct FsmField {
     const float* d;
     FieldLayout l;
     bool yfast;
+    bool deferred = false;
+    float* other = nullptr;
 };
+
+// [D2] audit of a y-fastest copy d (rows along y, pitch and plane of L).
+int audit_rows_y(Ctx& c, int64_t* far, const float* d, const FieldLayout& L) {
+    HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FAR), 0, sizeof(unsigned long long), c.s));
+    HIP_TRY(launch_count_far_rows(d, L.pitch, L.plane, c.g.dims[1], c.g.dims[0], c.g.dims[2], cnt_ptr(c.w, C_FAR),
+                                  c.s));
+    const int rc = read_counters(c);
+    if (rc) return rc;
+    *far = (int64_t)hc(c, C_FAR);
+    return RHZ_OK;
+}
 
 // [D] far-field re-distancing with the reference round policy: the rounds
 // and the audit-gated extension rounds; *far = voxels still at the sentinel.
@@ -705,13 +720,7 @@ int run_fsm_rounds(Ctx& c, int64_t* far_out, FsmField* keep = nullptr) {
     // sentinel count of the current field (rows along y on the y-fastest copy)
     auto audit_cur = [&](int64_t* far) -> int {
         if (!cur_y) return audit(c, far, cur, &cur_l);
-        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FAR), 0, sizeof(unsigned long long), c.s));
-        HIP_TRY(launch_count_far_rows(cur, cur_l.pitch, cur_l.plane, c.g.dims[1], c.g.dims[0], c.g.dims[2],
-                                      cnt_ptr(c.w, C_FAR), c.s));
-        const int rc = read_counters(c);
-        if (rc) return rc;
-        *far = (int64_t)hc(c, C_FAR);
-        return RHZ_OK;
+        return audit_rows_y(c, far, cur, cur_l);
     };
     int* changed = flag_ptr(c.w, F_CHANGED);
     const bool converge = c.p->fsm_policy == RHZ_FSM_CONVERGE || octant;
@@ -738,18 +747,25 @@ int run_fsm_rounds(Ctx& c, int64_t* far_out, FsmField* keep = nullptr) {
         if (hf(c, F_CHANGED) == 0) break;
     }
     int64_t far = 0;
-    int rc = audit_cur(&far);
-    if (rc) return rc;
     int extra = 0;
-    while (far > 0 && extra < c.p->max_extra_rounds) {
-        rc = round(changed);
+    // A y-fastest copy the caller keeps is audited by its write-back, which
+    // counts the sentinel cells as it transposes them (run_fsm_finish runs
+    // the extension rounds there when any are left): one pass over the field
+    // fewer.
+    const bool defer = cur_y && keep;
+    if (!defer) {
+        int rc = audit_cur(&far);
         if (rc) return rc;
-        ++extra;
-        rc = audit_cur(&far);
-        if (rc) return rc;
+        while (far > 0 && extra < c.p->max_extra_rounds) {
+            rc = round(changed);
+            if (rc) return rc;
+            ++extra;
+            rc = audit_cur(&far);
+            if (rc) return rc;
+        }
     }
-    if (cur_y && keep) {
-        *keep = FsmField{cur, ly, true};  // run_fsm_finish transposes it back
+    if (defer) {
+        *keep = FsmField{cur, ly, true, true, q};  // run_fsm_finish audits and transposes it back
     } else if (cur_y) {
         // back to the x-fastest layout, straight into the caller's field
         if (c.trans_t) c.trans_t->begin(c.s);
@@ -763,7 +779,7 @@ int run_fsm_rounds(Ctx& c, int64_t* far_out, FsmField* keep = nullptr) {
     }
     c.st->eikonal_rounds = rounds + extra;
     c.st->extra_rounds = extra;
-    *far_out = far;
+    *far_out = defer ? -1 : far;
     return RHZ_OK;
 }
 
@@ -797,6 +813,36 @@ int run_fsm(Ctx& c, const uint8_t* sign) {
 template <typename Mark>
 int run_fsm_finish(Ctx& c, const uint8_t* sign, int64_t far, const FsmField& f, Mark&& mark) {
     const FieldLayout dense = dense_xy(c.g);
+    if (f.deferred) {
+        // the write-back with the signs, auditing the copy on the way; when
+        // nothing is left at the sentinel (the usual case) it is final
+        HIP_TRY(hipMemsetAsync(cnt_ptr(c.w, C_FAR), 0, sizeof(unsigned long long), c.s));
+        mark();
+        HIP_TRY(launch_transpose_apply(f.d, f.l, c.d, sign, c.g, c.s, cnt_ptr(c.w, C_FAR)));
+        int rc = read_counters(c);
+        if (rc) return rc;
+        far = (int64_t)hc(c, C_FAR);
+        if (far == 0) return run_fsm_clamp(c, sign, 0);
+        // cells left at the sentinel: the extension rounds on the copy (still
+        // intact), then the write-back without the fusion over what was written
+        int* changed = flag_ptr(c.w, F_CHANGED);
+        int extra = 0;
+        while (far > 0 && extra < c.p->max_extra_rounds) {
+            rc = sweep_round_y(c, f.d, f.l, true, const_cast<float*>(f.d), f.other, f.l, changed);
+            if (rc) return rc;
+            ++extra;
+            rc = audit_rows_y(c, &far, f.d, f.l);
+            if (rc) return rc;
+        }
+        c.st->eikonal_rounds += extra;
+        c.st->extra_rounds = extra;
+        HIP_TRY(launch_transpose_xy(f.d, c.d, c.g, false, c.s, f.l, dense));
+        rc = run_fsm_clamp(c, sign, far);
+        if (rc) return rc;
+        mark();
+        HIP_TRY(launch_apply(c.d, sign, c.g.n(), c.s));
+        return RHZ_OK;
+    }
     const bool fused = f.d != c.d && far == 0;
     if (f.d != c.d && !fused) {
         if (f.yfast)

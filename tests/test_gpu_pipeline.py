@@ -73,6 +73,22 @@ def test_extension_rounds(rhz, orc, small_sphere):
     assert st.extra_rounds == 1 and st.far_residual == 0
 
 
+def test_extension_rounds_after_rounds_on_the_working_copy(rhz, orc, small_sphere):
+    """A lattice above the sphere, farther than the band: its columns cross the
+    mesh (so the path does not decline) but nothing is splatted, the rounds
+    leave every voxel at the sentinel, and the audit that the write-back of the
+    y-fastest working copy runs (41 x 36 x 35, rows padded to 64) finds them:
+    the extension round and the safe clamp then run after the rounds, as in
+    the oracle."""
+    tris, h, _, _ = small_sphere
+    mn, mx = (-18, -17, 40), (22, 18, 74)
+    st, ost, got, ref = _compare_fast(rhz, orc, tris, h, mn, mx, {"max_rounds": 2, "max_extra_rounds": 1},
+                                      {"max_rounds": 2, "max_extra_rounds": 1})
+    assert got is not None, st.as_dict()
+    assert st.eikonal_rounds == ost.eikonal_rounds and st.extra_rounds == 1
+    assert st.safe_clamped == got.size and st.far_residual == 0
+
+
 def test_decline_csr(rhz, orc, small_sphere):
     tris, h, mn, mx = small_sphere
     st, ost, _, _ = _compare_fast(rhz, orc, tris, h, mn, mx, {"csr_max_entries": 10}, {"csr_max_entries": 10})
