@@ -235,6 +235,14 @@ int rhz_tuning_get(const char* key, int64_t* value);
 /* The size class of a lattice: 0 up to 2^21 voxels, 1 up to 2^24, 2 up to
  * 2^27, 3 above. */
 int rhz_sweep_shape_class(const int32_t dims[3]);
+
+/* Exhaustive self-check of the directional sweeps' square root on the current
+ * device: a hardware reciprocal square root and one Newton step, compared
+ * with IEEE sqrt for every binary32 argument in [2^-96, inf), the range the
+ * sweeps use (each of two references counts).  *mismatches = 0 means the
+ * sweeps' Godunov update rounds exactly as the reference's sqrtf.  No
+ * counterpart in the reference. */
+int rhz_selftest_sweep_sqrt(uint64_t* mismatches);
 /* Frees the cached device workspaces of the current device: the one the
  * single-lattice calls share and the batched bake lanes' own. */
 void rhz_release_workspace(void);

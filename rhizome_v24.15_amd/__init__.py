@@ -239,6 +239,7 @@ def lib():
     L.rhz_tuning_set.argtypes = [C.c_char_p, i64]
     L.rhz_tuning_get.argtypes = [C.c_char_p, C.POINTER(C.c_int64)]
     L.rhz_sweep_shape_class.argtypes = [C.POINTER(C.c_int32)]
+    L.rhz_selftest_sweep_sqrt.argtypes = [C.POINTER(C.c_uint64)]
     L.rhz_mlp_param_count.restype = C.c_int
     L.rhz_mlp_blob_count.restype = C.c_int
     L.rhz_mlp_builtin.argtypes = [vp]
@@ -300,6 +301,14 @@ def sweep_shape_class(dims) -> int:
     """Lattice-size class of the per-class sweep tuning keys ("sweep_shape_x@C",
     ...): 0 up to 2^21 voxels, 1 up to 2^24, 2 up to 2^27, 3 above."""
     return check(lib().rhz_sweep_shape_class(_i32x3(dims)), "rhz_sweep_shape_class")
+
+
+def selftest_sweep_sqrt() -> int:
+    """Mismatches of the sweeps' square root against IEEE sqrt over every
+    binary32 argument in [2^-96, inf), checked on the device (0 = exact)."""
+    m = C.c_uint64(0)
+    check(lib().rhz_selftest_sweep_sqrt(C.byref(m)), "rhz_selftest_sweep_sqrt")
+    return int(m.value)
 
 
 def device_count() -> int:

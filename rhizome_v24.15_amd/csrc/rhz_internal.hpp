@@ -165,6 +165,9 @@ hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool 
 // y-fastest field (layout lin) -> dense x-fastest field with the [F] sign apply
 hipError_t launch_transpose_apply(const float* in, const FieldLayout& lin, float* out, const uint8_t* sign,
                                   const Grid& g, hipStream_t s, unsigned long long* far = nullptr);
+// Exhaustive device check of the sweeps' square root: mismatches against IEEE
+// sqrt over every binary32 argument in [2^-96, inf) are added to *bad.
+hipError_t launch_sqrt_selftest(unsigned long long* bad, hipStream_t s);
 hipError_t launch_transpose_xy(const float* in, float* out, const Grid& g, bool to_yx, hipStream_t s,
                                const FieldLayout& lin, const FieldLayout& lout);
 // Any sweep on the y-fastest layout L (rows padded to 32 floats for axis 1):

@@ -3017,6 +3017,22 @@ int rhz_interior_count_device(const float* d_grid, int64_t n, int64_t* count, vo
     return interior_count_run(w, d_grid, n, count, (hipStream_t)stream);
 }
 
+int rhz_selftest_sweep_sqrt(uint64_t* mismatches) {
+    int rc = need_device();
+    if (rc) return rc;
+    if (!mismatches) return set_err(RHZ_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace* w = workspace();
+    if (!w) return set_err(RHZ_ERR_NOMEM, "workspace initialisation failed");
+    unsigned long long* bad = cnt_ptr(w, C_MISM);
+    HIP_TRY(hipMemsetAsync(bad, 0, sizeof(unsigned long long), nullptr));
+    HIP_TRY(launch_sqrt_selftest(bad, nullptr));
+    unsigned long long h = 0;
+    HIP_TRY(hipMemcpy(&h, bad, sizeof(h), hipMemcpyDeviceToHost));
+    *mismatches = (uint64_t)h;
+    return RHZ_OK;
+}
+
 int rhz_interior_count(const float* grid, int64_t n, int64_t* count) {
     int rc = need_device();
     if (rc) return rc;

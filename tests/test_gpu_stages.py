@@ -139,6 +139,14 @@ def test_sweep_bit_exact(rhz, orc, torch_gpu, small_sphere, axis, direction):
         f"{(got != ref).sum()} voxels differ, max {np.abs(got - ref).max()}"
 
 
+def test_sweep_sqrt_is_ieee_over_its_whole_range(rhz):
+    """The lane sweeps take their square roots as a hardware reciprocal
+    square root plus one Newton step; on this device that equals IEEE sqrt
+    (binary32, and binary64 rounded) for every binary32 argument in
+    [2^-96, inf), which is every argument the sweeps pass it."""
+    assert rhz.selftest_sweep_sqrt() == 0
+
+
 @pytest.mark.parametrize("shape", [(1, 1, 1), (1, 7, 5), (70, 3, 2), (65, 66, 3), (5, 130, 67)])
 def test_sweep_odd_shapes(rhz, orc, torch_gpu, shape):
     """Ragged shapes: partial waves, partial x-chunks, single-voxel axes."""
