@@ -726,7 +726,10 @@ __global__ __launch_bounds__(256) void sweep_run_kernel(const float* __restrict_
 //    instruction writes whole 16-byte-aligned row pieces (a lane storing its
 //    own row's pieces straight from registers measured 40% slower);
 //  * LDS rows are padded to CH + 4 floats, so 16-byte reads of a lane's own
-//    row and 16-byte writes of row pieces touch distinct banks.
+//    row touch distinct banks, and so do the row pieces of the staging and
+//    the stores, whose lanes pair rows 4 apart (with adjacent rows the
+//    counters showed 2.4 times the bank-conflict cycles; the sweep's time
+//    moved by 0.5%: LDS is not what bounds it).
 // Measured on the 883 x 823 x 923 lattice and not adopted: direct global->LDS
 // loads (global_load_lds_dwordx4; a lane's 16 bytes became a request of its
 // own), two planes per wave (C = 2: about 200 VGPRs, two waves per SIMD), and
@@ -849,8 +852,12 @@ __global__ __launch_bounds__(64 * W) void sweep_runp_kernel(const float* __restr
         out[c] = dst + (int64_t)zc * plane;
     }
     // load instruction t of this wave: flattened j = w + t W = s Q + i (plane s,
-    // instruction i); lane -> row i RPI + lane / Q, quad lane % Q
-    const int lr = lane / Q, lq = lane % Q;
+    // instruction i); lane -> row i RPI + lr, quad lane % Q.  With 16-cell
+    // chunks the 8 lanes that LDS serves together (8 g .. 8 g + 7) take two
+    // rows 4 apart: at a stride of 20 floats their 16-byte pieces then fill 8
+    // distinct 4-bank groups (two adjacent rows would share one)
+    const int lr = Q == 4 ? ((lane >> 2) & 1) * 4 + ((lane >> 3) & 3) + ((lane >> 5) << 3) : lane / Q;
+    const int lq = lane % Q;
     const float* lsrc[NLW];
     int lofs[NLW];
 #pragma unroll
