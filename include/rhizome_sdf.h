@@ -191,9 +191,9 @@ int rhz_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
  *   "sweep_shape_x" / "_y" / "_z"  tile each lane of an x-, y- or z-sweep walks:
  *                                  1 = one line (rows / fmajor apply), or
  *                                  10 NF + NC = NF fast-axis segments x NC cross
- *                                  lines: 21, 22, 12, 14 (defaults 14, 21, 12);
+ *                                  lines: 21, 22, 12, 14 (defaults 14, 21, 14);
  *   "sweep_step"                   2, 4, 8: steps whose loads a sweep wave issues
- *                                  together (default 4);
+ *                                  together (default 2);
  *   "sweep_rows", "sweep_fmajor"   waves per block (4, 8, 16) and 1: a block's
  *                                  waves take adjacent segments of one row, for
  *                                  the one-line walker;

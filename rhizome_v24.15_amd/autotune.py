@@ -135,6 +135,7 @@ def autotune_bake(rhz, torch, items, params, sample=16, log=None, shapes=True):
 # ---------------------------------------------------------------------------
 SHAPE_KEYS = ("sweep_shape_x", "sweep_shape_y", "sweep_shape_z")
 TILE_SHAPES = (14, 21, 22, 12, 1)  # 10 NF + NC tiles, and 1 = the one-line walker
+KEEP_MARGIN = 0.005  # relative: 0.5% of a build's sweep time
 LOAD_STEPS = (2, 4, 8)
 # vessel lattice spacings (mm) whose lattices fall in classes 0..3
 CLASS_SPACINGS = {0: 0.9, 1: 0.5, 2: 0.25, 3: 0.10}  # 1.6M, 7.2M, 48M, 671M voxels
@@ -210,20 +211,22 @@ def tune_sweep_class(rhz, torch, geom, cls, params=None, reps=3, log=None):
 
     _sweeps_ms(rhz, torch, case, p, 1)  # warm the workspace
     best = trial(cfg)
+    # a candidate replaces the incumbent only when faster by more than the
+    # spread between repeated timings
     for k in SHAPE_KEYS:
         for shape in TILE_SHAPES:
             if shape == cfg[k]:
                 continue
             c = dict(cfg, **{k: shape})
             ms = trial(c)
-            if ms < best:
+            if ms < best * (1.0 - KEEP_MARGIN):
                 best, cfg = ms, c
     for step in LOAD_STEPS:
         if step == cfg["sweep_step"]:
             continue
         c = dict(cfg, sweep_step=step)
         ms = trial(c)
-        if ms < best:
+        if ms < best * (1.0 - KEEP_MARGIN):
             best, cfg = ms, c
     for k, v in cfg.items():
         rhz.tuning_set(f"{k}@{cls}", v)

@@ -1284,7 +1284,10 @@ __global__ void redblack_kernel(float* __restrict__ g, int dx, int dy, int dz, f
 // vessel lattice on MI355X (x 1 x 4, z 1 x 2 tiles, four-step load groups;
 // the y-sweep of the y-fastest copy in the prefetching kernel, 4 waves of one
 // plane each).
-static SweepTuning g_sweep_tuning = {4, 4, 0, {14, 21, 12}, 16ll << 20, 0, {}, {}, {}, {}, 32, 1, 42};
+// Class-free defaults: the shape and load-group depth the autotuner keeps for
+// the largest lattices (1 x 4 tiles for x and z on the y-fastest copy, two-step
+// groups), so a caller that never tunes gets them where they matter most.
+static SweepTuning g_sweep_tuning = {2, 4, 0, {14, 21, 14}, 16ll << 20, 0, {}, {}, {}, {}, 32, 1, 42};
 SweepTuning sweep_tuning() { return g_sweep_tuning; }
 bool set_sweep_tuning(int step, int rows, int fmajor) {
     if ((step != 2 && step != 4 && step != 8) || (rows != 4 && rows != 8 && rows != 16) ||
