@@ -1160,11 +1160,21 @@ __device__ __forceinline__ void transpose_group(const float* __restrict__ in, fl
     load(tc0);
     for (int tc = tc0; tc < tc1; ++tc) {
         const int buf = (tc - tc0) & 1;
+        const int r = r0 + tx;
+        // this tile's sign bytes, requested before the LDS exchange so that
+        // they arrive while it runs (not one round trip per store)
+        uint8_t sg[16];
+        if (signz) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int c = tc * 64 + ty + 4 * k;
+                sg[k] = (r < rows && c < cols) ? signz[(int64_t)c * out_pitch + r] : (uint8_t)0;
+            }
+        }
 #pragma unroll
         for (int k = 0; k < 16; ++k) tile[buf][ty + 4 * k][tx] = v[k];
         __syncthreads();  // also orders the previous use of the other buffer
         if (tc + 1 < tc1) load(tc + 1);
-        const int r = r0 + tx;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int c = tc * 64 + ty + 4 * k;
@@ -1172,7 +1182,7 @@ __device__ __forceinline__ void transpose_group(const float* __restrict__ in, fl
                 const int64_t o = (int64_t)c * out_pitch + r;
                 const float v = tile[buf][tx][ty + 4 * k];
                 if (far) nfar += v >= kFarTest;
-                outz[o] = signz ? apply_sign(v, signz[o]) : v;
+                outz[o] = signz ? apply_sign(v, sg[k]) : v;
             }
         }
     }
